@@ -1,0 +1,246 @@
+"""Benchmark: LSQR time-to-tol on the 67^3-grid / 2,000,000-ray system, one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+
+A step is one complete LSQR solve from x = 0 to test2 <= 1e-6 (test2 as
+lsqr.c:742-744) on the device-resident system; W untimed solves, then K timed
+solves between device synchronisations.  value = mean seconds per solve.
+
+Also measured inside the run, on the handle's stream with HIP events: the
+average duration of one aprod of each mode, priced against the algorithmic
+bytes of that launch (each stored value and index read once, each pointer
+array once, each dense input once, each dense output read and written once).
+The CPU baseline is the oracle (the C restatement of the reference, one
+thread) timed on a bounded sample of the same solve and scaled to the same
+iteration count.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "LSQR time-to-tol 1e-6 (s) + aprod HBM GB/s, mixed/learned vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "plsqr_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg = os.path.join(ROOT, "plsqr3.2013.09_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Hip:
+    """The few HIP runtime calls the benchmark needs (device buffers, events)."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        L = self.lib
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        L.hipEventSynchronize.argtypes = [C.c_void_p]
+        L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.hipMalloc(C.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self.lib.hipFree(C.c_void_p(p))
+
+    def h2d(self, dst, arr):
+        self.check(self.lib.hipMemcpy(C.c_void_p(dst), arr.ctypes.data_as(C.c_void_p), arr.nbytes, 1),
+                   "hipMemcpy")
+
+    def d2h(self, arr, src):
+        self.check(self.lib.hipMemcpy(arr.ctypes.data_as(C.c_void_p), C.c_void_p(src), arr.nbytes, 2),
+                   "hipMemcpy")
+
+    def sync(self):
+        self.check(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def event(self):
+        e = C.c_void_p()
+        self.check(self.lib.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, stream):
+        self.check(self.lib.hipEventRecord(e, C.c_void_p(stream)), "hipEventRecord")
+
+    def elapsed_ms(self, e0, e1):
+        self.check(self.lib.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float()
+        self.check(self.lib.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        return ms.value
+
+
+def aprod_bytes(m_k, m_d, n, nnz_k, nnz_d, s_v=8, s_x=8, s_y=8):
+    """Algorithmic bytes of one aprod call (mode 1, mode 2)."""
+    b1 = (nnz_k * (s_v + 4) + 4 * (m_k + 1) + s_x * n + 2 * s_y * m_k
+          + nnz_d * (s_v + 4) + 4 * (m_d + 1) + 2 * s_y * m_d)
+    b2 = (nnz_k * (s_v + 4) + 4 * (n + 1) + s_y * m_k
+          + nnz_d * (s_v + 4) + 4 * (n + 1) + s_y * m_d + 2 * s_x * n)
+    return b1, b2
+
+
+def time_aprod(hip, op, mode, x_dev, y_dev, reps):
+    stream = op.lib.plsqr_op_get_stream(op.h)
+    for _ in range(3):
+        op.aprod_device(mode, x_dev, y_dev)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(reps):
+        op.aprod_device(mode, x_dev, y_dev)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1) / reps
+    hip.lib.hipEventDestroy(e0)
+    hip.lib.hipEventDestroy(e1)
+    return ms
+
+
+def read_traffic(workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summaries."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(P, s, iters_to_tol, sample_iters):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    orc = oracle.Oracle(s)
+    t0 = time.perf_counter()
+    orc.lsqr(s.b, sample_iters, trace=False)
+    t = time.perf_counter() - t0
+    # the sample is the start-up (one A^T u, about half an iteration) plus
+    # sample_iters iterations; scale to the GPU solve's iteration count
+    per_it = t / (sample_iters + 0.5)
+    return {"value": per_it * (iters_to_tol + 0.5), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle LSQR, {sample_iters} iterations of the same system from x=0 "
+                      f"({t:.1f} s measured, {per_it:.3f} s/iteration), scaled to "
+                      f"{iters_to_tol} iterations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="grid67_rays2m")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iters", type=int, default=5000)
+    ap.add_argument("--aprod-reps", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(the row-sharded multi-GPU design is described but not run)", file=sys.stderr)
+        sys.exit(2)
+    P = load_pkg()
+    hip = Hip()
+    t0 = time.perf_counter()
+    s = P.generate_config(a.workload)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] generated {a.workload}: m_k={s.m_k} m_d={s.m_d} n={s.n} nnz_k={s.nnz_k} "
+          f"nnz_d={s.nnz_d} ({gen_s:.1f} s)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    op = P.Operator(s)
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] operator on device ({setup_s:.1f} s, {op.device_bytes / 2**30:.2f} GiB)",
+          file=sys.stderr, flush=True)
+    prec = P.PREC_FP64 if a.precision == "fp64" else P.PREC_MIXED
+    b_dev = hip.malloc(8 * s.m_k)
+    x_dev = hip.malloc(8 * s.n)
+    y_dev = hip.malloc(8 * (s.m_k + s.m_d))
+    hip.h2d(b_dev, np.ascontiguousarray(s.b))
+    hip.h2d(x_dev, np.random.default_rng(0).standard_normal(s.n))
+    hip.h2d(y_dev, np.random.default_rng(1).standard_normal(s.m_k + s.m_d) * 1e-3)
+
+    for _ in range(a.warmup):
+        op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec)
+    hip.sync()
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        infos.append(op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec))
+    hip.sync()
+    wall = time.perf_counter() - t0
+    iters = infos[-1]["itn"]
+    reached = infos[-1]["arnorm"] <= a.tol
+    ms_step = 1000.0 * wall / a.steps
+    print(f"[bench] {a.steps} solves: {ms_step:.2f} ms each, {iters} iterations, "
+          f"test2={infos[-1]['arnorm']:.3e} (device loop {infos[-1]['elapsed_ms']:.2f} ms)",
+          file=sys.stderr, flush=True)
+
+    # aprod kernels, timed with events on the handle's stream
+    ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
+    ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
+    b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
+    gbs1, gbs2 = b1 / ms1 / 1e6, b2 / ms2 / 1e6
+    gbs = (b1 + b2) / (ms1 + ms2) / 1e6
+    print(f"[bench] aprod mode1 {ms1:.3f} ms {gbs1:.0f} GB/s; mode2 {ms2:.3f} ms {gbs2:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    dom_mode, dom_ms, dom_b = (2, ms2, b2) if ms2 >= ms1 else (1, ms1, b1)
+    traffic = read_traffic(a.workload)
+    tr_val = None
+    if traffic and str(dom_mode) in traffic.get("bytes_per_launch", {}):
+        tr_val = traffic["bytes_per_launch"][str(dom_mode)]
+    roofline = {"bound": "hbm", "achieved": round(dom_b / dom_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(dom_b / dom_ms / 1e6 / HBM_PEAK_GBS, 4),
+                "traffic": tr_val,
+                "kernel": f"aprod mode {dom_mode} ("
+                          + ("k_rows: y += A x on the stacked CSR" if dom_mode == 1 else
+                             "k_cols_sliced + k_cols_combine: x += A^T y") + ")",
+                "algorithmic_bytes": dom_b, "avg_ms": round(dom_ms, 4)}
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(P, s, iters, a.cpu_iters)
+    out = {
+        "metric": METRIC, "value": round(wall / a.steps, 6) if reached else None, "unit": "s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32/f64",
+        "data": "synthetic (seeded straight-ray generator, inputs resident in HBM)",
+        "config": {"workload": a.workload, "grid": [67, 67, 67] if "67" in a.workload else None,
+                   "rays": s.m_k, "nnz_k": s.nnz_k, "m_d": s.m_d, "nnz_d": s.nnz_d,
+                   "precision": a.precision, "tol": a.tol, "iterations_to_tol": iters,
+                   "tol_reached": bool(reached), "parallelism": "single GPU"},
+        "ms_per_iteration": round(infos[-1]["elapsed_ms"] / max(iters, 1), 4),
+        "aprod_gbs": {"mode1": round(gbs1, 1), "mode2": round(gbs2, 1), "both": round(gbs, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_s": {"generate": round(gen_s, 2), "operator_create": round(setup_s, 2)},
+    }
+    print(json.dumps(out), flush=True)
+    op.close()
+    for p in (b_dev, x_dev, y_dev):
+        hip.free(p)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,278 @@
+/*
+ * plsqr_hip.h -- C-ABI of the MI355X (gfx950) LSQR hot path.
+ *
+ * The hot path of PLSQR3 is the LSQR bidiagonalisation loop (lsqr.c:195-917)
+ * and its operator callback aprod (aprod.c:24-84 -> aprod_impl.c:22-524),
+ * which computes  u += A v  (mode 1) and  v += A^T u  (mode 2) for
+ * A = [A_k ; D]: a column-sorted kernel matrix A_k (m_k x n) stacked on a
+ * damping matrix D (m_d x n).  This header is the drop-in boundary: plain C
+ * types, host or device pointers and sizes, no C++ or torch types.
+ *
+ * Conventions
+ *   - Matrices enter exactly as the reference loaders build them: 0-based
+ *     int32 pointers and indices, FP64 values (plsqr3.c:1515-1626 for the
+ *     kernel CSC, plsqr3.c:1133-1189 for the damping CSR and CSC).
+ *   - u = [y_k ; y_d] is one vector of m_k + m_d doubles, kernel rows first
+ *     (the split of PPDataV3_TRI, main.h:421-432).
+ *   - Functions returning int return PLSQR_OK (0) or a PLSQR_ERR_* code and
+ *     never call exit(); the message is kept on the handle
+ *     (plsqr_last_error) or, for calls without a handle, in thread-local
+ *     storage (plsqr_last_global_error).
+ *   - A handle is bound to one device and one HIP stream.  It is
+ *     thread-compatible, not thread-safe: one host thread per handle.
+ */
+#ifndef PLSQR_HIP_H_
+#define PLSQR_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes                                                            */
+/* ---------------------------------------------------------------------- */
+enum {
+    PLSQR_OK = 0,
+    PLSQR_ERR_ARG = 1,     /* invalid argument or size mismatch            */
+    PLSQR_ERR_BOUNDS = 2,  /* index out of range in matrix data            */
+    PLSQR_ERR_IO = 3,      /* file open/read/write failure                 */
+    PLSQR_ERR_HIP = 4,     /* HIP runtime failure (no device, launch, ...) */
+    PLSQR_ERR_NOMEM = 5,   /* host or device allocation failed             */
+    PLSQR_ERR_STATE = 6,   /* call not valid in the handle's current state */
+    PLSQR_ERR_FORMAT = 7,  /* malformed file contents                      */
+    PLSQR_ERR_NONFINITE = 8/* NaN or Inf in the input vectors or matrix    */
+};
+
+/* ---------------------------------------------------------------------- */
+/* Host-side system in the reference's loaded layout                      */
+/* ---------------------------------------------------------------------- */
+
+/* Kernel CSC + damping CSR + damping CSC-by-column + kernel right-hand side.
+ * Replaces the three SpMatCSC/SpMatCSR objects the reference driver builds
+ * before initPPDataV3_TRI (plsqr3.c:320-335, 420-529; main.h:123-165) and the
+ * b vector it reads (plsqr3.c:580-589).  Arrays owned by the library when the
+ * struct was filled by plsqr_gen_system / plsqr_read_reference_files (free
+ * with plsqr_system_free); arrays supplied by the caller stay the caller's. */
+typedef struct plsqr_system {
+    int m_k;            /* kernel rows  (-row_k)                               */
+    int m_d;            /* damping rows (-row_d)                               */
+    int n;              /* columns      (-col)                                 */
+    int owned;          /* 1 if the arrays below were allocated by the library */
+    long long nnz_k;
+    long long nnz_d;
+    int    *k_col_ptr;  /* n+1                                                 */
+    int    *k_row_idx;  /* nnz_k, 0-based kernel row, ascending per column     */
+    double *k_val;      /* nnz_k                                               */
+    int    *d_row_ptr;  /* m_d+1                                               */
+    int    *d_col_idx;  /* nnz_d, 0-based column, ascending per row            */
+    double *d_val;      /* nnz_d                                               */
+    int    *dc_col_ptr; /* n+1                                                 */
+    int    *dc_row_idx; /* nnz_d, 0-based damping row, ascending per column    */
+    double *dc_val;     /* nnz_d                                               */
+    double *b;          /* m_k: kernel right-hand side (damping rhs is zero)   */
+    double *x_true;     /* n: model used to synthesise b, or NULL              */
+} plsqr_system;
+
+void plsqr_system_free(plsqr_system *sys);
+
+/* ---------------------------------------------------------------------- */
+/* Synthetic straight-ray tomography generator                             */
+/* ---------------------------------------------------------------------- */
+/* Replaces the offline chain ker_ascii2bin -> Ker2PLSQR3 -> damping_binary.py
+ * (PLSQR3_tools/kernel_format/Ker2PLSQR3.c:358-415,595-606;
+ *  PLSQR3_tools/damping_format/damping_binary.py:84-289) for seeded
+ * synthetic systems.  Columns are iz*ny*nx + iy*nx + ix (damping_binary.py:54).
+ * Damping rows per cell, x fastest: identity*lambda, then the 1,-2,1 second
+ * difference along x (if ix<nx-2), y (if iy<ny-2), z (if iz<nz-2), each
+ * scaled by its weight; no cross terms.  All damping values are exact FP64
+ * in both the row and the column layout. */
+enum {
+    PLSQR_GEOM_OPPOSITE_FACES = 0, /* axis drawn per ray; endpoints on the two faces normal to it */
+    PLSQR_GEOM_TOP_SOURCES = 1     /* source on the z=nz face; receiver on one of the 5 other faces */
+};
+
+typedef struct plsqr_gen_params {
+    int nx, ny, nz;
+    long long n_rays;
+    int geometry;                 /* PLSQR_GEOM_*                                  */
+    unsigned long long seed;
+    double sigma;                 /* std of the Gaussian noise added to b          */
+    double lambda;                /* identity damping weight                       */
+    double w_xx, w_yy, w_zz;      /* second-difference weights                     */
+    int n_blobs;                  /* Gaussian blobs in x_true (5)                  */
+    double blob_width;            /* blob standard deviation in voxels (3)         */
+} plsqr_gen_params;
+
+int plsqr_gen_system(const plsqr_gen_params *p, plsqr_system *out);
+
+/* New right-hand side for an existing matrix: b = A_k x_true + N(0, sigma^2)
+ * with x_true drawn from p->seed, p->n_blobs, p->blob_width and noise from
+ * p->seed, p->sigma (p->nx*p->ny*p->nz must equal sys->n).  b and x_true must
+ * hold m_k and n doubles. */
+int plsqr_gen_rhs(const plsqr_system *sys, const plsqr_gen_params *p,
+                  double *b, double *x_true);
+
+/* Thread-local message of the last failing call that has no handle. */
+const char *plsqr_last_global_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Reference on-disk formats                                               */
+/* ---------------------------------------------------------------------- */
+/* Files, all written 1-based like the reference tools:
+ *   kernel  .bin  16-byte records {int32 row, int32 col, float64 val},
+ *                 column-major, rows ascending      (Ker2PLSQR3.c:595-606)
+ *   kernel  .info ASCII "col nnz start" per column; start is the 1-based
+ *                 record offset, 0 when nnz==0      (Ker2PLSQR3.c:505-511)
+ *   damping row data/info, col data/info            (damping_binary.py:97-101,523-551)
+ *   b       ASCII, one value per line ("%.17e")     (read at plsqr3.c:580-589)
+ * Any name may be NULL to skip that file when writing. */
+int plsqr_write_reference_files(const plsqr_system *sys, const char *dir,
+                                const char *ker_f, const char *ker_i,
+                                const char *damp_f, const char *damp_i,
+                                const char *damp_f_bycol, const char *damp_i_bycol,
+                                const char *b_k);
+
+/* Loads the same seven files the reference's CLI names (plsqr3.c:123-186:
+ * -dir -ker_f -ker_i -damp_f -damp_i -damp_f_bycol -damp_i_bycol -b_k
+ * -row_k -row_d -col) into a library-owned plsqr_system, with the reference
+ * loaders' bounds checks (plsqr3.c:1149-1153,1178-1182,1606-1615) reported as
+ * PLSQR_ERR_BOUNDS instead of exit(-1). */
+int plsqr_read_reference_files(const char *dir,
+                               const char *ker_f, const char *ker_i,
+                               const char *damp_f, const char *damp_i,
+                               const char *damp_f_bycol, const char *damp_i_bycol,
+                               const char *b_k, int row_k, int row_d, int col,
+                               plsqr_system *out);
+
+/* Raw float64[n] in global column order, the layout of mpiio_write_result
+ * (plsqr3.c:1986-2013) for the FP64 build. */
+int plsqr_write_x(const char *path, const double *x, int n);
+int plsqr_read_x(const char *path, double *x, int n);
+
+/* ---------------------------------------------------------------------- */
+/* Device operator                                                         */
+/* ---------------------------------------------------------------------- */
+typedef struct plsqr_op plsqr_op;
+
+enum {
+    PLSQR_OP_DEFAULT = 0,      /* FP64 operator, wavefront-parallel deterministic sums    */
+    PLSQR_OP_EXACT_ORDER = 1,  /* FP64, every row/column summed sequentially in the
+                                  reference's order (bit-identical aprod)                */
+    PLSQR_OP_WITH_FP32 = 2     /* also keep FP32 copies of the values (mixed solves)     */
+};
+
+/* Uploads the system to the current HIP device, re-lays the kernel CSC as
+ * CSR on the device (stable counting sort, bit-exact triplets) and allocates
+ * the resident u (m_k+m_d) and v (n) vectors, both zero like
+ * initPPDataV3_TRI (tridiagonal.c:33-57).  Replaces the GPU set-up the
+ * reference calls InitPPDataV3_TRI_GPU (lsqr.c:451-457). */
+int  plsqr_op_create(const plsqr_system *sys, int flags, plsqr_op **out);
+void plsqr_op_destroy(plsqr_op *op);
+const char *plsqr_last_error(const plsqr_op *op);
+void plsqr_clear_error(plsqr_op *op);
+int  plsqr_op_dims(const plsqr_op *op, int *m_k, int *m_d, int *n,
+                   long long *nnz_k, long long *nnz_d);
+/* Device memory held by the handle, in bytes. */
+size_t plsqr_op_device_bytes(const plsqr_op *op);
+/* Attach a HIP stream (hipStream_t as void*; NULL = the default stream). */
+int  plsqr_op_set_stream(plsqr_op *op, void *stream);
+/* The stream the handle enqueues on (hipStream_t as void*), e.g. for events. */
+void *plsqr_op_get_stream(const plsqr_op *op);
+
+/* Resident vectors used when aprod is called with NULL x/y (PLSQR3's
+ * convention, lsqr.c:366,507,564).  NULL arguments are skipped. */
+int plsqr_op_set_vectors(plsqr_op *op, const double *u_host, const double *v_host);
+int plsqr_op_get_vectors(plsqr_op *op, double *u_host, double *v_host);
+
+/* ---------------------------------------------------------------------- */
+/* aprod: the callback boundary                                            */
+/* ---------------------------------------------------------------------- */
+/* Exact signature of the reference callback (lsqr.h:38-40, aprod.h:10-13),
+ * with MY_FLOAT_TYPE = double.  UsrWrk is a plsqr_op*.
+ *   mode 1:  y <- y + A x     mode 2:  x <- x + A^T y
+ * x (n) and y (m = m_k + m_d) are host pointers; when NULL the handle's
+ * resident v and u are used instead.  version, procRank, numProc,
+ * isFinalIteration and iteration are accepted and ignored: scratch memory
+ * belongs to the handle, so there is no final-iteration free
+ * (aprod_impl.c:237-261,501-523) and a second solve is safe.  m and n are
+ * checked against the handle; errors are recorded on the handle. */
+void plsqr_aprod_ref(int version, int procRank, int numProc, int mode,
+                     int m, int n, double x[], double y[], void *UsrWrk,
+                     int isFinalIteration, int iteration);
+
+/* Classic Paige-Saunders callback shape (clsqr2/lsqr_s.h:15-16,
+ * clsqr2/aprod_s.h:12-13): host x and y, both required. */
+void plsqr_aprod(int mode, int m, int n, double x[], double y[], void *UsrWrk);
+
+/* PLSQR3's split u: x = vec_x (n), y_k = vec_y_k (m_k), y_d = vec_y_d (m_d),
+ * host pointers (main.h:421-432).  This is the call a PLSQR3 aprod.c body
+ * maps onto. */
+int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_d);
+
+/* Device pointers (x: n doubles, y: m_k+m_d doubles), enqueued on the
+ * handle's stream; returns without synchronising. */
+int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident LSQR                                                    */
+/* ---------------------------------------------------------------------- */
+enum {
+    PLSQR_PREC_FP64 = 0,   /* plain FP64, the reference's arithmetic order          */
+    PLSQR_PREC_MIXED = 1   /* FP32 values and u, v with compensated accumulation,
+                              FP64 scalars and an FP64 residual refinement          */
+};
+
+typedef struct plsqr_solve_opts {
+    int    itn;            /* iteration count (-itn); the loop runs exactly itn
+                              iterations when tol == 0, like lsqr.c:833-834      */
+    double tol;            /* > 0: stop at the first iteration with
+                              test2 <= tol (test2 as lsqr.c:742-744)            */
+    int    precision;      /* PLSQR_PREC_*                                       */
+    int    refine_every;   /* mixed only: inner iterations per FP64 refinement   */
+    double damp;           /* LSQR damp parameter (the reference passes 0.0)     */
+    double atol, btol, conlim; /* istop tests (lsqr.c:759-775); reference values
+                              1e-10, 1e-10, 100 when all three are 0            */
+    int    itnlim;         /* istop=5 threshold; 1000 (lsqr.c:50) when 0          */
+} plsqr_solve_opts;
+
+typedef struct plsqr_solve_out {
+    int    istop, itn;     /* istop as computed (the reference never acts on it) */
+    double anorm, acond, rnorm, arnorm, xnorm, bnorm;
+                           /* arnorm holds test2, as lsqr.c:913 assigns         */
+    double arnorm_true;    /* alpha*|tau| of the last iteration                  */
+    double elapsed_ms;     /* device time of the iteration loop                  */
+    int    refinements;    /* mixed: FP64 residual refinements performed         */
+    int    inner_total;    /* mixed: total inner iterations                      */
+} plsqr_solve_out;
+
+/* Per-iteration trace row: alpha, beta, rnorm, test1, test2, anorm, acond, xnorm */
+#define PLSQR_TRACE_WIDTH 8
+
+/* Solve min ||A x - [b;0]|| from x = 0 (lsqr.c:311-312).  b_host: m_k
+ * doubles.  x_host (n) receives x; w_host (n) may be NULL.  trace may be
+ * NULL or hold (itn+1) * PLSQR_TRACE_WIDTH doubles (row 0 = initial state,
+ * lsqr.c:425-445).  The resident u and v are overwritten. */
+int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts *opts,
+                     double *x_host, double *w_host, plsqr_solve_out *out,
+                     double *trace);
+
+/* Same on device pointers (b_dev: m_k, x_dev and w_dev: n; w_dev may be
+ * NULL), enqueued on the handle's stream; the call synchronises that
+ * stream before it returns so that *out is complete. */
+int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *opts,
+                            double *x_dev, double *w_dev, plsqr_solve_out *out,
+                            double *trace);
+
+/* Residual and normal-equation residual in FP64 on the device:
+ * r = [b;0] - A x,  rnorm = ||r||,  arnorm = ||A^T r||,  anorm_f = ||A||_F.
+ * Used to confirm a test2 reading with ||A^T r|| / (||A||_F ||r||). */
+int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_host,
+                         double *rnorm, double *arnorm, double *anorm_f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLSQR_HIP_H_ */

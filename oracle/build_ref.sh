@@ -1,0 +1,43 @@
+#!/bin/bash
+# Compiles the reference PLSQR3 (the C sources under /root/reference/source,
+# read in place, never copied into the repository) as its FP64 configuration
+# into oracle/_ref/plsqr3_ref_fp64.  Test infrastructure: the binary writes the
+# golden fixtures under tests/golden/ and can be timed as the CPU baseline.
+#
+# main.h:26 hard-codes "#define SDVERSION 1" (float); the double build is the
+# same header with SDVERSION 2 (main.h:27-35).  The recipe writes that one-line
+# variant of main.h to a temporary directory, compiles the reference sources
+# with the current directory dropped from the quote-include search (-I-) so the
+# variant is the main.h they see, and deletes the temporary directory.
+#
+# Needs MPICH (the reference's MPI; /opt/conda ships MPICH 3.3.2 with mpicc).
+set -euo pipefail
+REF=${PLSQR_REF_SRC:-/root/reference/source}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+MPICC=${MPICC:-/opt/conda/bin/mpicc}
+if [ ! -f "$REF/lsqr.c" ]; then
+    echo "reference sources not found at $REF; nothing built" >&2
+    exit 0
+fi
+if [ ! -x "$MPICC" ]; then
+    echo "mpicc not found at $MPICC; reference not built" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+TMP=$(mktemp -d)
+trap 'rm -rf "$TMP"' EXIT
+sed 's/^#define SDVERSION 1 /#define SDVERSION 2 /' "$REF/main.h" > "$TMP/main.h"
+grep -q '^#define SDVERSION 2 ' "$TMP/main.h"
+export MPICH_CC=gcc
+FLAGS="-O3 -w"
+for f in main lsqr mycblas FileIO aprod loadbalance plsqr3 CPUSpMV tridiagonal vecops \
+         aprod_impl vector_comm sparse_vec; do
+    "$MPICC" $FLAGS -I"$TMP" -I"$REF" -I"$REF/tool" -I- -c "$REF/$f.c" -o "$TMP/$f.o" 2>/dev/null
+done
+for f in gptl threadutil GPTLutil; do
+    "$MPICC" $FLAGS -DHAVE_MPI -DHAVE_GETTIMEOFDAY -I"$REF/tool" -I- -c "$REF/tool/$f.c" \
+        -o "$TMP/$f.o" 2>/dev/null
+done
+"$MPICC" -o "$OUT/plsqr3_ref_fp64" "$TMP"/*.o -lm
+echo "built $OUT/plsqr3_ref_fp64"

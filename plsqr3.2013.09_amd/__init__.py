@@ -1,0 +1,154 @@
+"""MI355X-native LSQR hot path of PLSQR3 (host-side Python mirror).
+
+The product is libplsqr_hip.so (C-ABI in include/plsqr_hip.h); this package
+loads it with ctypes and mirrors the reference's operator interface:
+
+    Operator.aprod(mode, x, y)   <- aprod(version, procRank, numProc, mode, m, n,
+                                          x, y, UsrWrk, isFinalIteration, iteration)
+                                    (source/aprod.h:10-13, source/lsqr.h:38-40)
+    Operator.lsqr(b, itn, ...)   <- call_lsqr / lsqr (source/lsqr.c:34-184, 195-917)
+
+Every call runs on the GPU through the library; when the library is missing,
+load() raises instead of falling back to anything else.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (  # noqa: F401
+    CONFIGS, ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT,
+    OP_EXACT_ORDER, OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH,
+    PlsqrError, SolveOptsC, SolveOutC, System, dptr, gen_rhs, generate, generate_config, load,
+    read_x, write_x, _check,
+)
+
+
+class Operator:
+    """Device-resident operator A = [A_k ; D] of one system (plsqr_op)."""
+
+    def __init__(self, system, flags=OP_DEFAULT):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.plsqr_op_create(C.byref(system.c), int(flags), C.byref(h))
+        if rc != 0:
+            msg = lib.plsqr_last_error(h) if h.value else lib.plsqr_last_global_error()
+            if h.value:
+                lib.plsqr_op_destroy(h)
+            raise PlsqrError(rc, (msg or b"").decode(errors="replace"))
+        self.h = h
+        self.lib = lib
+        self.m_k, self.m_d, self.n = system.m_k, system.m_d, system.n
+        self.m = self.m_k + self.m_d
+        self.nnz_k, self.nnz_d = system.nnz_k, system.nnz_d
+        self.flags = flags
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.plsqr_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.plsqr_op_device_bytes(self.h))
+
+    def last_error(self):
+        return (self.lib.plsqr_last_error(self.h) or b"").decode(errors="replace")
+
+    def _raise_if_error(self):
+        msg = self.last_error()
+        if msg:
+            self.lib.plsqr_clear_error(self.h)
+            raise PlsqrError(1, msg)
+
+    # -- the callback boundary ------------------------------------------
+    def aprod(self, mode, x, y):
+        """mode 1: y += A x ; mode 2: x += A^T y   (in place, host float64 arrays)."""
+        assert x.dtype == np.float64 and y.dtype == np.float64
+        self.lib.plsqr_aprod_ref(0, 0, 1, int(mode), self.m, self.n, dptr(x), dptr(y),
+                                 self.h, 0, 0)
+        self._raise_if_error()
+
+    def aprod_classic(self, mode, x, y):
+        self.lib.plsqr_aprod(int(mode), self.m, self.n, dptr(x), dptr(y), self.h)
+        self._raise_if_error()
+
+    def aprod_split(self, mode, x, y_k, y_d):
+        _check(self.lib.plsqr_aprod_split(self.h, int(mode), dptr(x), dptr(y_k), dptr(y_d)), self.h)
+
+    def aprod_device(self, mode, x_ptr, y_ptr):
+        """Device pointers (ints), enqueued on the handle's stream."""
+        _check(self.lib.plsqr_aprod_device(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
+               self.h)
+
+    def set_vectors(self, u=None, v=None):
+        _check(self.lib.plsqr_op_set_vectors(self.h, dptr(u) if u is not None else None,
+                                             dptr(v) if v is not None else None), self.h)
+
+    def get_vectors(self):
+        u = np.empty(self.m, np.float64)
+        v = np.empty(self.n, np.float64)
+        _check(self.lib.plsqr_op_get_vectors(self.h, dptr(u), dptr(v)), self.h)
+        return u, v
+
+    def aprod_resident(self, mode):
+        """aprod with NULL x/y: acts on the handle's resident u and v (PLSQR3 convention)."""
+        self.lib.plsqr_aprod_ref(0, 0, 1, int(mode), self.m, self.n, None, None, self.h, 0, 0)
+        self._raise_if_error()
+
+    # -- LSQR -------------------------------------------------------------
+    @staticmethod
+    def make_opts(itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+        o = SolveOptsC()
+        o.itn, o.tol, o.precision, o.refine_every, o.damp = int(itn), float(tol), int(precision), \
+            int(refine_every), float(damp)
+        o.atol = o.btol = o.conlim = 0.0
+        o.itnlim = 0
+        return o
+
+    def lsqr(self, b, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0,
+             trace=False, want_w=False):
+        """Returns (x, info dict[, trace array (itn+1, 8)])."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.size == self.m_k
+        x = np.empty(self.n, np.float64)
+        w = np.empty(self.n, np.float64) if want_w else None
+        out = SolveOutC()
+        tr = np.full((itn + 1, TRACE_WIDTH), np.nan) if trace else None
+        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        _check(self.lib.plsqr_lsqr_solve(self.h, dptr(b), C.byref(o), dptr(x),
+                                         dptr(w) if w is not None else None, C.byref(out),
+                                         tr.ctypes.data_as(C.POINTER(C.c_double))
+                                         if tr is not None else None), self.h)
+        info = out.as_dict()
+        if want_w:
+            info["w"] = w
+        return (x, info, tr) if trace else (x, info)
+
+    def lsqr_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
+                    damp=0.0, w_ptr=None):
+        out = SolveOutC()
+        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        _check(self.lib.plsqr_lsqr_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
+                                                C.c_void_p(x_ptr),
+                                                C.c_void_p(w_ptr) if w_ptr else None,
+                                                C.byref(out), None), self.h)
+        return out.as_dict()
+
+    def residual_norms(self, b, x):
+        rn, an, af = C.c_double(), C.c_double(), C.c_double()
+        _check(self.lib.plsqr_residual_norms(self.h, dptr(np.ascontiguousarray(b, np.float64)),
+                                             dptr(np.ascontiguousarray(x, np.float64)),
+                                             C.byref(rn), C.byref(an), C.byref(af)), self.h)
+        return rn.value, an.value, af.value

@@ -1,0 +1,69 @@
+"""Build recipe for libplsqr_hip.so (gfx950) and the host CLI.
+
+    python plsqr3.2013.09_amd/build.py          # library + CLI
+Compiles in-tree so the built files travel with the source tree.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(HERE, "libplsqr_hip.so")
+CLI = os.path.join(HERE, "plsqr3_hip")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PLSQR_ARCH", "gfx950")
+
+# -ffp-contract=off: products and sums are rounded separately, as in the
+# reference's x86-64 build; explicit fma() calls are unaffected.
+# -packed-fp32-ops off: no v_pk_*_f32 generation.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", INCLUDE, "-I", CSRC]
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib(out=LIB):
+    tmp = out + ".tmp"
+    objs = []
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    for src, extra in (("plsqr_api.hip", DEVICE), ("plsqr_mixed.hip", DEVICE),
+                       ("tomo_io.cpp", [])):
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(obj_dir, src + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "c++"]
+        run([HIPCC] + COMMON + extra + lang + ["-c", path, "-o", obj])
+        objs.append(obj)
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli(out=CLI):
+    src = os.path.join(CSRC, "plsqr3_hip_main.c")
+    if not os.path.exists(src):
+        return None
+    cc = shutil.which("gcc") or "cc"
+    run([cc, "-O2", "-std=c99", "-Wall", "-I", INCLUDE, src, "-o", out,
+         "-L", HERE, "-lplsqr_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
+    return out
+
+
+def main():
+    build_lib()
+    build_cli()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,896 @@
+// C-ABI of the LSQR hot path: operator handle, aprod entry points and the
+// device-resident LSQR loop (include/plsqr_hip.h).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "plsqr_hip.h"
+#include "plsqr_kernels.h"
+#include "plsqr_spmv.h"
+
+using plsqr::Scal;
+
+struct plsqr_op {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int m_k = 0, m_d = 0, n = 0, m = 0;
+    long long nnz_k = 0, nnz_d = 0;
+    int flags = 0;
+    bool exact = false;
+    // stacked CSR of [A_k ; D]: kernel rows re-laid on the device, then damping rows
+    int *rp = nullptr, *ci = nullptr;
+    double *rv = nullptr;
+    // kernel CSC as loaded (kept for the exact-order kernels)
+    int *kcp = nullptr, *kri = nullptr;
+    double *kv = nullptr;
+    // kernel CSC split into row slices (one per XCD)
+    int nsl = 1;
+    int *scp = nullptr, *sri = nullptr;
+    double *sv = nullptr;
+    int *bounds = nullptr;
+    // damping CSC by column
+    int *dcp = nullptr, *dri = nullptr;
+    double *dcv = nullptr;
+    // vectors
+    double *u = nullptr, *v = nullptr, *x = nullptr, *w = nullptr, *se = nullptr;
+    double *bst = nullptr;  // b staging (m_k)
+    double *P = nullptr;    // nsl * n slice partials
+    double *part = nullptr;
+    int part_len = 0;
+    Scal *sc = nullptr;
+    double *trace_d = nullptr;
+    int trace_rows = 0;
+    double *hx = nullptr, *hy = nullptr;  // staging for host-pointer aprod (n, m)
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    // launch shapes
+    int cu = 256;
+    int L1 = 16, L2 = 16, G1 = 0, G2 = 0, G4 = 0, G6 = 0, Gs = 0;
+    std::string err;
+};
+
+namespace {
+
+int set_err(plsqr_op *op, int code, const std::string &msg) {
+    if (op && op->err.empty()) op->err = msg;
+    return code;
+}
+
+#define HIP_TRY(op, expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err(op, PLSQR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(op) HIP_TRY(op, hipGetLastError())
+
+template <class T>
+int dalloc(plsqr_op *op, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(op, PLSQR_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
+                                                " bytes failed: " + hipGetErrorString(e));
+    }
+    op->allocs.push_back((void *)*p);
+    op->bytes += sizeof(T) * count;
+    return PLSQR_OK;
+}
+
+template <class T>
+void dfree(plsqr_op *op, T *&p, size_t count) {
+    if (!p) return;
+    auto it = std::find(op->allocs.begin(), op->allocs.end(), (void *)p);
+    if (it != op->allocs.end()) {
+        op->allocs.erase(it);
+        op->bytes -= sizeof(T) * std::max<size_t>(count, 1);
+        (void)hipFree(p);
+    }
+    p = nullptr;
+}
+
+int grid_for(long long work, int per_block, int cap) {
+    long long g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// -------------------------------------------------------------------------
+// kernel dispatch over the lanes-per-row template parameter
+// -------------------------------------------------------------------------
+constexpr int BS = 256;
+
+int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double *part) {
+    using namespace plsqr;
+    dim3 g(op->G1), b(BS);
+    switch (op->L1) {
+        case 2: hipLaunchKernelGGL((k_rows<2, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        case 4: hipLaunchKernelGGL((k_rows<4, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        case 8: hipLaunchKernelGGL((k_rows<8, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        case 16: hipLaunchKernelGGL((k_rows<16, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        case 32: hipLaunchKernelGGL((k_rows<32, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        case 64: hipLaunchKernelGGL((k_rows<64, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        default: return set_err(op, PLSQR_ERR_ARG, "unsupported lanes per row");
+    }
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
+    using namespace plsqr;
+    dim3 g(op->G2), b(BS);
+    switch (op->L2) {
+        case 2: hipLaunchKernelGGL((k_cols_sliced<2, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        case 4: hipLaunchKernelGGL((k_cols_sliced<4, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        case 8: hipLaunchKernelGGL((k_cols_sliced<8, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        case 16: hipLaunchKernelGGL((k_cols_sliced<16, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        case 32: hipLaunchKernelGGL((k_cols_sliced<32, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        case 64: hipLaunchKernelGGL((k_cols_sliced<64, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
+        default: return set_err(op, PLSQR_ERR_ARG, "unsupported lanes per column");
+    }
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->nsl,
+                       op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+// y += A x  (mode 1) on device pointers
+int aprod1_dev(plsqr_op *op, const double *x, double *y) {
+    if (op->exact) {
+        hipLaunchKernelGGL(plsqr::k_rows_exact, dim3(grid_for(op->m, BS, op->cu * 8)), dim3(BS), 0, op->stream,
+                           op->m, op->rp, op->ci, op->rv, x, y, (const Scal *)nullptr);
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
+    return launch_rows(op, x, y, nullptr, nullptr);
+}
+
+// x += A^T y  (mode 2) on device pointers
+int aprod2_dev(plsqr_op *op, double *x, const double *y) {
+    if (op->exact) {
+        hipLaunchKernelGGL(plsqr::k_cols_exact, dim3(grid_for(op->n, BS, op->cu * 8)), dim3(BS), 0, op->stream,
+                           op->n, op->m_k, op->kcp, op->kri, op->kv, op->dcp, op->dri, op->dcv, y,
+                           (const Scal *)nullptr, x);
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
+    return launch_cols(op, y, x, nullptr, nullptr);
+}
+
+// -------------------------------------------------------------------------
+// host-side validation of the loaded arrays (the reference loaders' checks,
+// plsqr3.c:1149-1153, 1178-1182, 1606-1615, as error codes)
+// -------------------------------------------------------------------------
+int check_ptr(plsqr_op *op, const char *what, const int *p, int len, long long nnz) {
+    if (!p) return set_err(op, PLSQR_ERR_ARG, std::string(what) + " is NULL");
+    if (p[0] != 0) return set_err(op, PLSQR_ERR_FORMAT, std::string(what) + "[0] != 0");
+    for (int i = 0; i < len; ++i)
+        if (p[i + 1] < p[i]) return set_err(op, PLSQR_ERR_FORMAT, std::string(what) + " decreases at " + std::to_string(i));
+    if (p[len] != nnz) return set_err(op, PLSQR_ERR_FORMAT, std::string(what) + " does not end at nnz");
+    return PLSQR_OK;
+}
+
+int check_idx(plsqr_op *op, const char *what, const int *idx, const double *val, long long nnz, int bound) {
+    if (nnz > 0 && (!idx || !val)) return set_err(op, PLSQR_ERR_ARG, std::string(what) + " arrays are NULL");
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nnz < (1 << 20)) nt = 1;
+    std::vector<long long> bad_idx(nt, -1), bad_val(nt, -1);
+    std::vector<std::thread> th;
+    const long long chunk = (nnz + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t] {
+            const long long b = t * chunk, e = std::min(nnz, b + chunk);
+            for (long long i = b; i < e; ++i) {
+                if (idx[i] < 0 || idx[i] >= bound) {
+                    bad_idx[t] = i;
+                    return;
+                }
+                if (!std::isfinite(val[i])) {
+                    bad_val[t] = i;
+                    return;
+                }
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (unsigned t = 0; t < nt; ++t) {
+        if (bad_idx[t] >= 0)
+            return set_err(op, PLSQR_ERR_BOUNDS, std::string(what) + " index out of range at entry " + std::to_string(bad_idx[t]));
+        if (bad_val[t] >= 0)
+            return set_err(op, PLSQR_ERR_NONFINITE, std::string(what) + " value not finite at entry " + std::to_string(bad_val[t]));
+    }
+    return PLSQR_OK;
+}
+
+int bits_for(long long v) {
+    int b = 1;
+    while ((1LL << b) <= v) ++b;
+    return b;
+}
+
+// stable sort of entry indices by key (LSD radix sort: a sequence of stable
+// counting sorts), returns the permutation in *perm_out (device, len ints)
+int stable_order(plsqr_op *op, const int *keys, long long len, int bits, int **perm_out) {
+    int *kin = nullptr, *kout = nullptr, *vin = nullptr, *vout = nullptr;
+    int rc;
+    if ((rc = dalloc(op, &kin, len)) || (rc = dalloc(op, &kout, len)) || (rc = dalloc(op, &vin, len)) ||
+        (rc = dalloc(op, &vout, len)))
+        return rc;
+    HIP_TRY(op, hipMemcpyAsync(kin, keys, sizeof(int) * len, hipMemcpyDeviceToDevice, op->stream));
+    hipLaunchKernelGGL(plsqr::k_iota, dim3(grid_for(len, 256, 4096)), dim3(256), 0, op->stream, len, vin);
+    LAUNCH_CHECK(op);
+    size_t tmp_bytes = 0;
+    HIP_TRY(op, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, vin, vout, (int)len, 0, bits,
+                                                   op->stream));
+    void *tmp = nullptr;
+    if ((rc = dalloc(op, (char **)&tmp, tmp_bytes))) return rc;
+    HIP_TRY(op, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)len, 0, bits,
+                                                   op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    char *tmpc = (char *)tmp;
+    dfree(op, tmpc, tmp_bytes);
+    dfree(op, kin, len);
+    dfree(op, kout, len);
+    dfree(op, vin, len);
+    *perm_out = vout;
+    return PLSQR_OK;
+}
+
+// exclusive prefix sum of counts (len+1 entries, last count 0) into out
+int exclusive_scan(plsqr_op *op, const int *cnt, int *out, long long len) {
+    size_t tmp_bytes = 0;
+    HIP_TRY(op, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, out, (int)len, op->stream));
+    char *tmp = nullptr;
+    int rc;
+    if ((rc = dalloc(op, &tmp, tmp_bytes))) return rc;
+    HIP_TRY(op, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, out, (int)len, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    dfree(op, tmp, tmp_bytes);
+    return PLSQR_OK;
+}
+
+int build(plsqr_op *op, const plsqr_system *s) {
+    using namespace plsqr;
+    int rc;
+    const long long nk = s->nnz_k, nd = s->nnz_d;
+    const int gcap = op->cu * 16;
+    // ---- kernel CSC upload
+    if ((rc = dalloc(op, &op->kcp, (size_t)op->n + 1)) || (rc = dalloc(op, &op->kri, nk)) ||
+        (rc = dalloc(op, &op->kv, nk)))
+        return rc;
+    HIP_TRY(op, hipMemcpyAsync(op->kcp, s->k_col_ptr, sizeof(int) * (op->n + 1), hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(op->kri, s->k_row_idx, sizeof(int) * nk, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(op->kv, s->k_val, sizeof(double) * nk, hipMemcpyHostToDevice, op->stream));
+    // ---- column index of every CSC entry
+    int *col = nullptr;
+    if ((rc = dalloc(op, &col, nk))) return rc;
+    hipLaunchKernelGGL(k_expand_ptr, dim3(grid_for(op->n, 256, gcap)), dim3(256), 0, op->stream, op->n, op->kcp, col);
+    LAUNCH_CHECK(op);
+    // ---- CSR of A_k: stable sort of the column-major entries by row
+    int *perm = nullptr;
+    if ((rc = stable_order(op, op->kri, nk, bits_for(op->m_k), &perm))) return rc;
+    const long long nr = nk + nd;
+    if ((rc = dalloc(op, &op->rp, (size_t)op->m + 1)) || (rc = dalloc(op, &op->ci, nr)) || (rc = dalloc(op, &op->rv, nr)))
+        return rc;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(nk, 256, gcap)), dim3(256), 0, op->stream, nk, perm, col, op->kv,
+                       op->ci, op->rv);
+    LAUNCH_CHECK(op);
+    dfree(op, perm, nk);
+    {
+        int *cnt = nullptr;
+        if ((rc = dalloc(op, &cnt, (size_t)op->m_k + 1))) return rc;
+        HIP_TRY(op, hipMemsetAsync(cnt, 0, sizeof(int) * (op->m_k + 1), op->stream));
+        hipLaunchKernelGGL(k_count, dim3(grid_for(nk, 256, gcap)), dim3(256), 0, op->stream, nk, op->kri, cnt);
+        LAUNCH_CHECK(op);
+        if ((rc = exclusive_scan(op, cnt, op->rp, (long long)op->m_k + 1))) return rc;
+        dfree(op, cnt, (size_t)op->m_k + 1);
+    }
+    // ---- damping rows appended
+    {
+        int *drp = nullptr, *dci = nullptr;
+        double *dav = nullptr;
+        if ((rc = dalloc(op, &drp, (size_t)op->m_d + 1)) || (rc = dalloc(op, &dci, nd)) || (rc = dalloc(op, &dav, nd)))
+            return rc;
+        HIP_TRY(op, hipMemcpyAsync(drp, s->d_row_ptr, sizeof(int) * (op->m_d + 1), hipMemcpyHostToDevice, op->stream));
+        if (nd > 0) {
+            HIP_TRY(op, hipMemcpyAsync(dci, s->d_col_idx, sizeof(int) * nd, hipMemcpyHostToDevice, op->stream));
+            HIP_TRY(op, hipMemcpyAsync(dav, s->d_val, sizeof(double) * nd, hipMemcpyHostToDevice, op->stream));
+        }
+        hipLaunchKernelGGL(k_append_rows, dim3(grid_for(std::max<long long>(nd, op->m_d + 1), 256, gcap)), dim3(256),
+                           0, op->stream, op->m_k, op->m_d, nk, nd, drp, dci, dav, op->rp, op->ci, op->rv);
+        LAUNCH_CHECK(op);
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        dfree(op, drp, (size_t)op->m_d + 1);
+        dfree(op, dci, nd);
+        dfree(op, dav, nd);
+    }
+    // ---- damping CSC
+    if ((rc = dalloc(op, &op->dcp, (size_t)op->n + 1)) || (rc = dalloc(op, &op->dri, nd)) || (rc = dalloc(op, &op->dcv, nd)))
+        return rc;
+    HIP_TRY(op, hipMemcpyAsync(op->dcp, s->dc_col_ptr, sizeof(int) * (op->n + 1), hipMemcpyHostToDevice, op->stream));
+    if (nd > 0) {
+        HIP_TRY(op, hipMemcpyAsync(op->dri, s->dc_row_idx, sizeof(int) * nd, hipMemcpyHostToDevice, op->stream));
+        HIP_TRY(op, hipMemcpyAsync(op->dcv, s->dc_val, sizeof(double) * nd, hipMemcpyHostToDevice, op->stream));
+    }
+    // ---- row slices of the kernel CSC, balanced by nonzeros
+    int nsl = env_int("PLSQR_SLICES", 8);
+    if (nsl < 1) nsl = 1;
+    if (nsl > 64) nsl = 64;
+    if (op->m_k < nsl * 64) nsl = 1;
+    op->nsl = nsl;
+    std::vector<int> hb(nsl + 1, 0);
+    {
+        std::vector<int> hrp((size_t)op->m_k + 1);
+        HIP_TRY(op, hipMemcpy(hrp.data(), op->rp, sizeof(int) * (op->m_k + 1), hipMemcpyDeviceToHost));
+        for (int sl = 1; sl < nsl; ++sl) {
+            const long long target = (long long)nk * sl / nsl;
+            hb[sl] = (int)(std::lower_bound(hrp.begin(), hrp.end(), (int)target) - hrp.begin());
+            hb[sl] = std::max(hb[sl], hb[sl - 1]);
+        }
+        hb[nsl] = op->m_k;
+    }
+    if ((rc = dalloc(op, &op->bounds, (size_t)nsl + 1))) return rc;
+    HIP_TRY(op, hipMemcpy(op->bounds, hb.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice));
+    if (nsl == 1) {
+        op->scp = op->kcp;
+        op->sri = op->kri;
+        op->sv = op->kv;
+    } else {
+        const long long nkey = (long long)nsl * op->n;
+        int *key = nullptr, *cnt = nullptr, *off = nullptr;
+        if ((rc = dalloc(op, &key, nk)) || (rc = dalloc(op, &cnt, nkey + 1)) || (rc = dalloc(op, &off, nkey + 1)))
+            return rc;
+        hipLaunchKernelGGL(k_slice_key, dim3(grid_for(nk, 256, gcap)), dim3(256), 0, op->stream, nk, op->n, nsl,
+                           op->bounds, op->kri, col, key);
+        LAUNCH_CHECK(op);
+        HIP_TRY(op, hipMemsetAsync(cnt, 0, sizeof(int) * (nkey + 1), op->stream));
+        hipLaunchKernelGGL(k_count, dim3(grid_for(nk, 256, gcap)), dim3(256), 0, op->stream, nk, key, cnt);
+        LAUNCH_CHECK(op);
+        if ((rc = exclusive_scan(op, cnt, off, nkey + 1))) return rc;
+        if ((rc = dalloc(op, &op->scp, (size_t)nsl * (op->n + 1)))) return rc;
+        hipLaunchKernelGGL(k_slice_ptr, dim3(grid_for((long long)nsl * (op->n + 1), 256, gcap)), dim3(256), 0,
+                           op->stream, op->n, nsl, off, nk, op->scp);
+        LAUNCH_CHECK(op);
+        int *perm2 = nullptr;
+        if ((rc = stable_order(op, key, nk, bits_for(nkey), &perm2))) return rc;
+        if ((rc = dalloc(op, &op->sri, nk)) || (rc = dalloc(op, &op->sv, nk))) return rc;
+        hipLaunchKernelGGL(k_permute, dim3(grid_for(nk, 256, gcap)), dim3(256), 0, op->stream, nk, perm2, op->kri,
+                           op->kv, op->sri, op->sv);
+        LAUNCH_CHECK(op);
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        dfree(op, perm2, nk);
+        dfree(op, key, nk);
+        dfree(op, cnt, nkey + 1);
+        dfree(op, off, nkey + 1);
+        if (!op->exact) {  // the sliced copy serves mode 2; the loaded CSC is no longer needed
+            dfree(op, op->kcp, (size_t)op->n + 1);
+            dfree(op, op->kri, nk);
+            dfree(op, op->kv, nk);
+        }
+    }
+    dfree(op, col, nk);
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+int ensure_trace(plsqr_op *op, int rows) {
+    if (rows <= op->trace_rows) return PLSQR_OK;
+    dfree(op, op->trace_d, (size_t)op->trace_rows * 8);
+    op->trace_rows = 0;
+    int rc = dalloc(op, &op->trace_d, (size_t)rows * 8);
+    if (rc) return rc;
+    op->trace_rows = rows;
+    return PLSQR_OK;
+}
+
+void default_opts(const plsqr_solve_opts *o, plsqr_solve_opts *d) {
+    *d = *o;
+    if (d->atol == 0.0 && d->btol == 0.0 && d->conlim == 0.0) {
+        d->atol = 1e-10;  // lsqr.c:47-49
+        d->btol = 1e-10;
+        d->conlim = 100.0;
+    }
+    if (d->itnlim <= 0) d->itnlim = 1000;  // lsqr.c:50
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *plsqr_last_error(const plsqr_op *op) { return op ? op->err.c_str() : "null handle"; }
+
+void plsqr_clear_error(plsqr_op *op) {
+    if (op) op->err.clear();
+}
+
+int plsqr_op_create(const plsqr_system *s, int flags, plsqr_op **out) {
+    if (!out) return PLSQR_ERR_ARG;
+    *out = nullptr;
+    plsqr_op tmp;
+    if (!s) return PLSQR_ERR_ARG;
+    if (s->m_k <= 0 || s->n <= 0 || s->m_d < 0)
+        return PLSQR_ERR_ARG;
+    if (s->nnz_k < 0 || s->nnz_d < 0 || s->nnz_k + s->nnz_d >= (1LL << 31) - 1)
+        return PLSQR_ERR_ARG;
+    plsqr_op *op = new plsqr_op();
+    op->flags = flags;
+    op->exact = (flags & PLSQR_OP_EXACT_ORDER) != 0;
+    op->m_k = s->m_k;
+    op->m_d = s->m_d;
+    op->n = s->n;
+    op->m = s->m_k + s->m_d;
+    op->nnz_k = s->nnz_k;
+    op->nnz_d = s->nnz_d;
+    int rc = PLSQR_OK;
+    auto fail = [&](int code) {
+        // keep the handle so the caller can read the message; it holds no device memory
+        for (void *p : op->allocs) (void)hipFree(p);
+        op->allocs.clear();
+        op->bytes = 0;
+        if (op->own_stream && op->stream) (void)hipStreamDestroy(op->stream);
+        op->stream = nullptr;
+        op->own_stream = false;
+        *out = op;
+        return code;
+    };
+    if ((rc = check_ptr(op, "kernel col_ptr", s->k_col_ptr, s->n, s->nnz_k)) ||
+        (rc = check_idx(op, "kernel row_idx", s->k_row_idx, s->k_val, s->nnz_k, s->m_k)))
+        return fail(rc);
+    if (s->m_d > 0) {
+        if ((rc = check_ptr(op, "damping row_ptr", s->d_row_ptr, s->m_d, s->nnz_d)) ||
+            (rc = check_idx(op, "damping col_idx", s->d_col_idx, s->d_val, s->nnz_d, s->n)) ||
+            (rc = check_ptr(op, "damping col_ptr", s->dc_col_ptr, s->n, s->nnz_d)) ||
+            (rc = check_idx(op, "damping-by-column row_idx", s->dc_row_idx, s->dc_val, s->nnz_d, s->m_d)))
+            return fail(rc);
+    } else if (s->nnz_d != 0 || !s->d_row_ptr || !s->dc_col_ptr) {
+        return fail(set_err(op, PLSQR_ERR_ARG, "empty damping needs nnz_d == 0 and zero pointer arrays"));
+    }
+    hipError_t e = hipGetDevice(&op->dev);
+    if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, op->dev);
+    if (e != hipSuccess)
+        return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)));
+    op->cu = std::max(1, prop.multiProcessorCount);
+    e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    op->own_stream = true;
+    if ((rc = build(op, s))) return fail(rc);
+    // vectors and scratch
+    op->L1 = env_int("PLSQR_L1", 16);
+    op->L2 = env_int("PLSQR_L2", 16);
+    op->G1 = grid_for((long long)op->m_k * op->L1, BS, env_int("PLSQR_G1", op->cu * 8));
+    {
+        int q = grid_for((long long)op->n * op->L2, BS, std::max(1, env_int("PLSQR_G2", op->cu * 8) / op->nsl));
+        op->G2 = q * op->nsl;
+    }
+    op->G4 = grid_for(op->n, BS, op->cu * 4);
+    op->G6 = grid_for(op->n, BS, op->cu * 4);
+    op->Gs = grid_for(op->m, BS, op->cu * 4);
+    op->part_len = 2 * std::max({op->G1, op->G4, op->G6, op->Gs, 1});
+    if ((rc = dalloc(op, &op->u, op->m)) || (rc = dalloc(op, &op->v, op->n)) || (rc = dalloc(op, &op->x, op->n)) ||
+        (rc = dalloc(op, &op->w, op->n)) || (rc = dalloc(op, &op->bst, op->m_k)) ||
+        (rc = dalloc(op, &op->P, (size_t)op->nsl * op->n)) || (rc = dalloc(op, &op->part, op->part_len)) ||
+        (rc = dalloc(op, &op->sc, 1)) || (rc = dalloc(op, &op->hx, op->n)) || (rc = dalloc(op, &op->hy, op->m)))
+        return fail(rc);
+    e = hipMemsetAsync(op->u, 0, sizeof(double) * op->m, op->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(op->v, 0, sizeof(double) * op->n, op->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->stream);
+    if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("init: ") + hipGetErrorString(e)));
+    *out = op;
+    return PLSQR_OK;
+}
+
+void plsqr_op_destroy(plsqr_op *op) {
+    if (!op) return;
+    if (op->stream) (void)hipStreamSynchronize(op->stream);
+    for (void *p : op->allocs) (void)hipFree(p);
+    if (op->own_stream && op->stream) (void)hipStreamDestroy(op->stream);
+    delete op;
+}
+
+int plsqr_op_dims(const plsqr_op *op, int *m_k, int *m_d, int *n, long long *nnz_k, long long *nnz_d) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (m_k) *m_k = op->m_k;
+    if (m_d) *m_d = op->m_d;
+    if (n) *n = op->n;
+    if (nnz_k) *nnz_k = op->nnz_k;
+    if (nnz_d) *nnz_d = op->nnz_d;
+    return PLSQR_OK;
+}
+
+size_t plsqr_op_device_bytes(const plsqr_op *op) { return op ? op->bytes : 0; }
+
+int plsqr_op_set_stream(plsqr_op *op, void *stream) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if (op->own_stream) (void)hipStreamDestroy(op->stream);
+    op->own_stream = false;
+    op->stream = (hipStream_t)stream;
+    return PLSQR_OK;
+}
+
+void *plsqr_op_get_stream(const plsqr_op *op) { return op ? (void *)op->stream : nullptr; }
+
+int plsqr_op_set_vectors(plsqr_op *op, const double *u_host, const double *v_host) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (u_host) HIP_TRY(op, hipMemcpyAsync(op->u, u_host, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
+    if (v_host) HIP_TRY(op, hipMemcpyAsync(op->v, v_host, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_op_get_vectors(plsqr_op *op, double *u_host, double *v_host) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (u_host) HIP_TRY(op, hipMemcpyAsync(u_host, op->u, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
+    if (v_host) HIP_TRY(op, hipMemcpyAsync(v_host, op->v, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!x_dev || !y_dev) return set_err(op, PLSQR_ERR_ARG, "device pointers required");
+    if (mode == 1) return aprod1_dev(op, x_dev, y_dev);
+    if (mode == 2) return aprod2_dev(op, x_dev, y_dev);
+    return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+}
+
+int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_d) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    if (!x || !y_k || (op->m_d > 0 && !y_d)) return set_err(op, PLSQR_ERR_ARG, "host vectors required");
+    HIP_TRY(op, hipMemcpyAsync(op->hx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(op->hy, y_k, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
+    if (op->m_d > 0)
+        HIP_TRY(op, hipMemcpyAsync(op->hy + op->m_k, y_d, sizeof(double) * op->m_d, hipMemcpyHostToDevice, op->stream));
+    int rc = mode == 1 ? aprod1_dev(op, op->hx, op->hy) : aprod2_dev(op, op->hx, op->hy);
+    if (rc) return rc;
+    if (mode == 1) {
+        HIP_TRY(op, hipMemcpyAsync(y_k, op->hy, sizeof(double) * op->m_k, hipMemcpyDeviceToHost, op->stream));
+        if (op->m_d > 0)
+            HIP_TRY(op, hipMemcpyAsync(y_d, op->hy + op->m_k, sizeof(double) * op->m_d, hipMemcpyDeviceToHost, op->stream));
+    } else {
+        HIP_TRY(op, hipMemcpyAsync(x, op->hx, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    }
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+static int aprod_host(plsqr_op *op, int mode, int m, int n, double *x, double *y) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    if (m != op->m || n != op->n)
+        return set_err(op, PLSQR_ERR_ARG, "aprod sizes m=" + std::to_string(m) + " n=" + std::to_string(n) +
+                                              " do not match the handle (m=" + std::to_string(op->m) +
+                                              " n=" + std::to_string(op->n) + ")");
+    double *xd = op->v, *yd = op->u;
+    if (x) {
+        HIP_TRY(op, hipMemcpyAsync(op->hx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+        xd = op->hx;
+    }
+    if (y) {
+        HIP_TRY(op, hipMemcpyAsync(op->hy, y, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
+        yd = op->hy;
+    }
+    int rc = mode == 1 ? aprod1_dev(op, xd, yd) : aprod2_dev(op, xd, yd);
+    if (rc) return rc;
+    if (mode == 1 && y) HIP_TRY(op, hipMemcpyAsync(y, yd, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
+    if (mode == 2 && x) HIP_TRY(op, hipMemcpyAsync(x, xd, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+void plsqr_aprod_ref(int version, int procRank, int numProc, int mode, int m, int n, double x[], double y[],
+                     void *UsrWrk, int isFinalIteration, int iteration) {
+    (void)version;
+    (void)procRank;
+    (void)numProc;
+    (void)isFinalIteration;
+    (void)iteration;
+    (void)aprod_host((plsqr_op *)UsrWrk, mode, m, n, x, y);
+}
+
+void plsqr_aprod(int mode, int m, int n, double x[], double y[], void *UsrWrk) {
+    plsqr_op *op = (plsqr_op *)UsrWrk;
+    if (!op) return;
+    if (!x || !y) {
+        set_err(op, PLSQR_ERR_ARG, "plsqr_aprod needs host x and y");
+        return;
+    }
+    (void)aprod_host(op, mode, m, n, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident LSQR
+// ---------------------------------------------------------------------------
+static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se) {
+    using namespace plsqr;
+    int rc;
+    if (op->exact) {
+        hipLaunchKernelGGL(k_rows_exact, dim3(grid_for(op->m, BS, op->cu * 8)), dim3(BS), 0, op->stream, op->m, op->rp,
+                           op->ci, op->rv, op->v, op->u, (const Scal *)op->sc);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->m_k, op->u, &op->sc->pk,
+                           (const Scal *)op->sc, 0);
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->m_d, op->u + op->m_k, &op->sc->pd,
+                           (const Scal *)op->sc, 0);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_cols_exact, dim3(grid_for(op->n, BS, op->cu * 8)), dim3(BS), 0, op->stream, op->n,
+                           op->m_k, op->kcp, op->kri, op->kv, op->dcp, op->dri, op->dcv, op->u, (const Scal *)op->sc,
+                           op->v);
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->n, op->v, &op->sc->pv,
+                           (const Scal *)op->sc, 1);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0);
+        hipLaunchKernelGGL(k_dknorm_seq, dim3(1), dim3(64), 0, op->stream, op->n, w, op->sc);
+        hipLaunchKernelGGL((k_update<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v,
+                           w, x, se, (double *)nullptr);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0, op->trace_d);
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
+    if ((rc = launch_rows(op, op->v, op->u, op->sc, op->part))) return rc;
+    hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
+    LAUNCH_CHECK(op);
+    if ((rc = launch_cols(op, op->u, op->v, op->sc, op->part))) return rc;
+    hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4);
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL((k_update<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v, w,
+                       x, se, op->part);
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G6, op->trace_d);
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, double *x, double *w,
+                      plsqr_solve_out *out, double *trace_host) {
+    using namespace plsqr;
+    int rc;
+    const int itn = std::max(0, o->itn);
+    const bool want_trace = trace_host != nullptr;
+    if (want_trace && (rc = ensure_trace(op, itn + 1))) return rc;
+    double *trace_d = want_trace ? op->trace_d : nullptr;
+    double *se = nullptr;  // standard errors are not exposed through this entry point
+    Scal h;
+    memset(&h, 0, sizeof(h));
+    h.damp = o->damp;
+    h.atol = o->atol;
+    h.btol = o->btol;
+    h.ctol = o->conlim > 0.0 ? 1.0 / o->conlim : 0.0;
+    h.tol = o->tol;
+    h.itnlim = o->itnlim;
+    h.itn_max = itn;
+    h.exact = op->exact ? 1 : 0;
+    h.cs2 = -1.0;  // lsqr.c:293
+    HIP_TRY(op, hipMemcpyAsync(op->sc, &h, sizeof(h), hipMemcpyHostToDevice, op->stream));
+    // u = [b ; 0], v = x = w = 0 (tridiagonal.c:33-57, lsqr.c:311-312)
+    HIP_TRY(op, hipMemcpyAsync(op->u, b_dev, sizeof(double) * op->m_k, hipMemcpyDeviceToDevice, op->stream));
+    const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m_d, 0.0, op->u + op->m_k);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->v);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, x);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, w);
+    LAUNCH_CHECK(op);
+    // beta = ||u||
+    if (op->exact) {
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->m_k, op->u, &op->sc->pk,
+                           (const Scal *)nullptr, 0);
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->m_d, op->u + op->m_k, &op->sc->pd,
+                           (const Scal *)nullptr, 0);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0);
+    } else {
+        // partials laid out as [kernel, damping] pairs, like k_rows
+        const int g = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
+        hipLaunchKernelGGL((k_ssq2<BS>), dim3(g), dim3(BS), 0, op->stream, op->m_k, op->m, op->u, op->part);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, g);
+    }
+    LAUNCH_CHECK(op);
+    // v = A^T u (scaled by 1/beta on the fly), alpha
+    if (op->exact) {
+        hipLaunchKernelGGL(k_cols_exact, dim3(grid_for(op->n, BS, op->cu * 8)), dim3(BS), 0, op->stream, op->n,
+                           op->m_k, op->kcp, op->kri, op->kv, op->dcp, op->dri, op->dcv, op->u, (const Scal *)op->sc,
+                           op->v);
+        hipLaunchKernelGGL(k_dnrm2_seq, dim3(1), dim3(64), 0, op->stream, op->n, op->v, &op->sc->pv,
+                           (const Scal *)op->sc, 1);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_init_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0, trace_d);
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(op->part_len, 256, 64)), dim3(256), 0, op->stream,
+                           (long long)op->part_len, 0.0, op->part);
+        if ((rc = launch_cols(op, op->u, op->v, op->sc, op->part))) return rc;
+        hipLaunchKernelGGL(k_s_init_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4, trace_d);
+    }
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL(k_init_vw, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                       (const Scal *)op->sc, op->v, w);
+    LAUNCH_CHECK(op);
+    // main loop, timed with events on the handle's stream
+    hipEvent_t e0, e1;
+    HIP_TRY(op, hipEventCreate(&e0));
+    HIP_TRY(op, hipEventCreate(&e1));
+    HIP_TRY(op, hipEventRecord(e0, op->stream));
+    const bool use_graph = env_int("PLSQR_GRAPH", 1) != 0 && itn > 1;
+    const int check_every = o->tol > 0.0 ? std::max(1, env_int("PLSQR_CHECK_EVERY", 8)) : 0;
+    int *done_h = nullptr;
+    if (check_every) HIP_TRY(op, hipHostMalloc((void **)&done_h, sizeof(int)));
+    if (use_graph) {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_iteration(op, x, w, se);
+        hipGraph_t g2 = nullptr;
+        hipError_t ce = hipStreamEndCapture(op->stream, &g2);
+        if (rc) return rc;
+        if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+        graph = g2;
+        HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int it = 0; it < itn; ++it) {
+            HIP_TRY(op, hipGraphLaunch(exec, op->stream));
+            if (check_every && (it + 1) % check_every == 0) {
+                HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+                HIP_TRY(op, hipStreamSynchronize(op->stream));
+                if (*done_h) break;
+            }
+        }
+        HIP_TRY(op, hipEventRecord(e1, op->stream));
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    } else {
+        for (int it = 0; it < itn; ++it) {
+            if ((rc = enqueue_iteration(op, x, w, se))) return rc;
+            if (check_every && (it + 1) % check_every == 0) {
+                HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+                HIP_TRY(op, hipStreamSynchronize(op->stream));
+                if (*done_h) break;
+            }
+        }
+        HIP_TRY(op, hipEventRecord(e1, op->stream));
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+    }
+    if (done_h) (void)hipHostFree(done_h);
+    float ms = 0.f;
+    HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the loop leaves the last 1/beta of u pending (lsqr.c:538); apply it so
+    // that the resident u is the reference's u after the loop
+    hipLaunchKernelGGL(k_scale_pending, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream,
+                       (long long)op->m, (const Scal *)op->sc, op->u);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if (out) {
+        memset(out, 0, sizeof(*out));
+        int istop = h.istop;
+        if (h.damp > 0.0 && istop == 2) istop = 3;  // lsqr.c:895-896
+        out->istop = h.init_zero ? 0 : istop;
+        out->itn = h.itn;
+        out->anorm = h.anorm;
+        out->acond = h.acond;
+        out->rnorm = h.rnorm;
+        out->arnorm = h.test2;  // lsqr.c:913
+        out->xnorm = h.xnorm;
+        out->bnorm = h.bnorm;
+        out->arnorm_true = h.arnorm;
+        out->elapsed_ms = ms;
+    }
+    if (want_trace) {
+        const int rows = std::min(h.itn, itn) + 1;
+        HIP_TRY(op, hipMemcpy(trace_host, op->trace_d, sizeof(double) * 8 * rows, hipMemcpyDeviceToHost));
+        for (int r = rows; r <= itn; ++r)
+            for (int k = 0; k < 8; ++k) trace_host[(size_t)r * 8 + k] = NAN;
+    }
+    return PLSQR_OK;
+}
+
+int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *opts, double *x_dev,
+                            double *w_dev, plsqr_solve_out *out, double *trace) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_dev || !opts || !x_dev) return set_err(op, PLSQR_ERR_ARG, "b, opts and x are required");
+    plsqr_solve_opts o;
+    default_opts(opts, &o);
+    if (o.precision != PLSQR_PREC_FP64) return set_err(op, PLSQR_ERR_ARG, "precision not supported by this build");
+    return solve_fp64(op, b_dev, &o, x_dev, w_dev ? w_dev : op->w, out, trace);
+}
+
+int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts *opts, double *x_host,
+                     double *w_host, plsqr_solve_out *out, double *trace) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_host || !opts || !x_host) return set_err(op, PLSQR_ERR_ARG, "b, opts and x are required");
+    for (int i = 0; i < op->m_k; ++i)
+        if (!std::isfinite(b_host[i])) return set_err(op, PLSQR_ERR_NONFINITE, "b holds a NaN or Inf at " + std::to_string(i));
+    HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
+    int rc = plsqr_lsqr_solve_device(op, op->bst, opts, op->x, op->w, out, trace);
+    if (rc) return rc;
+    HIP_TRY(op, hipMemcpyAsync(x_host, op->x, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    if (w_host) HIP_TRY(op, hipMemcpyAsync(w_host, op->w, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_host, double *rnorm, double *arnorm,
+                         double *anorm_f) {
+    using namespace plsqr;
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_host || !x_host) return set_err(op, PLSQR_ERR_ARG, "b and x are required");
+    HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(op->hx, x_host, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    const int g = grid_for(op->m, 256, op->cu * 4);
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, op->hx,
+                       op->bst, op->hy);
+    LAUNCH_CHECK(op);
+    std::vector<double> hp((size_t)op->part_len);
+    auto host_sum = [&](int count) {
+        double s = 0.0;
+        for (int i = 0; i < count; ++i) s += hp[(size_t)i];
+        return s;
+    };
+    const int gs = std::min(op->part_len, grid_for(op->m, BS, op->cu * 4));
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, op->hy, op->part);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * gs, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    const double rn = std::sqrt(host_sum(gs));
+    // A^T r into hx
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, (long long)op->n, 0.0,
+                       op->hx);
+    LAUNCH_CHECK(op);
+    int rc = aprod2_dev(op, op->hx, op->hy);
+    if (rc) return rc;
+    const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(gn), dim3(BS), 0, op->stream, (long long)op->n, op->hx, op->part);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * gn, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    const double an = std::sqrt(host_sum(gn));
+    const long long nnz = op->nnz_k + op->nnz_d;
+    const int ga = std::min(op->part_len, grid_for(nnz, BS, op->cu * 4));
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(ga), dim3(BS), 0, op->stream, nnz, op->rv, op->part);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * ga, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if (rnorm) *rnorm = rn;
+    if (arnorm) *arnorm = an;
+    if (anorm_f) *anorm_f = std::sqrt(host_sum(ga));
+    return PLSQR_OK;
+}
+
+}  // extern "C"

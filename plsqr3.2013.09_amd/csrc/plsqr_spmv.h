@@ -1,0 +1,685 @@
+// SpMV and vector kernels of the LSQR hot path (gfx950); included once, by plsqr_api.hip.
+//
+// Operator A = [A_k ; D] (m = m_k + m_d rows, n columns).
+//   mode 1  y <- y + A x     (aprod_impl.c:22-267)  on a CSR of [A_k ; D]
+//   mode 2  x <- x + A^T y   (aprod_impl.c:276-524) on column-major data
+//
+// Mode 1 (k_rows):   kernel rows use L lanes per row (strided partial sums,
+//                    then a fixed lane butterfly); damping rows (1 or 3
+//                    nonzeros) use one lane per row, summed in column order.
+// Mode 2 (k_cols_sliced + k_cols_combine): the kernel CSC is split into S row
+//                    slices (S = 8, one per XCD).  Workgroup b works on slice
+//                    b % S, which the hardware's round-robin dispatch places on
+//                    XCD b % 8, so each XCD gathers only its own slice of u and
+//                    that slice stays resident in the XCD's 4 MB L2.  The S
+//                    per-slice column sums are combined in a fixed order with the
+//                    damping column sums (one lane per column, rows ascending).
+// Exact-order kernels: one lane per row / column, summed sequentially in the
+//                    reference's order (CPUSpMV.c:50-89,162-183,203-245,255-296).
+#include "plsqr_kernels.h"
+
+namespace plsqr {
+
+// ---------------------------------------------------------------------------
+// Mode 1
+// ---------------------------------------------------------------------------
+// y[r] <- fl(fl(y[r]*s1)*s2) + sum_e val[e]*x[col[e]]   (s1 = s2 = 1: plain aprod)
+// In the LSQR loop s1 is the pending 1/beta of u (lsqr.c:538) and s2 = -alpha
+// (lsqr.c:502): the reference scales u twice and then adds A v, and so do we.
+// part[2b], part[2b+1] receive block b's sums of squares of the new y over the
+// kernel rows and the damping rows (the two halves of the split norm,
+// mycblas.c:99-133).
+template <int L, int BS>
+__global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restrict__ rp,
+                                             const int *__restrict__ ci,
+                                             const double *__restrict__ av,
+                                             const double *__restrict__ x,
+                                             double *__restrict__ y, const Scal *__restrict__ sc,
+                                             double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    double s1 = 1.0, s2 = 1.0;
+    const bool scaled = sc != nullptr;
+    if (scaled) {
+        if (sc->done) return;
+        s1 = sc->ib;
+        s2 = -sc->alpha;
+    }
+    constexpr int R = BS / L;  // rows per block step
+    const int lane = threadIdx.x % L;
+    const int slot = threadIdx.x / L;
+    double ssq_k = 0.0, ssq_d = 0.0;
+    const int groups = (m_k + R - 1) / R;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int r = g * R + slot;
+        double acc = 0.0;
+        if (r < m_k) {
+            const int b = rp[r], e = rp[r + 1];
+            int k = b + lane;
+            // two independent loads in flight per lane
+            for (; k + L < e; k += 2 * L) {
+                const double v0 = av[k] * x[ci[k]];
+                const double v1 = av[k + L] * x[ci[k + L]];
+                acc += v0;
+                acc += v1;
+            }
+            if (k < e) acc += av[k] * x[ci[k]];
+        }
+        acc = group_sum<L>(acc);
+        if (lane == 0 && r < m_k) {
+            double yn = y[r];
+            if (scaled) yn = (yn * s1) * s2;
+            yn = yn + acc;
+            y[r] = yn;
+            ssq_k += yn * yn;
+        }
+    }
+    // damping rows: one lane per row, columns ascending (CPUSpMV.c:75-84)
+    for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
+         r += (long long)gridDim.x * BS) {
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) acc += av[k] * x[ci[k]];
+        double yn = y[r];
+        if (scaled) yn = (yn * s1) * s2;
+        yn = yn + acc;
+        y[r] = yn;
+        ssq_d += yn * yn;
+    }
+    if (part) {
+        const double a = block_sum<BS>(ssq_k, sh);
+        const double d = block_sum<BS>(ssq_d, sh);
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = a;
+            part[2 * blockIdx.x + 1] = d;
+        }
+    }
+}
+
+// Exact order: one lane per row, sequential sum from 0 in ascending column
+// order.  For a kernel row this is the sequence of partial sums the CSC
+// scatter cpuSpMV_CSC (CPUSpMV.c:171-180) builds in tmp[r]; for a damping row
+// it is cpuSpMV_CSR_v2 (CPUSpMV.c:75-84).
+__global__ __launch_bounds__(256) void k_rows_exact(int m, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ av,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ y,
+                                                    const Scal *__restrict__ sc) {
+    double s1 = 1.0, s2 = 1.0;
+    const bool scaled = sc != nullptr;
+    if (scaled) {
+        if (sc->done) return;
+        s1 = sc->ib;
+        s2 = -sc->alpha;
+    }
+    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < m; r += (long long)gridDim.x * 256) {
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) acc += av[k] * x[ci[k]];
+        double yn = y[r];
+        if (scaled) yn = (yn * s1) * s2;
+        y[r] = yn + acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mode 2
+// ---------------------------------------------------------------------------
+// P[s*n + c] = sum over slice s of column c of val * fl(u[row]*ib)
+// The stored u of the reference is fl(u*(1/beta)) (lsqr.c:538); multiplying
+// on the fly gives the same operand.
+template <int L, int BS>
+__global__ __launch_bounds__(BS) void k_cols_sliced(int n, int nsl, const int *__restrict__ cp,
+                                                    const int *__restrict__ ri,
+                                                    const double *__restrict__ av,
+                                                    const double *__restrict__ u,
+                                                    const Scal *__restrict__ sc,
+                                                    double *__restrict__ P) {
+    double ib = 1.0;
+    if (sc) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+    }
+    const int s = blockIdx.x % nsl;
+    const int q = blockIdx.x / nsl;
+    const int Q = gridDim.x / nsl;
+    const int *cps = cp + (size_t)s * (n + 1);
+    constexpr int R = BS / L;
+    const int lane = threadIdx.x % L;
+    const int slot = threadIdx.x / L;
+    const int groups = (n + R - 1) / R;
+    for (int g = q; g < groups; g += Q) {
+        const int c = g * R + slot;
+        double acc = 0.0;
+        if (c < n) {
+            const int b = cps[c], e = cps[c + 1];
+            int k = b + lane;
+            for (; k + L < e; k += 2 * L) {
+                const double v0 = av[k] * (u[ri[k]] * ib);
+                const double v1 = av[k + L] * (u[ri[k + L]] * ib);
+                acc += v0;
+                acc += v1;
+            }
+            if (k < e) acc += av[k] * (u[ri[k]] * ib);
+        }
+        acc = group_sum<L>(acc);
+        if (lane == 0 && c < n) P[(size_t)s * n + c] = acc;
+    }
+}
+
+// v[c] <- fl(v[c]*vscale) + (tk + td) with tk the slice partials summed in
+// slice order and td the damping column sum in row order
+// (cpuSpMVTranspose_CSC_v2, CPUSpMV.c:281-291); x += (tk + td) groups as
+// aprod_impl.c:487-489.  part[b] = block b's sum of squares of the new v.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cols_combine(int n, int m_k, int nsl,
+                                                     const double *__restrict__ P,
+                                                     const int *__restrict__ dcp,
+                                                     const int *__restrict__ dri,
+                                                     const double *__restrict__ dav,
+                                                     const double *__restrict__ u,
+                                                     const Scal *__restrict__ sc,
+                                                     double *__restrict__ v,
+                                                     double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    double ib = 1.0, vs = 1.0;
+    const bool scaled = sc != nullptr;
+    if (scaled) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+        vs = sc->vscale;
+    }
+    const double *ud = u + m_k;
+    double ssq = 0.0;
+    for (int c = blockIdx.x * BS + threadIdx.x; c < n; c += gridDim.x * BS) {
+        double tk = P[c];
+        for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
+        double td = 0.0;
+        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * (ud[dri[k]] * ib);
+        double vn = v[c];
+        if (scaled) vn = vn * vs;
+        vn = vn + (tk + td);
+        v[c] = vn;
+        ssq += vn * vn;
+    }
+    if (part) {
+        const double a = block_sum<BS>(ssq, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = a;
+    }
+}
+
+// Exact order (cpuSpMVTranspose_CSC and _v2, CPUSpMV.c:230-240,281-291, and
+// aprod_impl.c:487-489): one lane per column.
+__global__ __launch_bounds__(256) void k_cols_exact(int n, int m_k, const int *__restrict__ cp,
+                                                    const int *__restrict__ ri,
+                                                    const double *__restrict__ av,
+                                                    const int *__restrict__ dcp,
+                                                    const int *__restrict__ dri,
+                                                    const double *__restrict__ dav,
+                                                    const double *__restrict__ u,
+                                                    const Scal *__restrict__ sc,
+                                                    double *__restrict__ v) {
+    double ib = 1.0, vs = 1.0;
+    const bool scaled = sc != nullptr;
+    if (scaled) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+        vs = sc->vscale;
+    }
+    const double *ud = u + m_k;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
+        double tk = 0.0;
+        for (int k = cp[c]; k < cp[c + 1]; ++k) tk += av[k] * (u[ri[k]] * ib);
+        double td = 0.0;
+        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * (ud[dri[k]] * ib);
+        double vn = v[c];
+        if (scaled) vn = vn * vs;
+        v[c] = vn + (tk + td);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Norms
+// ---------------------------------------------------------------------------
+// Block partial sums of squares of y[0:len) (used for ||b|| at start).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_ssq(long long len, const double *__restrict__ y,
+                                            double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < len; i += (long long)gridDim.x * BS)
+        s += y[i] * y[i];
+    const double a = block_sum<BS>(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// Block partial sums of squares in the pair layout of k_rows: part[2b] over
+// y[0:m_k) (kernel rows), part[2b+1] over y[m_k:m) (damping rows).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_ssq2(int m_k, int m, const double *__restrict__ y,
+                                             double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    double a = 0.0, d = 0.0;
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < m; i += (long long)gridDim.x * BS) {
+        const double q = y[i] * y[i];
+        if (i < m_k)
+            a += q;
+        else
+            d += q;
+    }
+    a = block_sum<BS>(a, sh);
+    d = block_sum<BS>(d, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = d;
+    }
+}
+
+// u <- u * ib for the 1/beta scale still pending when the loop ends
+__global__ void k_scale_pending(long long len, const Scal *__restrict__ sc, double *__restrict__ u) {
+    const double ib = sc->ib;
+    if (!sc->beta_pos || ib == 1.0) return;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        u[i] = u[i] * ib;
+}
+
+// cblas_dnrm2 (mycblas.c:194-219) in its own sequential order, one lane.
+// Writes the norm to *out.  Exact-order mode only.
+__global__ void k_dnrm2_seq(int len, const double *__restrict__ v, double *out, const Scal *sc,
+                            int check_beta) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc && (sc->done || (check_beta && !sc->beta_pos))) return;
+    double scale = 0.0, ssq = 1.0;
+    if (len <= 0) {
+        *out = 0.0;
+        return;
+    }
+    if (len == 1) {
+        *out = fabs(v[0]);
+        return;
+    }
+    for (int i = 0; i < len; ++i) {
+        const double x = v[i];
+        if (x != 0.0) {
+            const double ax = fabs(x);
+            if (scale < ax) {
+                const double q = scale / ax;
+                ssq = 1.0 + ssq * q * q;
+                scale = ax;
+            } else {
+                const double q = ax / scale;
+                ssq += q * q;
+            }
+        }
+    }
+    *out = scale * sqrt(ssq);
+}
+
+// dknorm of lsqr.c:685-690 in the reference's order, one lane: reads the
+// w of before the update.  Exact-order mode only.
+__global__ void k_dknorm_seq(int n, const double *__restrict__ w, Scal *sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc->done) return;
+    const double t3 = sc->t3;
+    double dk = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double t = w[i];
+        dk = (t3 * t) * (t3 * t) + dk;
+    }
+    sc->dk_seq = dk;
+}
+
+// ---------------------------------------------------------------------------
+// x / w update (lsqr.c:590-597 then 685-690), fused with the v scale
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(BS) void k_update(int n, const Scal *__restrict__ sc,
+                                               double *__restrict__ v, double *__restrict__ w,
+                                               double *__restrict__ x, double *__restrict__ se,
+                                               double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    if (sc->done) return;
+    const bool scl = sc->scale_v != 0;
+    const double ia = sc->inv_alpha, t1 = sc->t1, t2 = sc->t2, t3 = sc->t3;
+    double dk = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        double vi = v[i];
+        if (scl) {
+            vi = vi * ia;
+            v[i] = vi;
+        }
+        const double t = w[i];
+        x[i] = t1 * t + x[i];
+        w[i] = t2 * t + vi;
+        const double q = t3 * t;
+        const double qq = q * q;
+        if (se) se[i] = qq + se[i];
+        dk += qq;
+    }
+    if (part) {
+        const double a = block_sum<BS>(dk, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = a;
+    }
+}
+
+// v <- v * inv_alpha (if alpha > 0), w <- v   (lsqr.c:392-395)
+__global__ void k_init_vw(int n, const Scal *__restrict__ sc, double *__restrict__ v,
+                          double *__restrict__ w) {
+    if (sc->done) return;
+    const bool scl = sc->scale_v != 0;
+    const double ia = sc->inv_alpha;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double vi = v[i];
+        if (scl) {
+            vi = vi * ia;
+            v[i] = vi;
+            w[i] = vi;
+        }
+    }
+}
+
+// final standard errors, lsqr.c:878-891
+__global__ void k_se_final(int n, double t, double *__restrict__ se) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) se[i] = t * sqrt(se[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Scalar recurrences (one block; thread 0 does the scalar work)
+// ---------------------------------------------------------------------------
+__device__ void trace_row(double *trace, const Scal &s, double alpha) {
+    if (!trace) return;
+    double *t = trace + (size_t)s.itn * 8;
+    t[0] = alpha;
+    t[1] = s.beta;
+    t[2] = s.rnorm;
+    t[3] = s.test1;
+    t[4] = s.test2;
+    t[5] = s.anorm;
+    t[6] = s.acond;
+    t[7] = s.xnorm;
+}
+
+// Start: beta = ||[b;0]|| (lsqr.c:343); u is scaled lazily (ib).
+__global__ __launch_bounds__(256) void k_s_init_beta(Scal *sc, const double *part, int count) {
+    __shared__ double sh[4];
+    const double sk = reduce_partials<256>(part, count, 2, sh);
+    const double sd = reduce_partials<256>(part + 1, count, 2, sh);
+    if (threadIdx.x != 0) return;
+    Scal &s = *sc;
+    double beta;
+    if (s.exact) {
+        const double pk = s.pk, pd = sqrt(s.pd * s.pd);
+        beta = sqrt(pk * pk + pd * pd);
+    } else {
+        beta = sqrt(sk + sd);
+    }
+    s.beta = beta;
+    s.alpha = 0.0;
+    if (beta > 0.0) {
+        s.ib = 1.0 / beta;
+        s.beta_pos = 1;
+    } else {
+        s.ib = 1.0;
+        s.beta_pos = 0;
+    }
+    s.vscale = 1.0;  // first v += A^T u starts from v = 0 (lsqr.c:311,366)
+}
+
+// After the first v += A^T u: alpha, w = v / alpha, initial estimates
+// (lsqr.c:375-445).
+__global__ __launch_bounds__(256) void k_s_init_alpha(Scal *sc, const double *part, int count,
+                                                      double *trace) {
+    __shared__ double sh[4];
+    const double sv = reduce_partials<256>(part, count, 1, sh);
+    if (threadIdx.x != 0) return;
+    Scal &s = *sc;
+    double alpha = 0.0;
+    if (s.beta_pos) alpha = s.exact ? sqrt(s.pv * s.pv) : sqrt(sv);
+    s.alpha = alpha;
+    if (alpha > 0.0) {
+        s.inv_alpha = 1.0 / alpha;
+        s.scale_v = 1;
+    } else {
+        s.inv_alpha = 1.0;
+        s.scale_v = 0;
+    }
+    s.arnorm = alpha * s.beta;
+    if (s.arnorm == 0.0) {  // goto_800
+        s.done = 1;
+        s.init_zero = 1;
+        s.itn = 0;
+        s.rnorm = 0.0;
+        s.test1 = 0.0;
+        s.test2 = 0.0;
+        trace_row(trace, s, alpha);
+        return;
+    }
+    s.rhobar = alpha;
+    s.phibar = s.beta;
+    s.bnorm = s.beta;
+    s.rnorm = s.beta;
+    s.test1 = 1.0;
+    s.test2 = alpha / s.beta;
+    s.itn = 0;
+    trace_row(trace, s, alpha);
+    if (s.itn_max <= 0) s.done = 1;
+}
+
+// After u <- u*(-alpha) + A v: beta, anorm (lsqr.c:487, 516-528, 530-538, 558).
+__global__ __launch_bounds__(256) void k_s_beta(Scal *sc, const double *part, int count) {
+    __shared__ double sh[4];
+    if (sc->done) return;  // uniform across the block
+    const double sk = reduce_partials<256>(part, count, 2, sh);
+    const double sd = reduce_partials<256>(part + 1, count, 2, sh);
+    if (threadIdx.x != 0) return;
+    Scal &s = *sc;
+    s.itn = s.itn + 1;
+    double beta;
+    if (s.exact) {
+        const double pk = s.pk, pd = sqrt(s.pd * s.pd);
+        beta = sqrt(pk * pk + pd * pd);
+    } else {
+        beta = sqrt(sk + sd);
+    }
+    s.beta = beta;
+    double temp = d2norm(s.alpha, beta);
+    temp = d2norm(temp, s.damp);
+    s.anorm = d2norm(s.anorm, temp);
+    if (beta > 0.0) {
+        s.ib = 1.0 / beta;
+        s.vscale = -beta;
+        s.beta_pos = 1;
+    } else {
+        s.ib = 1.0;
+        s.beta_pos = 0;
+    }
+}
+
+// After v <- v*(-beta) + A^T u: alpha and the two plane rotations
+// (lsqr.c:577-637).
+__global__ __launch_bounds__(256) void k_s_alpha(Scal *sc, const double *part, int count) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    const double sv = reduce_partials<256>(part, count, 1, sh);
+    if (threadIdx.x != 0) return;
+    Scal &s = *sc;
+    s.scale_v = 0;
+    if (s.beta_pos) {
+        const double alpha = s.exact ? sqrt(s.pv * s.pv) : sqrt(sv);
+        s.alpha = alpha;
+        if (alpha > 0.0) {
+            s.inv_alpha = 1.0 / alpha;
+            s.scale_v = 1;
+        }
+    }
+    const double alpha = s.alpha, beta = s.beta;
+    double rhbar1 = s.rhobar;
+    if (s.damp > 0.0) {
+        rhbar1 = d2norm(s.rhobar, s.damp);
+        const double cs1 = s.rhobar / rhbar1;
+        const double sn1 = s.damp / rhbar1;
+        s.psi = sn1 * s.phibar;
+        s.phibar = cs1 * s.phibar;
+    }
+    const double rho = d2norm(rhbar1, beta);
+    const double cs = rhbar1 / rho;
+    const double sn = beta / rho;
+    const double theta = sn * alpha;
+    s.rhobar = -cs * alpha;
+    const double phi = cs * s.phibar;
+    s.phibar = sn * s.phibar;
+    const double tau = sn * phi;
+    s.phi = phi;
+    s.rho = rho;
+    s.t1 = phi / rho;
+    s.t2 = -theta / rho;
+    s.t3 = 1.0 / rho;
+    s.theta = theta;
+    s.arnorm = alpha * fabs(tau);  // lsqr.c:735 (alpha and tau are final here)
+}
+
+// After the x/w update: dknorm, norm and condition estimates, test1..3,
+// istop and the stop decision (lsqr.c:703-775, 820-834).
+__global__ __launch_bounds__(256) void k_s_est(Scal *sc, const double *part, int count,
+                                               double *trace) {
+    __shared__ double sh[4];
+    if (sc->done) return;
+    const double sk = reduce_partials<256>(part, count, 1, sh);
+    if (threadIdx.x != 0) return;
+    Scal &s = *sc;
+    const double theta = s.theta;
+    double dknorm = s.exact ? s.dk_seq : sk;
+    dknorm = sqrt(dknorm);
+    s.dnorm = d2norm(s.dnorm, dknorm);
+    const double dxk = fabs(s.phi * dknorm);
+    if (s.dxmax < dxk) {
+        s.dxmax = dxk;
+        s.maxdx = s.itn;
+    }
+    const double rho = s.rho;
+    const double delta = s.sn2 * rho;
+    const double gambar = -s.cs2 * rho;
+    const double rhs = s.phi - delta * s.z;
+    const double zbar = rhs / gambar;
+    s.xnorm = d2norm(s.xnorm1, zbar);
+    const double gamma = d2norm(gambar, theta);
+    s.cs2 = gambar / gamma;
+    s.sn2 = theta / gamma;
+    s.z = rhs / gamma;
+    s.xnorm1 = d2norm(s.xnorm1, s.z);
+    s.acond = s.anorm * s.dnorm;
+    s.res2 = d2norm(s.res2, s.psi);
+    s.rnorm = d2norm(s.res2, s.phibar);
+    s.test1 = s.rnorm / s.bnorm;
+    s.test2 = 0.0;
+    if (s.rnorm > 0.0) s.test2 = s.arnorm / (s.anorm * s.rnorm);
+    s.test3 = 1.0 / s.acond;
+    double t1 = s.test1 / (1.0 + s.anorm * s.xnorm / s.bnorm);
+    const double rtol = s.btol + s.atol * s.anorm * s.xnorm / s.bnorm;
+    const double t3 = 1.0 + s.test3;
+    const double t2 = 1.0 + s.test2;
+    t1 = 1.0 + t1;
+    if (s.itn >= s.itnlim) s.istop = 5;
+    if (t3 <= 1.0) s.istop = 4;
+    if (t2 <= 1.0) s.istop = 2;
+    if (t1 <= 1.0) s.istop = 1;
+    if (s.test3 <= s.ctol) s.istop = 4;
+    if (s.test2 <= s.atol) s.istop = 2;
+    if (s.test1 <= rtol) s.istop = 1;
+    if (s.istop == 0) {
+        s.nstop = 0;
+    } else {
+        s.nstop = s.nstop + 1;
+        if (s.nstop < 1 && s.itn < s.itnlim) s.istop = 0;
+    }
+    trace_row(trace, s, s.alpha);
+    if (s.itn >= s.itn_max) s.done = 1;
+    if (s.tol > 0.0 && s.test2 <= s.tol) s.done = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Re-layout helpers
+// ---------------------------------------------------------------------------
+__global__ void k_expand_ptr(int n, const int *__restrict__ ptr, int *__restrict__ idx) {
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256)
+        for (int e = ptr[c]; e < ptr[c + 1]; ++e) idx[e] = c;
+}
+
+__global__ void k_iota(long long len, int *__restrict__ a) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        a[i] = (int)i;
+}
+
+__global__ void k_count(long long len, const int *__restrict__ key, int *__restrict__ cnt) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        atomicAdd(&cnt[key[i]], 1);
+}
+
+// dst_idx[p] = src_idx[perm[p]] (+ offset), dst_val[p] = src_val[perm[p]]
+__global__ void k_permute(long long len, const int *__restrict__ perm, const int *__restrict__ src_idx,
+                          const double *__restrict__ src_val, int *__restrict__ dst_idx,
+                          double *__restrict__ dst_val) {
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < len; p += (long long)gridDim.x * 256) {
+        const int q = perm[p];
+        dst_idx[p] = src_idx[q];
+        dst_val[p] = src_val[q];
+    }
+}
+
+// key[e] = slice(row_idx[e]) * n + col[e]
+__global__ void k_slice_key(long long len, int n, int nsl, const int *__restrict__ bounds,
+                            const int *__restrict__ row, const int *__restrict__ col,
+                            int *__restrict__ key) {
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < len; e += (long long)gridDim.x * 256) {
+        const int r = row[e];
+        int s = 0;
+        while (s + 1 < nsl && r >= bounds[s + 1]) ++s;
+        key[e] = s * n + col[e];
+    }
+}
+
+// sliced column pointers from the exclusive scan of per-(slice, column) counts
+__global__ void k_slice_ptr(int n, int nsl, const int *__restrict__ off, long long total,
+                            int *__restrict__ cp) {
+    const long long len = (long long)nsl * (n + 1);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256) {
+        const int s = (int)(i / (n + 1));
+        const int c = (int)(i % (n + 1));
+        const long long key = (long long)s * n + c;
+        cp[i] = key < (long long)nsl * n ? off[key] : (int)total;
+    }
+}
+
+// stacked CSR: damping rows appended after the kernel rows
+__global__ void k_append_rows(int m_k, int m_d, long long nnz_k, long long nnz_d,
+                              const int *__restrict__ drp, const int *__restrict__ dci,
+                              const double *__restrict__ dav, int *__restrict__ rp,
+                              int *__restrict__ ci, double *__restrict__ av) {
+    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = tid; i <= m_d; i += stride) rp[m_k + i] = (int)(nnz_k + drp[i]);
+    for (long long e = tid; e < nnz_d; e += stride) {
+        ci[nnz_k + e] = dci[e];
+        av[nnz_k + e] = dav[e];
+    }
+}
+
+__global__ void k_fill(long long len, double v, double *__restrict__ a) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256) a[i] = v;
+}
+
+__global__ void k_copy(long long len, const double *__restrict__ src, double *__restrict__ dst) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+// r = [b;0] - A x on the stacked CSR (rows), FP64
+__global__ void k_residual(int m_k, int m, const int *__restrict__ rp, const int *__restrict__ ci,
+                           const double *__restrict__ av, const double *__restrict__ x,
+                           const double *__restrict__ b, double *__restrict__ r) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
+        double acc = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) acc += av[k] * x[ci[k]];
+        r[i] = (i < m_k ? b[i] : 0.0) - acc;
+    }
+}
+
+}  // namespace plsqr

@@ -1,0 +1,147 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs in the build container only (it needs /root/reference for
+oracle/build_ref.sh and MPICH's mpiexec).  For each case it generates the
+system with the library's seeded generator, writes the seven files the
+reference CLI reads (plsqr3.c:123-186), runs the FP64 reference binary
+(oracle/_ref/plsqr3_ref_fp64) with `mpiexec -np 1`, and stores:
+
+  x        float64[n]  the reference's x after `itn` iterations, read from
+                       p_resultx_itn=<itn>_NProc=1_float64_mpiio.bin
+                       (lsqr.c:96-99, plsqr3.c:1986-2013)
+  alpha, beta          the per-iteration values the reference prints
+                       (lsqr.c:352,383,521,579; "%e", 7 significant digits)
+  log_*                the iteration lines of result.txt (lsqr.c:443,809:
+                       itn, x(1), rnorm, test1, test2, anorm, acond)
+  final_*              the exit block of result.txt (lsqr.c:899-904)
+  sha256               digest of the generated system's arrays, so tests
+                       can tell that the generator still produces the
+                       system the fixture was made from
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import importlib.util
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "plsqr3_ref_fp64")
+MPIEXEC = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
+
+
+def load_pkg():
+    name = "plsqr_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg = os.path.join(ROOT, "plsqr3.2013.09_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+# name -> generator arguments, iteration count, optional b override
+CASES = {
+    # 16^3 grid, 20,000 rays between opposite faces, lambda 0.1, sigma 0.01
+    "grid16_rays20k_itn100": dict(gen=dict(nx=16, ny=16, nz=16, n_rays=20000, geometry=0, seed=2013,
+                                           sigma=0.01, lam=0.1), itn=100),
+    "grid16_rays20k_itn1": dict(gen=dict(nx=16, ny=16, nz=16, n_rays=20000, geometry=0, seed=2013,
+                                         sigma=0.01, lam=0.1), itn=1),
+    # tiny cube, 12 rays: small enough for dense brute force
+    "grid4_rays12_itn10": dict(gen=dict(nx=4, ny=4, nz=4, n_rays=12, geometry=0, seed=7,
+                                        sigma=0.01, lam=0.5), itn=10),
+    # non-cubic grid with few rays from the top face: many empty kernel columns,
+    # nz = 3 leaves one z stencil row per column
+    "grid7x5x3_rays40_itn20": dict(gen=dict(nx=7, ny=5, nz=3, n_rays=40, geometry=1, seed=11,
+                                            sigma=0.02, lam=0.3), itn=20),
+    # b = 0: beta = 0 at start, the reference leaves x = 0 (lsqr.c:400-401)
+    "grid4_zero_rhs_itn5": dict(gen=dict(nx=4, ny=4, nz=4, n_rays=12, geometry=0, seed=7,
+                                         sigma=0.01, lam=0.5), itn=5, zero_b=True),
+}
+
+ARRAYS = ("k_col_ptr", "k_row_idx", "k_val", "d_row_ptr", "d_col_idx", "d_val", "dc_col_ptr",
+          "dc_row_idx", "dc_val", "b")
+
+
+def system_digest(s):
+    h = hashlib.sha256()
+    for k in ARRAYS:
+        h.update(np.ascontiguousarray(getattr(s, k)).tobytes())
+    return h.hexdigest()
+
+
+def make_system(P, case):
+    s = P.generate(**case["gen"])
+    if case.get("zero_b"):
+        s = P.System.from_arrays(s.m_k, s.m_d, s.n, s.k_col_ptr, s.k_row_idx, s.k_val, s.d_row_ptr,
+                                 s.d_col_idx, s.d_val, s.dc_col_ptr, s.dc_row_idx, s.dc_val,
+                                 np.zeros(s.m_k), s.x_true)
+    return s
+
+
+def run_reference(P, s, itn, workdir):
+    names = s.write_reference_files(workdir)
+    cmd = [MPIEXEC, "-np", "1", REF_BIN, "-dir", workdir]
+    for key, flag in (("ker_f", "-ker_f"), ("ker_i", "-ker_i"), ("damp_f", "-damp_f"),
+                      ("damp_i", "-damp_i"), ("damp_f_bycol", "-damp_f_bycol"),
+                      ("damp_i_bycol", "-damp_i_bycol"), ("b_k", "-b_k")):
+        cmd += [flag, names[key]]
+    cmd += ["-row_k", str(s.m_k), "-row_d", str(s.m_d), "-col", str(s.n), "-itn", str(itn)]
+    r = subprocess.run(cmd, cwd=workdir, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return r.stdout
+
+
+def parse(stdout, workdir, itn, n):
+    betas = [float(v) for v in re.findall(r"beta=(\S+) from vector y", stdout)]
+    alphas = [float(v) for v in re.findall(r"alpha=(\S+) from vector x", stdout)]
+    t = re.findall(r"Parallel LSQR time=(\S+) s", stdout)
+    x = np.fromfile(os.path.join(workdir, f"p_resultx_itn={itn}_NProc=1_float64_mpiio.bin"),
+                    dtype=np.float64)
+    assert x.size == n, (x.size, n)
+    log = []
+    final = {}
+    with open(os.path.join(workdir, "result.txt")) as f:
+        for line in f:
+            p = line.split()
+            if len(p) in (5, 7) and re.fullmatch(r"\d+", p[0]):
+                row = [float(v) for v in p] + [np.nan] * (7 - len(p))
+                log.append(row)
+            for key in ("istop", "itn", "anorm", "acond", "vnorm", "xnorm", "rnorm", "arnorm"):
+                m = re.search(rf"\b{key}\s+=\s+(\S+)", line)
+                if m and "Exit" in line:
+                    final[key] = float(m.group(1))
+    return dict(x=x, alpha=np.array(alphas), beta=np.array(betas),
+                loop_time_s=float(t[0]) if t else np.nan, log=np.array(log), final=final)
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
+    P = load_pkg()
+    for name, case in CASES.items():
+        s = make_system(P, case)
+        with tempfile.TemporaryDirectory() as d:
+            out = parse(run_reference(P, s, case["itn"], d), d, case["itn"], s.n)
+        path = os.path.join(HERE, name + ".npz")
+        fin = out["final"]
+        np.savez(path, x=out["x"], alpha=out["alpha"], beta=out["beta"], log=out["log"],
+                 final_keys=np.array(sorted(fin)), final_vals=np.array([fin[k] for k in sorted(fin)]),
+                 itn=np.array(case["itn"]), sha256=np.array(system_digest(s)),
+                 dims=np.array([s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d], dtype=np.int64))
+        print(f"{name}: n={s.n} m_k={s.m_k} m_d={s.m_d} nnz_k={s.nnz_k} -> {os.path.basename(path)} "
+              f"({os.path.getsize(path)} B, loop {out['loop_time_s']} s)")
+
+
+if __name__ == "__main__":
+    main()

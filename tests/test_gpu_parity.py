@@ -1,0 +1,176 @@
+"""GPU parity through the C-ABI against the oracle and the reference's golden x.
+
+Bars: exact-order operator (PLSQR_OP_EXACT_ORDER) bit-identical to the
+reference; default operator within 1e-12 relative per aprod and 1e-10
+relative in x after the reference's iteration count (the tolerance the
+plain FP64 path is held to).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["grid4_rays12_itn10", "grid7x5x3_rays40_itn20", "grid16_rays20k_itn1",
+         "grid16_rays20k_itn100"]
+
+
+def announce(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("exact", [True, False])
+def test_aprod_matches_oracle(P, O, MG, gpu_available, name, exact):
+    announce(f"aprod {name} exact={exact}")
+    s = MG.make_system(P, MG.CASES[name])
+    orc = O.Oracle(s)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    with P.Operator(s, P.OP_EXACT_ORDER if exact else P.OP_DEFAULT) as op:
+        for mode in (1, 2):
+            xg, yg = x.copy(), y.copy()
+            op.aprod(mode, xg, yg)
+            xo, yo = x.copy(), y.copy()
+            orc.aprod(mode, xo, yo)
+            got, want = (yg, yo) if mode == 1 else (xg, xo)
+            if exact:
+                assert np.array_equal(got, want), f"mode {mode}: max diff {np.abs(got - want).max()}"
+            else:
+                assert rel(got, want) < 1e-13, (mode, rel(got, want))
+            # the untouched operand is unchanged
+            if mode == 1:
+                assert np.array_equal(xg, x)
+            else:
+                assert np.array_equal(yg, y)
+            # run-to-run reproducible
+            xg2, yg2 = x.copy(), y.copy()
+            op.aprod(mode, xg2, yg2)
+            assert np.array_equal(xg2, xg) and np.array_equal(yg2, yg)
+
+
+def test_aprod_entry_points_agree(P, MG, gpu_available):
+    announce("aprod entry points")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn1"])
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    with P.Operator(s) as op:
+        for mode in (1, 2):
+            a_x, a_y = x.copy(), y.copy()
+            op.aprod(mode, a_x, a_y)
+            b_x, b_y = x.copy(), y.copy()
+            op.aprod_classic(mode, b_x, b_y)
+            c_x, yk, yd = x.copy(), y[:s.m_k].copy(), y[s.m_k:].copy()
+            op.aprod_split(mode, c_x, yk, yd)
+            op.set_vectors(u=y, v=x)
+            op.aprod_resident(mode)
+            d_y, d_x = op.get_vectors()
+            for got_x, got_y in ((b_x, b_y), (c_x, np.concatenate([yk, yd])), (d_x, d_y)):
+                assert np.array_equal(got_x, a_x) and np.array_equal(got_y, a_y)
+
+
+def test_aprod_rejects_bad_arguments(P, MG, gpu_available):
+    announce("aprod errors")
+    s = MG.make_system(P, MG.CASES["grid4_rays12_itn10"])
+    with P.Operator(s) as op:
+        x = np.zeros(s.n)
+        y = np.zeros(s.m_k + s.m_d)
+        with pytest.raises(P.PlsqrError):
+            op.aprod(3, x, y)
+        op.lib.plsqr_aprod_ref(0, 0, 1, 1, op.m + 1, op.n, P.dptr(x), P.dptr(y), op.h, 0, 0)
+        assert "do not match" in op.last_error()
+        op.lib.plsqr_clear_error(op.h)
+        # the handle still works after an error
+        op.aprod(1, x, y)
+
+
+@pytest.mark.parametrize("name", SMALL + ["grid4_zero_rhs_itn5"])
+def test_exact_order_lsqr_is_bitwise_reference(P, MG, gpu_available, name):
+    announce(f"exact lsqr {name}")
+    g = load_golden(name)
+    s = MG.make_system(P, MG.CASES[name])
+    with P.Operator(s, P.OP_EXACT_ORDER) as op:
+        x, info = op.lsqr(s.b, int(g["itn"]))
+    assert np.array_equal(x, g["x"]), f"max diff {np.abs(x - g['x']).max()}"
+
+
+@pytest.mark.parametrize("name", SMALL + ["grid4_zero_rhs_itn5"])
+def test_default_lsqr_within_1e10_of_reference(P, O, MG, gpu_available, name):
+    announce(f"default lsqr {name}")
+    g = load_golden(name)
+    s = MG.make_system(P, MG.CASES[name])
+    itn = int(g["itn"])
+    with P.Operator(s) as op:
+        x, info, tr = op.lsqr(s.b, itn, trace=True)
+        x2, info2 = op.lsqr(s.b, itn)            # second solve on the same handle
+    assert np.array_equal(x, x2), "not run-to-run reproducible"
+    if not g["x"].any():
+        assert not x.any()
+        return
+    assert rel(x, g["x"]) < 1e-10, rel(x, g["x"])
+    xo, res, tro = O.Oracle(s).lsqr(s.b, itn)
+    np.testing.assert_allclose(tr, tro, rtol=1e-9, atol=0)
+    for key in ("anorm", "acond", "rnorm", "arnorm", "xnorm", "bnorm"):
+        np.testing.assert_allclose(info[key], res[key], rtol=1e-9)
+    assert info["itn"] == res["itn"] and info["istop"] == res["istop"]
+
+
+def test_exact_trace_and_outputs_bitwise(P, O, MG, gpu_available):
+    announce("exact trace")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    with P.Operator(s, P.OP_EXACT_ORDER) as op:
+        x, info, tr = op.lsqr(s.b, 100, trace=True, want_w=True)
+        u, v = op.get_vectors()
+    xo, res, tro = O.Oracle(s).lsqr(s.b, 100)
+    assert np.array_equal(tr, tro)
+    assert np.array_equal(info["w"], res["w"])
+    assert np.array_equal(v, res["v"]) and np.array_equal(u, res["u"])
+    for key in ("anorm", "acond", "rnorm", "arnorm", "xnorm", "bnorm", "arnorm_true"):
+        assert info[key] == res[key], key
+
+
+def test_tolerance_stop(P, O, MG, gpu_available):
+    """tol > 0 stops at the first iteration with test2 <= tol (time-to-tol runs)."""
+    announce("tol stop")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    xo, res, tro = O.Oracle(s).lsqr(s.b, 100)
+    k = int(np.argmax(tro[1:, 4] <= 1e-6)) + 1
+    with P.Operator(s) as op:
+        x, info = op.lsqr(s.b, 100, tol=1e-6)
+    assert info["itn"] == k
+    xk, _, _ = O.Oracle(s).lsqr(s.b, k)
+    assert rel(x, xk) < 1e-10
+
+
+def test_zero_iterations_and_zero_rhs(P, MG, gpu_available):
+    announce("edge cases")
+    s = MG.make_system(P, MG.CASES["grid4_rays12_itn10"])
+    with P.Operator(s) as op:
+        x, info = op.lsqr(s.b, 0)
+        assert not x.any() and info["itn"] == 0
+        x, info = op.lsqr(np.zeros(s.m_k), 7)
+        assert not x.any() and info["itn"] == 0 and info["istop"] == 0
+        bad = s.b.copy()
+        bad[2] = np.nan
+        with pytest.raises(P.PlsqrError) as e:
+            op.lsqr(bad, 3)
+        assert e.value.code == 8
+
+
+def test_residual_norms_confirm_test2(P, MG, gpu_available):
+    announce("residual norms")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    import scipy.sparse as sp
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    with P.Operator(s) as op:
+        x, info = op.lsqr(s.b, 40)
+        rn, an, af = op.residual_norms(s.b, x)
+    r = np.concatenate([s.b, np.zeros(s.m_d)]) - A @ x
+    np.testing.assert_allclose(rn, np.linalg.norm(r), rtol=1e-12)
+    np.testing.assert_allclose(an, np.linalg.norm(A.T @ r), rtol=1e-10)
+    np.testing.assert_allclose(af, sp.linalg.norm(A), rtol=1e-12)
+    np.testing.assert_allclose(rn, info["rnorm"], rtol=1e-8)

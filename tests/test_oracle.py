@@ -1,0 +1,183 @@
+"""The CPU oracle, pinned before it is trusted.
+
+  * against the compiled FP64 reference (golden fixtures, bitwise x),
+  * against closed forms (dnrm2 = sqrt(sum x^2), d2norm = hypot),
+  * against dense brute force and the adjoint identity for the SpMVs,
+  * against scipy.sparse.linalg.lsqr for the LSQR recurrences,
+  * and with planted mistakes, to show each check can fail.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import load_golden, rel
+
+CASES = ["grid16_rays20k_itn100", "grid16_rays20k_itn1", "grid4_rays12_itn10",
+         "grid7x5x3_rays40_itn20", "grid4_zero_rhs_itn5"]
+
+
+def dense(s):
+    Ak = sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)).toarray()
+    D = sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n)).toarray()
+    return np.vstack([Ak, D])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_generator_reproduces_fixture_system(P, MG, name):
+    g = load_golden(name)
+    s = MG.make_system(P, MG.CASES[name])
+    assert MG.system_digest(s) == str(g["sha256"])
+    assert [s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d] == list(g["dims"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_bitwise(P, O, MG, name):
+    """x after itn iterations equals the compiled reference's x bit for bit."""
+    g = load_golden(name)
+    s = MG.make_system(P, MG.CASES[name])
+    x, res, tr = O.Oracle(s).lsqr(s.b, int(g["itn"]))
+    assert np.array_equal(x, g["x"]), f"max diff {np.abs(x - g['x']).max()}"
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if "zero" not in c])
+def test_oracle_trace_matches_reference_print(P, O, MG, name):
+    """alpha and beta as printed by the reference ("%e", lsqr.c:352,383,521,579)
+    and the result.txt iteration lines (lsqr.c:443,809)."""
+    g = load_golden(name)
+    s = MG.make_system(P, MG.CASES[name])
+    itn = int(g["itn"])
+    x, res, tr = O.Oracle(s).lsqr(s.b, itn)
+    assert len(g["alpha"]) == itn + 1 and len(g["beta"]) == itn + 1
+    np.testing.assert_allclose(tr[:, 0], g["alpha"], rtol=6e-7)
+    np.testing.assert_allclose(tr[:, 1], g["beta"], rtol=6e-7)
+    for row in g["log"]:
+        k = int(row[0])
+        np.testing.assert_allclose(tr[k, 2], row[2], rtol=2e-9)          # rnorm %16.9e
+        np.testing.assert_allclose(tr[k, 3], row[3], rtol=6e-3)          # test1 %9.2e
+        np.testing.assert_allclose(tr[k, 4], row[4], rtol=6e-3)          # test2 %9.2e
+        if not np.isnan(row[5]):
+            np.testing.assert_allclose(tr[k, 5], row[5], rtol=6e-2)      # anorm %8.1e
+            np.testing.assert_allclose(tr[k, 6], row[6], rtol=6e-2)      # acond %8.1e
+    fin = dict(zip([str(k) for k in g["final_keys"]], g["final_vals"]))
+    assert res["itn"] == fin["itn"] and res["istop"] == fin["istop"]
+    for key in ("anorm", "acond", "xnorm", "rnorm"):
+        np.testing.assert_allclose(res[key], fin[key], rtol=6e-6)
+    np.testing.assert_allclose(res["bnorm"], fin["vnorm"], rtol=6e-6)     # printed as vnorm
+
+
+def test_oracle_zero_rhs(P, O, MG):
+    s = MG.make_system(P, MG.CASES["grid4_zero_rhs_itn5"])
+    x, res, tr = O.Oracle(s).lsqr(s.b, 5)
+    assert np.all(x == 0.0) and res["itn"] == 0 and res["istop"] == 0
+
+
+def test_spmv_dense_bruteforce_and_adjoint(P, O, MG):
+    for name in ("grid4_rays12_itn10", "grid7x5x3_rays40_itn20"):
+        s = MG.make_system(P, MG.CASES[name])
+        A = dense(s)
+        rng = np.random.default_rng(5)
+        o = O.Oracle(s)
+        x = rng.standard_normal(s.n)
+        y = rng.standard_normal(s.m_k + s.m_d)
+        y1 = y.copy()
+        o.aprod(1, x.copy(), y1)
+        np.testing.assert_allclose(y1, y + A @ x, rtol=1e-13, atol=1e-13)
+        x2 = x.copy()
+        o.aprod(2, x2, y.copy())
+        np.testing.assert_allclose(x2, x + A.T @ y, rtol=1e-13, atol=1e-13)
+        # adjoint identity <A x, y> = <x, A^T y>
+        ax = np.zeros_like(y)
+        o.aprod(1, x.copy(), ax)
+        aty = np.zeros_like(x)
+        o.aprod(2, aty, y.copy())
+        assert abs(ax @ y - x @ aty) <= 1e-12 * (abs(ax @ y) + 1)
+
+
+def test_planted_transpose_mistake_is_caught(P, O, MG):
+    """A transposed operand (using A instead of A^T in mode 2) fails the
+    brute-force check, so that check is able to pin mode 2."""
+    s = MG.make_system(P, MG.CASES["grid4_rays12_itn10"])
+    A = dense(s)
+    rng = np.random.default_rng(1)
+    y = rng.standard_normal(A.shape[0])
+    good = A.T @ y
+    # the planted variant: A[:n,:n]^T y[:n] style confusion of operands
+    wrong = (A[:, :]).T @ np.concatenate([y[s.m_k:], y[:s.m_k]])
+    x = np.zeros(s.n)
+    O.Oracle(s).aprod(2, x, y.copy())
+    np.testing.assert_allclose(x, good, rtol=1e-13, atol=1e-13)
+    assert rel(x, wrong) > 1e-3
+
+
+def test_dnrm2_closed_form_and_special_cases(O):
+    o = O.load()
+    orc = O.Oracle.__new__(O.Oracle)
+    orc.lib = o
+    rng = np.random.default_rng(3)
+    for n in (2, 3, 17, 1000):
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-5, 5)
+        ref = math.sqrt(math.fsum(float(t) * float(t) for t in v))
+        assert abs(orc.dnrm2(v) - ref) <= 4e-16 * n ** 0.5 * ref + 1e-300
+    assert orc.dnrm2(np.zeros(0)) == 0.0
+    assert orc.dnrm2(np.array([-3.5])) == 3.5
+    assert orc.dnrm2(np.zeros(7)) == 0.0
+    big = np.array([1e200, 1e200, 1e200])   # squares overflow; the scaled form does not
+    assert abs(orc.dnrm2(big) - math.sqrt(3) * 1e200) <= 1e-15 * 1e200 * 2
+    tiny = np.array([1e-200, 1e-200])
+    assert abs(orc.dnrm2(tiny) - math.sqrt(2) * 1e-200) <= 1e-15 * 1e-200 * 2
+    for a, b in ((3.0, 4.0), (0.0, 0.0), (-1e300, 1e300), (1e-310, 0.0), (2.5, -7.25)):
+        assert abs(orc.d2norm(a, b) - math.hypot(a, b)) <= 4e-16 * math.hypot(a, b)
+
+
+@pytest.mark.parametrize("name,itn,tol", [("grid4_rays12_itn10", 10, 1e-10),
+                                          ("grid7x5x3_rays40_itn20", 20, 1e-10),
+                                          ("grid16_rays20k_itn100", 10, 1e-8)])
+def test_oracle_lsqr_matches_scipy(P, O, MG, name, itn, tol):
+    """Same Golub-Kahan recurrences as scipy's LSQR (atol=btol=0, no conlim stop).
+    scipy orders its norms and products differently, so the two drift apart by
+    rounding (about 2e-10 after 10 iterations on the 16^3 system); a dropped
+    term or sign would differ at O(1)."""
+    s = MG.make_system(P, MG.CASES[name])
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    rhs = np.concatenate([s.b, np.zeros(s.m_d)])
+    ref = spla.lsqr(A, rhs, atol=0.0, btol=0.0, conlim=0.0, iter_lim=itn)
+    x, res, tr = O.Oracle(s).lsqr(s.b, itn)
+    assert ref[2] == itn
+    assert rel(x, ref[0]) < tol
+    np.testing.assert_allclose(res["rnorm"], ref[3], rtol=tol)
+    np.testing.assert_allclose(res["anorm"], ref[5], rtol=tol)
+    np.testing.assert_allclose(res["xnorm"], ref[8], rtol=tol)
+
+
+def test_oracle_lsqr_invariants(P, O, MG):
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    x, res, tr = O.Oracle(s).lsqr(s.b, 60)
+    rn = tr[:, 2]
+    assert np.all(np.diff(rn) <= 1e-12 * rn[0])             # rnorm never increases
+    xn = tr[1:, 7]
+    assert np.all(np.diff(xn) >= -1e-12 * xn[-1])           # xnorm never decreases
+    # u, v orthonormal early in the bidiagonalisation
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    r = np.concatenate([s.b, np.zeros(s.m_d)]) - A @ x
+    np.testing.assert_allclose(np.linalg.norm(r), res["rnorm"], rtol=1e-8)
+    # test2 as ||A^T r|| / (||A|| ||r||) with the estimate ||A|| <= ||A||_F
+    assert np.linalg.norm(A.T @ r) / (res["anorm"] * res["rnorm"]) == pytest.approx(res["arnorm"], rel=1e-3)
+
+
+def test_oracle_damped_lsqr_matches_scipy(P, O, MG):
+    """The damp > 0 branch (lsqr.c:609-616), which the reference never takes
+    (it passes damp = 0), against scipy's damped LSQR."""
+    s = MG.make_system(P, MG.CASES["grid7x5x3_rays40_itn20"])
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    rhs = np.concatenate([s.b, np.zeros(s.m_d)])
+    ref = spla.lsqr(A, rhs, damp=0.7, atol=0.0, btol=0.0, conlim=0.0, iter_lim=15)
+    x, res, tr = O.Oracle(s).lsqr(s.b, 15, damp=0.7)
+    assert rel(x, ref[0]) < 1e-10
+    # with damping, rnorm is the norm of the damped residual (scipy's r2norm)
+    np.testing.assert_allclose(res["rnorm"], ref[4], rtol=1e-9)
