@@ -156,6 +156,11 @@ def main():
     ap.add_argument("--aprod-reps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--aprod-only", action="store_true",
+                    help="only time aprod launches and a device copy of known size "
+                         "(the workload for rocprofv3 counter passes)")
+    ap.add_argument("--exact-order", action="store_true",
+                    help="operator summing every row/column in the reference's order")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -169,7 +174,7 @@ def main():
     print(f"[bench] generated {a.workload}: m_k={s.m_k} m_d={s.m_d} n={s.n} nnz_k={s.nnz_k} "
           f"nnz_d={s.nnz_d} ({gen_s:.1f} s)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    op = P.Operator(s)
+    op = P.Operator(s, P.OP_EXACT_ORDER if a.exact_order else P.OP_DEFAULT)
     setup_s = time.perf_counter() - t0
     print(f"[bench] operator on device ({setup_s:.1f} s, {op.device_bytes / 2**30:.2f} GiB)",
           file=sys.stderr, flush=True)
@@ -181,6 +186,21 @@ def main():
     hip.h2d(x_dev, np.random.default_rng(0).standard_normal(s.n))
     hip.h2d(y_dev, np.random.default_rng(1).standard_normal(s.m_k + s.m_d) * 1e-3)
 
+    if a.aprod_only:
+        # a plain device-to-device copy of 1 GiB calibrates the byte counters
+        # (FETCH_SIZE and WRITE_SIZE should read 1 GiB each for it)
+        cp_src, cp_dst = hip.malloc(1 << 30), hip.malloc(1 << 30)
+        hip.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        hip.check(hip.lib.hipMemcpy(C.c_void_p(cp_dst), C.c_void_p(cp_src), 1 << 30, 3), "hipMemcpy")
+        hip.sync()
+        ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
+        ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
+        b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
+        print(json.dumps({"aprod_only": True, "mode1_ms": ms1, "mode2_ms": ms2,
+                          "mode1_bytes": b1, "mode2_bytes": b2}), flush=True)
+        hip.free(cp_src)
+        hip.free(cp_dst)
+        return
     for _ in range(a.warmup):
         op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec)
     hip.sync()
@@ -229,7 +249,8 @@ def main():
         "config": {"workload": a.workload, "grid": [67, 67, 67] if "67" in a.workload else None,
                    "rays": s.m_k, "nnz_k": s.nnz_k, "m_d": s.m_d, "nnz_d": s.nnz_d,
                    "precision": a.precision, "tol": a.tol, "iterations_to_tol": iters,
-                   "tol_reached": bool(reached), "parallelism": "single GPU"},
+                   "tol_reached": bool(reached), "parallelism": "single GPU",
+                   "operator": "exact-order" if a.exact_order else "default"},
         "ms_per_iteration": round(infos[-1]["elapsed_ms"] / max(iters, 1), 4),
         "aprod_gbs": {"mode1": round(gbs1, 1), "mode2": round(gbs2, 1), "both": round(gbs, 1)},
         "roofline": roofline,

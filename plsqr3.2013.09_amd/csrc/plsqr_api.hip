@@ -631,7 +631,7 @@ void plsqr_aprod(int mode, int m, int n, double x[], double y[], void *UsrWrk) {
 // ---------------------------------------------------------------------------
 // device-resident LSQR
 // ---------------------------------------------------------------------------
-static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se) {
+static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, double *trace_d) {
     using namespace plsqr;
     int rc;
     if (op->exact) {
@@ -656,7 +656,7 @@ static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se) {
         hipLaunchKernelGGL((k_update<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v,
                            w, x, se, (double *)nullptr);
         LAUNCH_CHECK(op);
-        hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0, op->trace_d);
+        hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, 0, trace_d);
         LAUNCH_CHECK(op);
         return PLSQR_OK;
     }
@@ -669,7 +669,7 @@ static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se) {
     hipLaunchKernelGGL((k_update<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v, w,
                        x, se, op->part);
     LAUNCH_CHECK(op);
-    hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G6, op->trace_d);
+    hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G6, trace_d);
     LAUNCH_CHECK(op);
     return PLSQR_OK;
 }
@@ -751,7 +751,7 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
         hipGraph_t graph;
         hipGraphExec_t exec;
         HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_iteration(op, x, w, se);
+        rc = enqueue_iteration(op, x, w, se, trace_d);
         hipGraph_t g2 = nullptr;
         hipError_t ce = hipStreamEndCapture(op->stream, &g2);
         if (rc) return rc;
@@ -772,7 +772,7 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
         (void)hipGraphDestroy(graph);
     } else {
         for (int it = 0; it < itn; ++it) {
-            if ((rc = enqueue_iteration(op, x, w, se))) return rc;
+            if ((rc = enqueue_iteration(op, x, w, se, trace_d))) return rc;
             if (check_every && (it + 1) % check_every == 0) {
                 HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
                 HIP_TRY(op, hipStreamSynchronize(op->stream));
