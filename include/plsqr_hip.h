@@ -169,6 +169,14 @@ enum {
  * initPPDataV3_TRI (tridiagonal.c:33-57).  Replaces the GPU set-up the
  * reference calls InitPPDataV3_TRI_GPU (lsqr.c:451-457). */
 int  plsqr_op_create(const plsqr_system *sys, int flags, plsqr_op **out);
+
+/* Grid the columns come from (column = iz*ny*nx + iy*nx + ix).  With it the
+ * default operator renumbers the columns into bricks of 4 x 4 x 8 voxels so
+ * that one ray crossing a brick is gathered once for all the brick's voxels
+ * in x += A^T y.  All vectors crossing this API stay in the caller's order. */
+typedef struct plsqr_grid { int nx, ny, nz; } plsqr_grid;
+int  plsqr_op_create_ex(const plsqr_system *sys, int flags, const plsqr_grid *grid,
+                        plsqr_op **out);
 void plsqr_op_destroy(plsqr_op *op);
 const char *plsqr_last_error(const plsqr_op *op);
 void plsqr_clear_error(plsqr_op *op);

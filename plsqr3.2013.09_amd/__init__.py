@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    CONFIGS, ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT,
+    GridC, CONFIGS, ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT,
     OP_EXACT_ORDER, OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH,
     PlsqrError, SolveOptsC, SolveOutC, System, dptr, gen_rhs, generate, generate_config, load,
     read_x, write_x, _check,
@@ -26,10 +26,18 @@ from ._lib import (  # noqa: F401
 class Operator:
     """Device-resident operator A = [A_k ; D] of one system (plsqr_op)."""
 
-    def __init__(self, system, flags=OP_DEFAULT):
+    def __init__(self, system, flags=OP_DEFAULT, grid="auto"):
+        """grid: (nx, ny, nz) of the voxel grid, None for none, "auto" = system.grid."""
         lib = load()
         h = C.c_void_p()
-        rc = lib.plsqr_op_create(C.byref(system.c), int(flags), C.byref(h))
+        if grid == "auto":
+            grid = getattr(system, "grid", None)
+        g = None
+        if grid is not None:
+            g = GridC()
+            g.nx, g.ny, g.nz = (int(v) for v in grid)
+        rc = lib.plsqr_op_create_ex(C.byref(system.c), int(flags), C.byref(g) if g else None,
+                                    C.byref(h))
         if rc != 0:
             msg = lib.plsqr_last_error(h) if h.value else lib.plsqr_last_global_error()
             if h.value:

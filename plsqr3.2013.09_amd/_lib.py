@@ -61,6 +61,10 @@ class GenParamsC(C.Structure):
     ]
 
 
+class GridC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+
 class SolveOptsC(C.Structure):
     _fields_ = [
         ("itn", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
@@ -97,6 +101,8 @@ SIGNATURES = {
     "plsqr_write_x": (C.c_int, [C.c_char_p, c_double_p, C.c_int]),
     "plsqr_read_x": (C.c_int, [C.c_char_p, c_double_p, C.c_int]),
     "plsqr_op_create": (C.c_int, [C.POINTER(SystemC), C.c_int, C.POINTER(C.c_void_p)]),
+    "plsqr_op_create_ex": (C.c_int, [C.POINTER(SystemC), C.c_int, C.POINTER(GridC),
+                                     C.POINTER(C.c_void_p)]),
     "plsqr_op_destroy": (None, [C.c_void_p]),
     "plsqr_last_error": (C.c_char_p, [C.c_void_p]),
     "plsqr_clear_error": (None, [C.c_void_p]),
@@ -175,13 +181,14 @@ def _view(ptr, n, dtype, owner=None):
 class System:
     """A system in the reference's loaded layout (kernel CSC, damping CSR and CSC, b)."""
 
-    def __init__(self, c_struct, keep=None):
+    def __init__(self, c_struct, keep=None, grid=None):
         self.c = c_struct
         self._keep = keep  # numpy arrays backing a caller-built struct
+        self.grid = grid   # (nx, ny, nz) of the voxel grid the columns come from, if known
 
     @classmethod
     def from_arrays(cls, m_k, m_d, n, k_col_ptr, k_row_idx, k_val, d_row_ptr, d_col_idx, d_val,
-                    dc_col_ptr, dc_row_idx, dc_val, b, x_true=None):
+                    dc_col_ptr, dc_row_idx, dc_val, b, x_true=None, grid=None):
         # copies: the inputs may be views of another System's library-owned memory
         arrs = [np.array(a, dtype=t, copy=True, order="C") for a, t in (
             (k_col_ptr, np.int32), (k_row_idx, np.int32), (k_val, np.float64),
@@ -197,7 +204,7 @@ class System:
         s.dc_col_ptr, s.dc_row_idx, s.dc_val = iptr(arrs[6]), iptr(arrs[7]), dptr(arrs[8])
         s.b = dptr(arrs[9])
         s.x_true = dptr(xt) if xt is not None else None
-        return cls(s, keep=(arrs, xt))
+        return cls(s, keep=(arrs, xt), grid=grid)
 
     def __del__(self):
         try:
@@ -263,7 +270,7 @@ def generate(nx, ny, nz, n_rays, geometry, seed, sigma, lam, w=1.0, n_blobs=5, b
     s = SystemC()
     p = gen_params(nx, ny, nz, n_rays, geometry, seed, sigma, lam, w, n_blobs, blob_width)
     _check(lib.plsqr_gen_system(C.byref(p), C.byref(s)))
-    return System(s)
+    return System(s, grid=(int(nx), int(ny), int(nz)))
 
 
 def gen_rhs(system, nx, ny, nz, seed, sigma, n_blobs=5, blob_width=3.0):

@@ -37,7 +37,7 @@ def build_lib(out=LIB):
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
     for src, extra in (("plsqr_api.hip", DEVICE), ("plsqr_mixed.hip", DEVICE),
-                       ("tomo_io.cpp", [])):
+                       ("tomo_io.cpp", []), ("tile_build.cpp", [])):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -14,6 +15,7 @@
 #include "plsqr_hip.h"
 #include "plsqr_kernels.h"
 #include "plsqr_spmv.h"
+#include "tile_build.h"
 
 using plsqr::Scal;
 
@@ -49,6 +51,19 @@ struct plsqr_op {
     double *trace_d = nullptr;
     int trace_rows = 0;
     double *hx = nullptr, *hy = nullptr;  // staging for host-pointer aprod (n, m)
+    // column renumbering (bricks of voxels); every column-space vector on the
+    // device is in the new order, the API boundary converts
+    bool permuted = false;
+    int *perm = nullptr, *inv = nullptr;  // old -> new, new -> old
+    double *hx2 = nullptr;                // n, new-order staging
+    // tiled mode-2 layout (tile_build.h)
+    bool tiled = false;
+    int tS = 1, tT = 0, tmax_seg = 0;
+    int *t_start = nullptr, *t_ray = nullptr;
+    long long *t_slot = nullptr, *t_seg = nullptr;
+    uint16_t *t_vlen = nullptr, *t_sseg = nullptr;
+    double *t_sval = nullptr;
+    long long t_nslot = 0, t_nseg = 0;
     std::vector<void *> allocs;
     size_t bytes = 0;
     // launch shapes
@@ -133,8 +148,21 @@ int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double
     return PLSQR_OK;
 }
 
+constexpr int TILE_W = 128;
+constexpr int TILE_CAP = 2048;
+
 int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
     using namespace plsqr;
+    if (op->tiled) {
+        hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
+                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval,
+                           op->t_sseg, u, sc, op->P);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
+                           op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
     dim3 g(op->G2), b(BS);
     switch (op->L2) {
         case 2: hipLaunchKernelGGL((k_cols_sliced<2, BS>), g, b, 0, op->stream, op->n, op->nsl, op->scp, op->sri, op->sv, u, sc, op->P); break;
@@ -173,6 +201,32 @@ int aprod2_dev(plsqr_op *op, double *x, const double *y) {
         return PLSQR_OK;
     }
     return launch_cols(op, y, x, nullptr, nullptr);
+}
+
+// Column renumbering at the API boundary: callers see the original column
+// order, the device works in the renumbered one.
+int to_internal(plsqr_op *op, const double *ext, double *in) {
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                       op->inv, ext, in);
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+int to_external(plsqr_op *op, const double *in, double *ext) {
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                       op->perm, in, ext);
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+// aprod on a caller-order x (device): renumber in, compute, renumber back
+int aprod_ext_dev(plsqr_op *op, int mode, double *x, double *y) {
+    int rc;
+    if (!op->permuted) return mode == 1 ? aprod1_dev(op, x, y) : aprod2_dev(op, x, y);
+    if ((rc = to_internal(op, x, op->hx2))) return rc;
+    if ((rc = mode == 1 ? aprod1_dev(op, op->hx2, y) : aprod2_dev(op, op->hx2, y))) return rc;
+    if (mode == 2) return to_external(op, op->hx2, x);
+    return PLSQR_OK;
 }
 
 // -------------------------------------------------------------------------
@@ -267,7 +321,7 @@ int exclusive_scan(plsqr_op *op, const int *cnt, int *out, long long len) {
     return PLSQR_OK;
 }
 
-int build(plsqr_op *op, const plsqr_system *s) {
+int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const plsqr::TileLayout *L) {
     using namespace plsqr;
     int rc;
     const long long nk = s->nnz_k, nd = s->nnz_d;
@@ -322,31 +376,76 @@ int build(plsqr_op *op, const plsqr_system *s) {
         dfree(op, dci, nd);
         dfree(op, dav, nd);
     }
+    // ---- column renumbering: CSR column indices to the new order
+    if (L) {
+        op->permuted = true;
+        if ((rc = dalloc(op, &op->perm, op->n)) || (rc = dalloc(op, &op->inv, op->n)) || (rc = dalloc(op, &op->hx2, op->n)))
+            return rc;
+        HIP_TRY(op, hipMemcpy(op->perm, L->perm.data(), sizeof(int) * op->n, hipMemcpyHostToDevice));
+        HIP_TRY(op, hipMemcpy(op->inv, L->inv.data(), sizeof(int) * op->n, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_remap, dim3(grid_for(nr, 256, gcap)), dim3(256), 0, op->stream, nr, op->perm, op->ci);
+        LAUNCH_CHECK(op);
+    }
     // ---- damping CSC
     if ((rc = dalloc(op, &op->dcp, (size_t)op->n + 1)) || (rc = dalloc(op, &op->dri, nd)) || (rc = dalloc(op, &op->dcv, nd)))
         return rc;
-    HIP_TRY(op, hipMemcpyAsync(op->dcp, s->dc_col_ptr, sizeof(int) * (op->n + 1), hipMemcpyHostToDevice, op->stream));
-    if (nd > 0) {
-        HIP_TRY(op, hipMemcpyAsync(op->dri, s->dc_row_idx, sizeof(int) * nd, hipMemcpyHostToDevice, op->stream));
-        HIP_TRY(op, hipMemcpyAsync(op->dcv, s->dc_val, sizeof(double) * nd, hipMemcpyHostToDevice, op->stream));
+    if (L) {
+        std::vector<int> cp((size_t)op->n + 1, 0), ri((size_t)std::max<long long>(nd, 1));
+        std::vector<double> cv((size_t)std::max<long long>(nd, 1));
+        for (int j = 0; j < op->n; ++j) {
+            const int o = L->inv[(size_t)j];
+            const int b = s->dc_col_ptr[o], e = s->dc_col_ptr[o + 1];
+            cp[(size_t)j + 1] = cp[(size_t)j] + (e - b);
+            std::copy(s->dc_row_idx + b, s->dc_row_idx + e, ri.begin() + cp[(size_t)j]);
+            std::copy(s->dc_val + b, s->dc_val + e, cv.begin() + cp[(size_t)j]);
+        }
+        HIP_TRY(op, hipMemcpy(op->dcp, cp.data(), sizeof(int) * (op->n + 1), hipMemcpyHostToDevice));
+        if (nd > 0) {
+            HIP_TRY(op, hipMemcpy(op->dri, ri.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+            HIP_TRY(op, hipMemcpy(op->dcv, cv.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+        }
+    } else {
+        HIP_TRY(op, hipMemcpyAsync(op->dcp, s->dc_col_ptr, sizeof(int) * (op->n + 1), hipMemcpyHostToDevice, op->stream));
+        if (nd > 0) {
+            HIP_TRY(op, hipMemcpyAsync(op->dri, s->dc_row_idx, sizeof(int) * nd, hipMemcpyHostToDevice, op->stream));
+            HIP_TRY(op, hipMemcpyAsync(op->dcv, s->dc_val, sizeof(double) * nd, hipMemcpyHostToDevice, op->stream));
+        }
+    }
+    // ---- tiled mode-2 layout
+    if (L) {
+        op->tiled = true;
+        op->tS = L->S;
+        op->tT = L->T;
+        op->nsl = L->S;
+        op->tmax_seg = L->max_seg_per_item;
+        op->t_nslot = L->n_slot;
+        op->t_nseg = L->n_seg;
+        const long long items = (long long)L->T * L->S;
+        if ((rc = dalloc(op, &op->t_start, (size_t)L->T + 1)) || (rc = dalloc(op, &op->t_slot, (size_t)items + 1)) ||
+            (rc = dalloc(op, &op->t_seg, (size_t)items + 1)) || (rc = dalloc(op, &op->t_vlen, (size_t)items * L->TW)) ||
+            (rc = dalloc(op, &op->t_ray, (size_t)L->n_seg)) || (rc = dalloc(op, &op->t_sval, (size_t)L->n_slot)) ||
+            (rc = dalloc(op, &op->t_sseg, (size_t)L->n_slot)))
+            return rc;
+        HIP_TRY(op, hipMemcpy(op->t_start, L->tile_start.data(), sizeof(int) * (L->T + 1), hipMemcpyHostToDevice));
+        HIP_TRY(op, hipMemcpy(op->t_slot, L->slot_base.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
+        HIP_TRY(op, hipMemcpy(op->t_seg, L->seg_start.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
+        HIP_TRY(op, hipMemcpy(op->t_vlen, L->vox_len.data(), sizeof(uint16_t) * items * L->TW, hipMemcpyHostToDevice));
+        if (L->n_seg > 0)
+            HIP_TRY(op, hipMemcpy(op->t_ray, L->seg_ray.data(), sizeof(int) * L->n_seg, hipMemcpyHostToDevice));
+        if (L->n_slot > 0) {
+            HIP_TRY(op, hipMemcpy(op->t_sval, L->sval.data(), sizeof(double) * L->n_slot, hipMemcpyHostToDevice));
+            HIP_TRY(op, hipMemcpy(op->t_sseg, L->sseg.data(), sizeof(uint16_t) * L->n_slot, hipMemcpyHostToDevice));
+        }
+        dfree(op, col, nk);
+        dfree(op, op->kcp, (size_t)op->n + 1);
+        dfree(op, op->kri, nk);
+        dfree(op, op->kv, nk);
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        return PLSQR_OK;
     }
     // ---- row slices of the kernel CSC, balanced by nonzeros
-    int nsl = env_int("PLSQR_SLICES", 8);
-    if (nsl < 1) nsl = 1;
-    if (nsl > 64) nsl = 64;
-    if (op->m_k < nsl * 64) nsl = 1;
+    const int nsl = (int)hb.size() - 1;
     op->nsl = nsl;
-    std::vector<int> hb(nsl + 1, 0);
-    {
-        std::vector<int> hrp((size_t)op->m_k + 1);
-        HIP_TRY(op, hipMemcpy(hrp.data(), op->rp, sizeof(int) * (op->m_k + 1), hipMemcpyDeviceToHost));
-        for (int sl = 1; sl < nsl; ++sl) {
-            const long long target = (long long)nk * sl / nsl;
-            hb[sl] = (int)(std::lower_bound(hrp.begin(), hrp.end(), (int)target) - hrp.begin());
-            hb[sl] = std::max(hb[sl], hb[sl - 1]);
-        }
-        hb[nsl] = op->m_k;
-    }
     if ((rc = dalloc(op, &op->bounds, (size_t)nsl + 1))) return rc;
     HIP_TRY(op, hipMemcpy(op->bounds, hb.data(), sizeof(int) * (nsl + 1), hipMemcpyHostToDevice));
     if (nsl == 1) {
@@ -391,6 +490,21 @@ int build(plsqr_op *op, const plsqr_system *s) {
     return PLSQR_OK;
 }
 
+// Row slice boundaries balanced by kernel nonzeros (per-row counts from the CSC).
+std::vector<int> slice_bounds_host(const plsqr_system *s, int nsl) {
+    std::vector<long long> cnt((size_t)s->m_k + 1, 0);
+    for (long long e = 0; e < s->nnz_k; ++e) cnt[(size_t)s->k_row_idx[e] + 1]++;
+    for (int r = 0; r < s->m_k; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
+    std::vector<int> hb((size_t)nsl + 1, 0);
+    for (int sl = 1; sl < nsl; ++sl) {
+        const long long target = s->nnz_k * sl / nsl;
+        hb[(size_t)sl] = (int)(std::lower_bound(cnt.begin(), cnt.end(), target) - cnt.begin());
+        hb[(size_t)sl] = std::min(std::max(hb[(size_t)sl], hb[(size_t)sl - 1]), s->m_k);
+    }
+    hb[(size_t)nsl] = s->m_k;
+    return hb;
+}
+
 int ensure_trace(plsqr_op *op, int rows) {
     if (rows <= op->trace_rows) return PLSQR_OK;
     dfree(op, op->trace_d, (size_t)op->trace_rows * 8);
@@ -425,14 +539,15 @@ void plsqr_clear_error(plsqr_op *op) {
 }
 
 int plsqr_op_create(const plsqr_system *s, int flags, plsqr_op **out) {
+    return plsqr_op_create_ex(s, flags, nullptr, out);
+}
+
+int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid, plsqr_op **out) {
     if (!out) return PLSQR_ERR_ARG;
     *out = nullptr;
-    plsqr_op tmp;
     if (!s) return PLSQR_ERR_ARG;
-    if (s->m_k <= 0 || s->n <= 0 || s->m_d < 0)
-        return PLSQR_ERR_ARG;
-    if (s->nnz_k < 0 || s->nnz_d < 0 || s->nnz_k + s->nnz_d >= (1LL << 31) - 1)
-        return PLSQR_ERR_ARG;
+    if (s->m_k <= 0 || s->n <= 0 || s->m_d < 0) return PLSQR_ERR_ARG;
+    if (s->nnz_k < 0 || s->nnz_d < 0 || s->nnz_k + s->nnz_d >= (1LL << 31) - 1) return PLSQR_ERR_ARG;
     plsqr_op *op = new plsqr_op();
     op->flags = flags;
     op->exact = (flags & PLSQR_OP_EXACT_ORDER) != 0;
@@ -466,6 +581,51 @@ int plsqr_op_create(const plsqr_system *s, int flags, plsqr_op **out) {
     } else if (s->nnz_d != 0 || !s->d_row_ptr || !s->dc_col_ptr) {
         return fail(set_err(op, PLSQR_ERR_ARG, "empty damping needs nnz_d == 0 and zero pointer arrays"));
     }
+    if (grid && (long long)grid->nx * grid->ny * grid->nz != s->n)
+        return fail(set_err(op, PLSQR_ERR_ARG, "grid dimensions do not multiply to the number of columns"));
+    // row slices of A_k, balanced by nonzeros (one per XCD)
+    int nsl = env_int("PLSQR_SLICES", 8);
+    nsl = std::max(1, std::min(nsl, 64));
+    if (s->m_k < nsl * 64) nsl = 1;
+    std::vector<int> hb = slice_bounds_host(s, nsl);
+    // tiled mode-2 layout on bricks of voxels (default operator only)
+    std::unique_ptr<plsqr::TileLayout> L;
+    if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
+        std::vector<int> perm, tiles;
+        if (grid) {
+            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, perm, tiles);
+        } else {
+            perm.resize((size_t)s->n);
+            for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
+            for (int c = 0; c < s->n; c += TILE_W) tiles.push_back(c);
+            tiles.push_back(s->n);
+        }
+        L.reset(new plsqr::TileLayout());
+        std::string why;
+        if (!plsqr::build_tile_layout(s, perm, tiles, hb, TILE_W, L.get(), &why)) {
+            L.reset();  // falls back to the sliced CSC
+        } else if (env_int("PLSQR_TILE_CHECK", 0) != 0) {
+            // compare the layout's per-slice column sums with the CSC's on the host
+            std::vector<double> u((size_t)s->m_k), Pt((size_t)L->S * s->n), Pr((size_t)L->S * s->n, 0.0);
+            for (int i = 0; i < s->m_k; ++i) u[(size_t)i] = std::sin(0.37 * i + 1.0);
+            plsqr::tile_layout_apply(*L, s->n, u.data(), Pt.data());
+            for (int o = 0; o < s->n; ++o) {
+                int sl = 0;
+                for (int e = s->k_col_ptr[o]; e < s->k_col_ptr[o + 1]; ++e) {
+                    const int r = s->k_row_idx[e];
+                    while (r >= hb[(size_t)sl + 1]) ++sl;
+                    Pr[(size_t)sl * s->n + L->perm[(size_t)o]] += s->k_val[e] * u[(size_t)r];
+                }
+            }
+            double dmax = 0.0, nrm = 0.0;
+            for (size_t i = 0; i < Pt.size(); ++i) {
+                dmax = std::max(dmax, std::fabs(Pt[i] - Pr[i]));
+                nrm = std::max(nrm, std::fabs(Pr[i]));
+            }
+            if (!(dmax <= 1e-12 * (nrm + 1.0)))
+                return fail(set_err(op, PLSQR_ERR_STATE, "tiled layout self-check failed: max diff " + std::to_string(dmax)));
+        }
+    }
     hipError_t e = hipGetDevice(&op->dev);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
     hipDeviceProp_t prop;
@@ -476,7 +636,8 @@ int plsqr_op_create(const plsqr_system *s, int flags, plsqr_op **out) {
     e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     op->own_stream = true;
-    if ((rc = build(op, s))) return fail(rc);
+    if ((rc = build(op, s, hb, L.get()))) return fail(rc);
+    L.reset();
     // vectors and scratch
     op->L1 = env_int("PLSQR_L1", 16);
     op->L2 = env_int("PLSQR_L2", 16);
@@ -538,7 +699,14 @@ int plsqr_op_set_vectors(plsqr_op *op, const double *u_host, const double *v_hos
     if (!op) return PLSQR_ERR_ARG;
     if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
     if (u_host) HIP_TRY(op, hipMemcpyAsync(op->u, u_host, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
-    if (v_host) HIP_TRY(op, hipMemcpyAsync(op->v, v_host, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    if (v_host) {
+        HIP_TRY(op, hipMemcpyAsync(op->permuted ? op->hx : op->v, v_host, sizeof(double) * op->n,
+                                   hipMemcpyHostToDevice, op->stream));
+        if (op->permuted) {
+            int rc = to_internal(op, op->hx, op->v);
+            if (rc) return rc;
+        }
+    }
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     return PLSQR_OK;
 }
@@ -547,7 +715,14 @@ int plsqr_op_get_vectors(plsqr_op *op, double *u_host, double *v_host) {
     if (!op) return PLSQR_ERR_ARG;
     if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
     if (u_host) HIP_TRY(op, hipMemcpyAsync(u_host, op->u, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
-    if (v_host) HIP_TRY(op, hipMemcpyAsync(v_host, op->v, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    if (v_host) {
+        if (op->permuted) {
+            int rc = to_external(op, op->v, op->hx);
+            if (rc) return rc;
+        }
+        HIP_TRY(op, hipMemcpyAsync(v_host, op->permuted ? op->hx : op->v, sizeof(double) * op->n,
+                                   hipMemcpyDeviceToHost, op->stream));
+    }
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     return PLSQR_OK;
 }
@@ -556,9 +731,8 @@ int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
     if (!op) return PLSQR_ERR_ARG;
     if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
     if (!x_dev || !y_dev) return set_err(op, PLSQR_ERR_ARG, "device pointers required");
-    if (mode == 1) return aprod1_dev(op, x_dev, y_dev);
-    if (mode == 2) return aprod2_dev(op, x_dev, y_dev);
-    return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    return aprod_ext_dev(op, mode, x_dev, y_dev);
 }
 
 int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_d) {
@@ -570,7 +744,7 @@ int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_
     HIP_TRY(op, hipMemcpyAsync(op->hy, y_k, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
     if (op->m_d > 0)
         HIP_TRY(op, hipMemcpyAsync(op->hy + op->m_k, y_d, sizeof(double) * op->m_d, hipMemcpyHostToDevice, op->stream));
-    int rc = mode == 1 ? aprod1_dev(op, op->hx, op->hy) : aprod2_dev(op, op->hx, op->hy);
+    int rc = aprod_ext_dev(op, mode, op->hx, op->hy);
     if (rc) return rc;
     if (mode == 1) {
         HIP_TRY(op, hipMemcpyAsync(y_k, op->hy, sizeof(double) * op->m_k, hipMemcpyDeviceToHost, op->stream));
@@ -592,15 +766,18 @@ static int aprod_host(plsqr_op *op, int mode, int m, int n, double *x, double *y
                                               " do not match the handle (m=" + std::to_string(op->m) +
                                               " n=" + std::to_string(op->n) + ")");
     double *xd = op->v, *yd = op->u;
-    if (x) {
-        HIP_TRY(op, hipMemcpyAsync(op->hx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
-        xd = op->hx;
-    }
     if (y) {
         HIP_TRY(op, hipMemcpyAsync(op->hy, y, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
         yd = op->hy;
     }
-    int rc = mode == 1 ? aprod1_dev(op, xd, yd) : aprod2_dev(op, xd, yd);
+    int rc;
+    if (x) {
+        HIP_TRY(op, hipMemcpyAsync(op->hx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+        xd = op->hx;
+        rc = aprod_ext_dev(op, mode, xd, yd);  // caller order
+    } else {
+        rc = mode == 1 ? aprod1_dev(op, xd, yd) : aprod2_dev(op, xd, yd);  // resident v, device order
+    }
     if (rc) return rc;
     if (mode == 1 && y) HIP_TRY(op, hipMemcpyAsync(y, yd, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
     if (mode == 2 && x) HIP_TRY(op, hipMemcpyAsync(x, xd, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
@@ -826,7 +1003,20 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
     plsqr_solve_opts o;
     default_opts(opts, &o);
     if (o.precision != PLSQR_PREC_FP64) return set_err(op, PLSQR_ERR_ARG, "precision not supported by this build");
-    return solve_fp64(op, b_dev, &o, x_dev, w_dev ? w_dev : op->w, out, trace);
+    // the loop works on the handle's x and w (device column order)
+    int rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, trace);
+    if (rc) return rc;
+    if (op->permuted) {
+        if ((rc = to_external(op, op->x, x_dev))) return rc;
+        if (w_dev && (rc = to_external(op, op->w, w_dev))) return rc;
+    } else {
+        if (x_dev != op->x)
+            HIP_TRY(op, hipMemcpyAsync(x_dev, op->x, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+        if (w_dev && w_dev != op->w)
+            HIP_TRY(op, hipMemcpyAsync(w_dev, op->w, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    }
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
 }
 
 int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts *opts, double *x_host,
@@ -837,11 +1027,19 @@ int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts 
     for (int i = 0; i < op->m_k; ++i)
         if (!std::isfinite(b_host[i])) return set_err(op, PLSQR_ERR_NONFINITE, "b holds a NaN or Inf at " + std::to_string(i));
     HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
-    int rc = plsqr_lsqr_solve_device(op, op->bst, opts, op->x, op->w, out, trace);
+    int rc = plsqr_lsqr_solve_device(op, op->bst, opts, op->hx, nullptr, out, trace);
     if (rc) return rc;
-    HIP_TRY(op, hipMemcpyAsync(x_host, op->x, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
-    if (w_host) HIP_TRY(op, hipMemcpyAsync(w_host, op->w, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(x_host, op->hx, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
     HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if (w_host) {
+        if (op->permuted) {
+            if ((rc = to_external(op, op->w, op->hx))) return rc;
+            HIP_TRY(op, hipMemcpyAsync(w_host, op->hx, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+        } else {
+            HIP_TRY(op, hipMemcpyAsync(w_host, op->w, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+        }
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+    }
     return PLSQR_OK;
 }
 
@@ -853,8 +1051,14 @@ int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_hos
     if (!b_host || !x_host) return set_err(op, PLSQR_ERR_ARG, "b and x are required");
     HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
     HIP_TRY(op, hipMemcpyAsync(op->hx, x_host, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    const double *xin = op->hx;
+    if (op->permuted) {
+        int rc0 = to_internal(op, op->hx, op->hx2);
+        if (rc0) return rc0;
+        xin = op->hx2;
+    }
     const int g = grid_for(op->m, 256, op->cu * 4);
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, op->hx,
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, xin,
                        op->bst, op->hy);
     LAUNCH_CHECK(op);
     std::vector<double> hp((size_t)op->part_len);

@@ -165,6 +165,78 @@ __global__ __launch_bounds__(BS) void k_cols_sliced(int n, int nsl, const int *_
     }
 }
 
+// Tiled mode 2.  Workgroup = one item (tile t, row slice s), item = t*S + s,
+// so consecutive workgroups cover the S slices of a tile and workgroup b runs
+// on XCD b % 8: with S = 8 each XCD only gathers rows of its own slice of u,
+// which stays in its L2.  Phase 1 gathers fl(u[row]*ib) once per segment (a
+// ray crossing the tile) into LDS; phase 2 lets thread (w, v) sum the
+// nonzeros j = w, w+NW, ... of voxel v, reading the segment values from LDS;
+// the NW partial sums of a voxel are then added in a fixed order.
+// Segments beyond the LDS capacity are processed in rounds of CAP.
+template <int TW, int BS, int CAP>
+__global__ __launch_bounds__(BS) void k_cols_tiled(
+    int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+    const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
+    const int *__restrict__ seg_ray, const double *__restrict__ sval,
+    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
+    double *__restrict__ P) {
+    constexpr int NW = BS / TW;
+    __shared__ double g[CAP];
+    __shared__ double part[NW][TW];
+    double ib = 1.0;
+    if (sc) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+    }
+    const int it = blockIdx.x;
+    const int s = it % S;
+    const int t = it / S;
+    const int c0 = tile_start[t];
+    const int tw = tile_start[t + 1] - c0;
+    const int v = threadIdx.x % TW;
+    const int w = threadIdx.x / TW;
+    const long long base = slot_base[it];
+    const long long s0 = seg_start[it];
+    const int ns = (int)(seg_start[it + 1] - s0);
+    const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
+    double acc = 0.0;
+    int j = w;
+    for (int r0 = 0; r0 < ns; r0 += CAP) {
+        const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
+        for (int i = threadIdx.x; i < cnt; i += BS) g[i] = u[seg_ray[s0 + r0 + i]] * ib;
+        __syncthreads();
+        const int hi = r0 + cnt;
+        while (j < len) {
+            const long long slot = base + (long long)j * TW + v;
+            const int sg = sseg[slot];
+            if (sg >= hi) break;
+            acc += sval[slot] * g[sg - r0];
+            j += NW;
+        }
+        __syncthreads();
+    }
+    part[w][v] = acc;
+    __syncthreads();
+    if (w == 0 && v < tw) {
+        double tot = part[0][v];
+#pragma unroll
+        for (int k = 1; k < NW; ++k) tot += part[k][v];
+        P[(size_t)s * n + c0 + v] = tot;
+    }
+}
+
+// dst[i] = src[idx[i]]  (column renumbering at the API boundary)
+__global__ void k_gather(int n, const int *__restrict__ idx, const double *__restrict__ src,
+                         double *__restrict__ dst) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) dst[i] = src[idx[i]];
+}
+
+// a[i] = map[a[i]]
+__global__ void k_remap(long long len, const int *__restrict__ map, int *__restrict__ a) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        a[i] = map[a[i]];
+}
+
 // v[c] <- fl(v[c]*vscale) + (tk + td) with tk the slice partials summed in
 // slice order and td the damping column sum in row order
 // (cpuSpMVTranspose_CSC_v2, CPUSpMV.c:281-291); x += (tk + td) groups as

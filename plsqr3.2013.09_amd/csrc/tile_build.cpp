@@ -1,0 +1,179 @@
+// Host-side construction of the tiled mode-2 layout; see tile_build.h.
+#include "tile_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
+
+namespace plsqr {
+
+namespace {
+
+template <class F>
+void par_for(long long n, F &&f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("PLSQR_HOST_THREADS")) nt = (unsigned)std::max(1, atoi(e));
+    nt = std::max(1u, std::min(nt, 64u));
+    if (n < 64) nt = 1;
+    std::vector<std::thread> th;
+    const long long chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const long long b = (long long)t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// rows of old column o inside [r0, r1): index range in the CSC arrays
+inline void slice_range(const plsqr_system *s, int o, int r0, int r1, int *b, int *e) {
+    const int *rb = s->k_row_idx + s->k_col_ptr[o];
+    const int *re = s->k_row_idx + s->k_col_ptr[o + 1];
+    *b = (int)(std::lower_bound(rb, re, r0) - s->k_row_idx);
+    *e = (int)(std::lower_bound(rb, re, r1) - s->k_row_idx);
+}
+
+}  // namespace
+
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, std::vector<int> &perm,
+                 std::vector<int> &tile_start) {
+    const long long n = (long long)nx * ny * nz;
+    perm.assign((size_t)n, 0);
+    tile_start.clear();
+    int next = 0;
+    for (int z0 = 0; z0 < nz; z0 += bz)
+        for (int y0 = 0; y0 < ny; y0 += by)
+            for (int x0 = 0; x0 < nx; x0 += bx) {
+                tile_start.push_back(next);
+                for (int z = z0; z < std::min(nz, z0 + bz); ++z)
+                    for (int y = y0; y < std::min(ny, y0 + by); ++y)
+                        for (int x = x0; x < std::min(nx, x0 + bx); ++x)
+                            perm[(size_t)z * ny * nx + (size_t)y * nx + x] = next++;
+            }
+    tile_start.push_back(next);
+}
+
+bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
+                       const std::vector<int> &tile_start, const std::vector<int> &slice_bounds,
+                       int TW, TileLayout *L, std::string *err) {
+    const int n = s->n;
+    L->TW = TW;
+    L->perm = perm;
+    L->inv.assign((size_t)n, 0);
+    for (int o = 0; o < n; ++o) L->inv[(size_t)perm[(size_t)o]] = o;
+    L->tile_start = tile_start;
+    L->slice_bounds = slice_bounds;
+    L->T = (int)tile_start.size() - 1;
+    L->S = (int)slice_bounds.size() - 1;
+    const int T = L->T, S = L->S;
+    const long long items = (long long)T * S;
+    for (int t = 0; t < T; ++t)
+        if (tile_start[(size_t)t + 1] - tile_start[(size_t)t] > TW || tile_start[(size_t)t + 1] < tile_start[(size_t)t]) {
+            *err = "tile wider than the kernel's tile width";
+            return false;
+        }
+    L->vox_len.assign((size_t)items * TW, 0);
+    std::vector<int> nseg((size_t)items, 0), depth((size_t)items, 0);
+    std::vector<std::vector<int>> rows_of((size_t)items);
+    std::atomic<bool> ok{true};
+    // pass 1: segments (distinct rows) and voxel lengths per item
+    par_for(T, [&](long long tb, long long te) {
+        std::vector<int> buf;
+        for (long long t = tb; t < te; ++t) {
+            const int c0 = tile_start[(size_t)t], tw = tile_start[(size_t)t + 1] - c0;
+            for (int sl = 0; sl < S; ++sl) {
+                const long long it = t * S + sl;
+                buf.clear();
+                int dmax = 0;
+                for (int v = 0; v < tw; ++v) {
+                    int b, e;
+                    slice_range(s, L->inv[(size_t)(c0 + v)], slice_bounds[(size_t)sl], slice_bounds[(size_t)sl + 1], &b, &e);
+                    const int len = e - b;
+                    if (len > 65535) ok.store(false);
+                    L->vox_len[(size_t)it * TW + v] = (uint16_t)std::min(len, 65535);
+                    dmax = std::max(dmax, len);
+                    buf.insert(buf.end(), s->k_row_idx + b, s->k_row_idx + e);
+                }
+                std::sort(buf.begin(), buf.end());
+                buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+                nseg[(size_t)it] = (int)buf.size();
+                depth[(size_t)it] = dmax;
+                rows_of[(size_t)it] = buf;
+            }
+        }
+    });
+    if (!ok.load()) {
+        *err = "a voxel holds more than 65535 nonzeros in one slice";
+        return false;
+    }
+    L->seg_start.assign((size_t)items + 1, 0);
+    L->slot_base.assign((size_t)items + 1, 0);
+    int mx = 0;
+    for (long long it = 0; it < items; ++it) {
+        L->seg_start[(size_t)it + 1] = L->seg_start[(size_t)it] + nseg[(size_t)it];
+        L->slot_base[(size_t)it + 1] = L->slot_base[(size_t)it] + (long long)depth[(size_t)it] * TW;
+        mx = std::max(mx, nseg[(size_t)it]);
+    }
+    L->max_seg_per_item = mx;
+    if (mx > 65535) {
+        *err = "an item needs more than 65535 segments";
+        return false;
+    }
+    L->n_seg = L->seg_start[(size_t)items];
+    L->n_slot = L->slot_base[(size_t)items];
+    L->seg_ray.assign((size_t)L->n_seg, 0);
+    L->sval.assign((size_t)L->n_slot, 0.0);
+    L->sseg.assign((size_t)L->n_slot, 0);
+    // pass 2: fill segments and slots
+    par_for(T, [&](long long tb, long long te) {
+        for (long long t = tb; t < te; ++t) {
+            const int c0 = tile_start[(size_t)t], tw = tile_start[(size_t)t + 1] - c0;
+            for (int sl = 0; sl < S; ++sl) {
+                const long long it = t * S + sl;
+                const std::vector<int> &rows = rows_of[(size_t)it];
+                std::copy(rows.begin(), rows.end(), L->seg_ray.begin() + L->seg_start[(size_t)it]);
+                const long long base = L->slot_base[(size_t)it];
+                for (int v = 0; v < tw; ++v) {
+                    int b, e;
+                    slice_range(s, L->inv[(size_t)(c0 + v)], slice_bounds[(size_t)sl], slice_bounds[(size_t)sl + 1], &b, &e);
+                    for (int k = b; k < e; ++k) {
+                        const long long slot = base + (long long)(k - b) * TW + v;
+                        const int seg = (int)(std::lower_bound(rows.begin(), rows.end(), s->k_row_idx[k]) - rows.begin());
+                        L->sval[(size_t)slot] = s->k_val[k];
+                        L->sseg[(size_t)slot] = (uint16_t)seg;
+                    }
+                }
+                std::vector<int>().swap(rows_of[(size_t)it]);
+            }
+        }
+    });
+    return true;
+}
+
+void tile_layout_apply(const TileLayout &L, int n, const double *u, double *P) {
+    const int S = L.S, TW = L.TW;
+    par_for(L.T, [&](long long tb, long long te) {
+        std::vector<double> g;
+        for (long long t = tb; t < te; ++t) {
+            const int c0 = L.tile_start[(size_t)t], tw = L.tile_start[(size_t)t + 1] - c0;
+            for (int sl = 0; sl < S; ++sl) {
+                const long long it = t * S + sl;
+                const long long s0 = L.seg_start[(size_t)it], ns = L.seg_start[(size_t)it + 1] - s0;
+                g.resize((size_t)ns);
+                for (long long i = 0; i < ns; ++i) g[(size_t)i] = u[L.seg_ray[(size_t)(s0 + i)]];
+                for (int v = 0; v < tw; ++v) {
+                    double acc = 0.0;
+                    const int len = L.vox_len[(size_t)it * TW + v];
+                    for (int j = 0; j < len; ++j) {
+                        const long long slot = L.slot_base[(size_t)it] + (long long)j * TW + v;
+                        acc += L.sval[(size_t)slot] * g[L.sseg[(size_t)slot]];
+                    }
+                    P[(size_t)sl * n + c0 + v] = acc;
+                }
+            }
+        }
+    });
+}
+
+}  // namespace plsqr

@@ -1,0 +1,60 @@
+// Host-side construction of the tiled mode-2 layout (internal to libplsqr_hip.so).
+//
+// Columns are renumbered so that a tile of consecutive new column indices is
+// a compact brick of voxels (4 x 4 x 8 when the grid is known).  For every
+// (row slice s, tile t) "item" the kernel gathers u once per ray that crosses
+// the tile (a segment) and reuses it for every voxel of the tile the ray
+// touches.
+//
+// Per item: the segments' rows (seg_ray), and the tile's nonzeros in "slots":
+// slot (j, v) = slot_base[item] + j * TW + v holds the j-th nonzero (rows
+// ascending) of the tile's voxel v within slice s, as (value, local segment
+// index).  vox_len[item * TW + v] nonzeros per voxel; slots past vox_len are
+// padding that the kernel never reads.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "plsqr_hip.h"
+
+namespace plsqr {
+
+struct TileLayout {
+    int S = 1;   // row slices
+    int T = 0;   // tiles
+    int TW = 128;
+    std::vector<int> perm;        // perm[old column] = new column
+    std::vector<int> inv;         // inv[new column] = old column
+    std::vector<int> tile_start;  // T+1, in new column numbering
+    std::vector<int> slice_bounds;  // S+1 kernel rows
+    std::vector<long long> slot_base;  // items+1 (item = t*S + s)
+    std::vector<long long> seg_start;  // items+1
+    std::vector<uint16_t> vox_len;     // items * TW
+    std::vector<int> seg_ray;          // segments
+    std::vector<double> sval;          // slots
+    std::vector<uint16_t> sseg;        // slots
+    long long n_seg = 0, n_slot = 0;
+    int max_seg_per_item = 0;
+};
+
+// Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
+// voxels in brick-major order, voxels inside a brick x fastest.  Fills
+// perm (old -> new) and the tile starts (one tile per brick).
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, std::vector<int> &perm,
+                 std::vector<int> &tile_start);
+
+// Builds the layout from the kernel CSC of `s`.  perm/tile_start describe the
+// column renumbering and the tiles (tile widths <= TW); slice_bounds the row
+// slices.  Returns false with *err set when an item needs more than 65535
+// segments (the local segment index is 16-bit) or a tile is wider than TW.
+bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
+                       const std::vector<int> &tile_start, const std::vector<int> &slice_bounds,
+                       int TW, TileLayout *out, std::string *err);
+
+// Host evaluation of the per-slice column sums P[s*n + new_col] from the
+// layout, used to check the layout against the CSC before it is uploaded.
+void tile_layout_apply(const TileLayout &L, int n, const double *u, double *P);
+
+}  // namespace plsqr

@@ -84,7 +84,7 @@ def make_system(P, case):
     if case.get("zero_b"):
         s = P.System.from_arrays(s.m_k, s.m_d, s.n, s.k_col_ptr, s.k_row_idx, s.k_val, s.d_row_ptr,
                                  s.d_col_idx, s.d_val, s.dc_col_ptr, s.dc_row_idx, s.dc_val,
-                                 np.zeros(s.m_k), s.x_true)
+                                 np.zeros(s.m_k), s.x_true, grid=s.grid)
     return s
 
 

@@ -174,3 +174,20 @@ def test_operator_rejects_bad_indices_before_touching_the_gpu(P):
     with pytest.raises(P.PlsqrError) as e:
         P.Operator(bad)
     assert e.value.code in (2, 7)
+
+
+def test_tiled_layout_self_check_on_host(P, monkeypatch):
+    """The bricked mode-2 layout is built on the host before any device call;
+    with PLSQR_TILE_CHECK=1 its per-slice column sums are compared with the
+    CSC's there.  Without a GPU the creation then stops at hipGetDevice, so a
+    HIP error (not a self-check error) means the layout passed."""
+    monkeypatch.setenv("PLSQR_TILE_CHECK", "1")
+    for args in ((16, 16, 16, 20000, 0, 2013, 0.01, 0.1), (9, 7, 20, 3000, 1, 4, 0.02, 0.05)):
+        s = P.generate(*args)
+        for grid in ("auto", None):
+            try:
+                op = P.Operator(s, grid=grid)
+            except P.PlsqrError as e:
+                assert e.code == 4 and "self-check" not in str(e), str(e)
+            else:  # a GPU is present: the operator was built
+                op.close()
