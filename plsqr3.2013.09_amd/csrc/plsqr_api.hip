@@ -64,6 +64,8 @@ struct plsqr_op {
     uint16_t *t_vlen = nullptr, *t_sseg = nullptr;
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
+    bool nt_loads = true;
+    int tiled_persistent = 0;  // > 0: workgroups of the persistent tiled kernel
     std::vector<void *> allocs;
     size_t bytes = 0;
     // launch shapes
@@ -154,9 +156,20 @@ constexpr int TILE_CAP = 2048;
 int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
     using namespace plsqr;
     if (op->tiled) {
-        hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
-                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval,
-                           op->t_sseg, u, sc, op->P);
+        if (op->tiled_persistent) {
+            const int q = std::max(1, std::min(op->tT, op->tiled_persistent / op->tS));
+            hipLaunchKernelGGL((k_cols_tiled_persistent<TILE_W, BS, TILE_CAP, false>), dim3(q * op->tS), dim3(BS), 0,
+                               op->stream, op->n, op->tS, op->tT, op->t_start, op->t_slot, op->t_seg, op->t_vlen,
+                               op->t_ray, op->t_sval, op->t_sseg, u, sc, op->P);
+        } else if (op->nt_loads) {
+            hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, true>), dim3(op->tT * op->tS), dim3(BS), 0,
+                               op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
+                               op->t_sval, op->t_sseg, u, sc, op->P);
+        } else {
+            hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, false>), dim3(op->tT * op->tS), dim3(BS), 0,
+                               op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
+                               op->t_sval, op->t_sseg, u, sc, op->P);
+        }
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
@@ -593,7 +606,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
         std::vector<int> perm, tiles;
         if (grid) {
-            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, perm, tiles);
+            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, env_int("PLSQR_ZCOL", 1) != 0, perm, tiles);
         } else {
             perm.resize((size_t)s->n);
             for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
@@ -639,6 +652,8 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     if ((rc = build(op, s, hb, L.get()))) return fail(rc);
     L.reset();
     // vectors and scratch
+    op->nt_loads = env_int("PLSQR_NT", 1) != 0;
+    op->tiled_persistent = env_int("PLSQR_TILED_WG", 0);
     op->L1 = env_int("PLSQR_L1", 16);
     op->L2 = env_int("PLSQR_L2", 16);
     op->G1 = grid_for((long long)op->m_k * op->L1, BS, env_int("PLSQR_G1", op->cu * 8));

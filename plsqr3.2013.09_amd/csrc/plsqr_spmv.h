@@ -20,6 +20,15 @@
 
 namespace plsqr {
 
+// Streamed matrix data is read once per launch: load it non-temporally so it
+// does not evict the gathered vector (x in mode 1, the XCD's slice of u in
+// mode 2) from L2.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 // ---------------------------------------------------------------------------
 // Mode 1
 // ---------------------------------------------------------------------------
@@ -55,14 +64,17 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
         if (r < m_k) {
             const int b = rp[r], e = rp[r + 1];
             int k = b + lane;
-            // two independent loads in flight per lane
-            for (; k + L < e; k += 2 * L) {
-                const double v0 = av[k] * x[ci[k]];
-                const double v1 = av[k + L] * x[ci[k + L]];
+            // four independent gathers in flight per lane
+            for (; k + 3 * L < e; k += 4 * L) {
+                const int c0 = ci[k], c1 = ci[k + L], c2 = ci[k + 2 * L], c3 = ci[k + 3 * L];
+                const double a0 = av[k], a1 = av[k + L], a2 = av[k + 2 * L], a3 = av[k + 3 * L];
+                const double v0 = a0 * x[c0], v1 = a1 * x[c1], v2 = a2 * x[c2], v3 = a3 * x[c3];
                 acc += v0;
                 acc += v1;
+                acc += v2;
+                acc += v3;
             }
-            if (k < e) acc += av[k] * x[ci[k]];
+            for (; k < e; k += L) acc += av[k] * x[ci[k]];
         }
         acc = group_sum<L>(acc);
         if (lane == 0 && r < m_k) {
@@ -173,22 +185,14 @@ __global__ __launch_bounds__(BS) void k_cols_sliced(int n, int nsl, const int *_
 // nonzeros j = w, w+NW, ... of voxel v, reading the segment values from LDS;
 // the NW partial sums of a voxel are then added in a fixed order.
 // Segments beyond the LDS capacity are processed in rounds of CAP.
-template <int TW, int BS, int CAP>
-__global__ __launch_bounds__(BS) void k_cols_tiled(
-    int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+template <int TW, int BS, int CAP, bool NT>
+__device__ __forceinline__ void cols_tiled_item(
+    int it, int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
-    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
-    double *__restrict__ P) {
+    const uint16_t *__restrict__ sseg, const double *__restrict__ u, double ib,
+    double *__restrict__ P, double *g, double (*part)[TW]) {
     constexpr int NW = BS / TW;
-    __shared__ double g[CAP];
-    __shared__ double part[NW][TW];
-    double ib = 1.0;
-    if (sc) {
-        if (sc->done || !sc->beta_pos) return;
-        ib = sc->ib;
-    }
-    const int it = blockIdx.x;
     const int s = it % S;
     const int t = it / S;
     const int c0 = tile_start[t];
@@ -201,19 +205,63 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
     const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
     double acc = 0.0;
     int j = w;
-    for (int r0 = 0; r0 < ns; r0 += CAP) {
-        const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
-        for (int i = threadIdx.x; i < cnt; i += BS) g[i] = u[seg_ray[s0 + r0 + i]] * ib;
-        __syncthreads();
-        const int hi = r0 + cnt;
-        while (j < len) {
-            const long long slot = base + (long long)j * TW + v;
-            const int sg = sseg[slot];
-            if (sg >= hi) break;
-            acc += sval[slot] * g[sg - r0];
-            j += NW;
+    if (ns <= CAP) {
+        // one round (the common case): every segment is in LDS, so the slot
+        // loop needs no bound check and keeps four slot loads in flight; the
+        // sum is still taken in slot order
+        // all of a thread's segment rows first, then all its gathers: CAP/BS
+        // independent loads in flight instead of one dependent pair at a time
+        constexpr int PER = CAP / BS;
+        int rr[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            rr[q] = i < ns ? ld_stream<NT>(seg_ray + s0 + i) : 0;
+        }
+        double gv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            gv[q] = i < ns ? u[rr[q]] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            if (i < ns) g[i] = gv[q] * ib;
         }
         __syncthreads();
+        const double *sv = sval + base + v;
+        const uint16_t *ss = sseg + base + v;
+        for (; j + 3 * NW < len; j += 4 * NW) {
+            const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+            const int g0 = ld_stream<NT>(ss + o0), g1 = ld_stream<NT>(ss + o1), g2 = ld_stream<NT>(ss + o2),
+                      g3 = ld_stream<NT>(ss + o3);
+            const double a0 = ld_stream<NT>(sv + o0), a1 = ld_stream<NT>(sv + o1), a2 = ld_stream<NT>(sv + o2),
+                         a3 = ld_stream<NT>(sv + o3);
+            acc += a0 * g[g0];
+            acc += a1 * g[g1];
+            acc += a2 * g[g2];
+            acc += a3 * g[g3];
+        }
+        for (; j < len; j += NW) {
+            const long long o = (long long)j * TW;
+            acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o)];
+        }
+    } else {
+        for (int r0 = 0; r0 < ns; r0 += CAP) {
+            const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
+            for (int i = threadIdx.x; i < cnt; i += BS) g[i] = u[seg_ray[s0 + r0 + i]] * ib;
+            __syncthreads();
+            const int hi = r0 + cnt;
+            while (j < len) {
+                const long long slot = base + (long long)j * TW + v;
+                const int sg = sseg[slot];
+                if (sg >= hi) break;
+                acc += sval[slot] * g[sg - r0];
+                j += NW;
+            }
+            __syncthreads();
+        }
     }
     part[w][v] = acc;
     __syncthreads();
@@ -223,6 +271,48 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
         for (int k = 1; k < NW; ++k) tot += part[k][v];
         P[(size_t)s * n + c0 + v] = tot;
     }
+    __syncthreads();  // g and part are reused by the next item
+}
+
+template <int TW, int BS, int CAP, bool NT>
+__global__ __launch_bounds__(BS) void k_cols_tiled(
+    int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+    const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
+    const int *__restrict__ seg_ray, const double *__restrict__ sval,
+    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
+    double *__restrict__ P) {
+    __shared__ double g[CAP];
+    __shared__ double part[BS / TW][TW];
+    double ib = 1.0;
+    if (sc) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+    }
+    cols_tiled_item<TW, BS, CAP, NT>(blockIdx.x, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
+                                     sval, sseg, u, ib, P, g, part);
+}
+
+// Persistent form: gridDim.x = S * Q workgroups; workgroup b keeps slice
+// b % S (its XCD) and walks the tiles b / S, b / S + Q, ...
+template <int TW, int BS, int CAP, bool NT>
+__global__ __launch_bounds__(BS) void k_cols_tiled_persistent(
+    int n, int S, int T, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+    const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
+    const int *__restrict__ seg_ray, const double *__restrict__ sval,
+    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
+    double *__restrict__ P) {
+    __shared__ double g[CAP];
+    __shared__ double part[BS / TW][TW];
+    double ib = 1.0;
+    if (sc) {
+        if (sc->done || !sc->beta_pos) return;
+        ib = sc->ib;
+    }
+    const int s = blockIdx.x % S;
+    const int Q = gridDim.x / S;
+    for (int t = blockIdx.x / S; t < T; t += Q)
+        cols_tiled_item<TW, BS, CAP, NT>(t * S + s, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
+                                         sval, sseg, u, ib, P, g, part);
 }
 
 // dst[i] = src[idx[i]]  (column renumbering at the API boundary)

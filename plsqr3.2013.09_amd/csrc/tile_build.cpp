@@ -36,21 +36,28 @@ inline void slice_range(const plsqr_system *s, int o, int r0, int r1, int *b, in
 
 }  // namespace
 
-void brick_order(int nx, int ny, int nz, int bx, int by, int bz, std::vector<int> &perm,
-                 std::vector<int> &tile_start) {
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, bool z_columns,
+                 std::vector<int> &perm, std::vector<int> &tile_start) {
     const long long n = (long long)nx * ny * nz;
     perm.assign((size_t)n, 0);
     tile_start.clear();
     int next = 0;
-    for (int z0 = 0; z0 < nz; z0 += bz)
+    auto brick = [&](int x0, int y0, int z0) {
+        tile_start.push_back(next);
+        for (int z = z0; z < std::min(nz, z0 + bz); ++z)
+            for (int y = y0; y < std::min(ny, y0 + by); ++y)
+                for (int x = x0; x < std::min(nx, x0 + bx); ++x)
+                    perm[(size_t)z * ny * nx + (size_t)y * nx + x] = next++;
+    };
+    if (z_columns) {
         for (int y0 = 0; y0 < ny; y0 += by)
-            for (int x0 = 0; x0 < nx; x0 += bx) {
-                tile_start.push_back(next);
-                for (int z = z0; z < std::min(nz, z0 + bz); ++z)
-                    for (int y = y0; y < std::min(ny, y0 + by); ++y)
-                        for (int x = x0; x < std::min(nx, x0 + bx); ++x)
-                            perm[(size_t)z * ny * nx + (size_t)y * nx + x] = next++;
-            }
+            for (int x0 = 0; x0 < nx; x0 += bx)
+                for (int z0 = 0; z0 < nz; z0 += bz) brick(x0, y0, z0);
+    } else {
+        for (int z0 = 0; z0 < nz; z0 += bz)
+            for (int y0 = 0; y0 < ny; y0 += by)
+                for (int x0 = 0; x0 < nx; x0 += bx) brick(x0, y0, z0);
+    }
     tile_start.push_back(next);
 }
 

@@ -40,10 +40,13 @@ struct TileLayout {
 };
 
 // Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
-// voxels in brick-major order, voxels inside a brick x fastest.  Fills
-// perm (old -> new) and the tile starts (one tile per brick).
-void brick_order(int nx, int ny, int nz, int bx, int by, int bz, std::vector<int> &perm,
-                 std::vector<int> &tile_start);
+// voxels, voxels inside a brick x fastest.  Bricks are numbered z fastest
+// (vertical stacks of bricks, then x, then y) when z_columns is set, so that
+// the workgroups running at the same time on one XCD cover a few stacks and
+// the rays crossing them; otherwise in x-fastest brick order.  Fills perm
+// (old -> new) and the tile starts (one tile per brick).
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, bool z_columns,
+                 std::vector<int> &perm, std::vector<int> &tile_start);
 
 // Builds the layout from the kernel CSC of `s`.  perm/tile_start describe the
 // column renumbering and the tiles (tile widths <= TW); slice_bounds the row

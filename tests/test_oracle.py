@@ -181,3 +181,55 @@ def test_oracle_damped_lsqr_matches_scipy(P, O, MG):
     assert rel(x, ref[0]) < 1e-10
     # with damping, rnorm is the norm of the damped residual (scipy's r2norm)
     np.testing.assert_allclose(res["rnorm"], ref[4], rtol=1e-9)
+
+
+def test_norm_rounding_alone_moves_x(P, O, MG):
+    """LSQR iterates are sensitive to the rounding of ||u|| and ||v||: with the
+    oracle's exact aprod and only the norms computed as a correctly rounded
+    sum of squares instead of the reference's scaled dnrm2 (mycblas.c:194-219),
+    x after 100 iterations on the 16^3 system moves by ~1e-10.  This bounds how
+    close any implementation that does not reproduce dnrm2 bit for bit can be
+    to the reference, and sets the default operator's tolerance."""
+    import math
+    g = load_golden("grid16_rays20k_itn100")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    orc = O.Oracle(s)
+    mk, md, n = s.m_k, s.m_d, s.n
+
+    def run(norm_u, norm_v, itn=100):
+        u = np.concatenate([s.b, np.zeros(md)])
+        v = np.zeros(n)
+        x = np.zeros(n)
+        beta = norm_u(u)
+        u *= 1.0 / beta
+        orc.aprod(2, v, u.copy())
+        alpha = norm_v(v)
+        v *= 1.0 / alpha
+        w = v.copy()
+        rhobar, phibar = alpha, beta
+        for _ in range(itn):
+            u *= -alpha
+            orc.aprod(1, v.copy(), u)
+            beta = norm_u(u)
+            u *= 1.0 / beta
+            v *= -beta
+            orc.aprod(2, v, u.copy())
+            alpha = norm_v(v)
+            v *= 1.0 / alpha
+            rho = orc.d2norm(rhobar, beta)
+            cs, sn = rhobar / rho, beta / rho
+            theta, rhobar = sn * alpha, -cs * alpha
+            phi, phibar = cs * phibar, sn * phibar
+            t = w.copy()
+            x[:] = (phi / rho) * t + x
+            w[:] = (-theta / rho) * t + v
+        return x
+
+    ref_u = lambda u: orc.lib.orc_norm_u(mk, O._d(np.ascontiguousarray(u[:mk])), md,
+                                         O._d(np.ascontiguousarray(u[mk:])))
+    ref_v = lambda v: orc.lib.orc_norm_v(n, O._d(v))
+    x_ref = run(ref_u, ref_v)
+    assert np.array_equal(x_ref, g["x"])                  # the mirror is the reference
+    fs = lambda a: math.sqrt(math.fsum(a * a))
+    d = rel(run(fs, fs), g["x"])
+    assert 2e-11 < d < 1e-9, d
