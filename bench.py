@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--workload", default="grid67_rays2m")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--refine-every", type=int, default=25,
+                    help="mixed precision: inner FP32 iterations per FP64 residual refinement")
     ap.add_argument("--max-iters", type=int, default=5000)
     ap.add_argument("--aprod-reps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=6)
@@ -202,12 +204,13 @@ def main():
         hip.free(cp_dst)
         return
     for _ in range(a.warmup):
-        op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec)
+        op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec, refine_every=a.refine_every)
     hip.sync()
     infos = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        infos.append(op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec))
+        infos.append(op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec,
+                                    refine_every=a.refine_every))
     hip.sync()
     wall = time.perf_counter() - t0
     iters = infos[-1]["itn"]

@@ -15,6 +15,7 @@
 #include "plsqr_hip.h"
 #include "plsqr_kernels.h"
 #include "plsqr_spmv.h"
+#include "plsqr_mixed.h"
 #include "tile_build.h"
 
 using plsqr::Scal;
@@ -65,6 +66,9 @@ struct plsqr_op {
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
     bool nt_loads = true;
+    // FP32 copies for mixed-precision solves (allocated on first use)
+    float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr;
+    double *dx = nullptr;
     int tiled_persistent = 0;  // > 0: workgroups of the persistent tiled kernel
     std::vector<void *> allocs;
     size_t bytes = 0;
@@ -1010,6 +1014,180 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
     return PLSQR_OK;
 }
 
+static int ensure_fp32(plsqr_op *op) {
+    using namespace plsqr;
+    if (op->rv32) return PLSQR_OK;
+    if (!op->tiled) return set_err(op, PLSQR_ERR_STATE, "mixed precision needs the tiled operator (not the exact-order one)");
+    int rc;
+    const long long nr = op->nnz_k + op->nnz_d;
+    if ((rc = dalloc(op, &op->rv32, nr)) || (rc = dalloc(op, &op->t_sval32, op->t_nslot)) ||
+        (rc = dalloc(op, &op->u32, op->m)) || (rc = dalloc(op, &op->v32, op->n)) || (rc = dalloc(op, &op->dx, op->n)))
+        return rc;
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nr, 256, op->cu * 16)), dim3(256), 0, op->stream, nr, op->rv, op->rv32);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(op->t_nslot, 256, op->cu * 16)), dim3(256), 0, op->stream,
+                       op->t_nslot, op->t_sval, op->t_sval32);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+static int enqueue_iteration32(plsqr_op *op) {
+    using namespace plsqr;
+    dim3 g1(op->G1), b(BS);
+    switch (op->L1) {
+        case 8: hipLaunchKernelGGL((k_rows32<8, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
+        case 32: hipLaunchKernelGGL((k_rows32<32, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
+        default: hipLaunchKernelGGL((k_rows32<16, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
+    }
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
+    hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->n,
+                       op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,
+                       op->u32, (const Scal *)op->sc, op->P);
+    hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
+                       op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
+    LAUNCH_CHECK(op);
+    hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4);
+    hipLaunchKernelGGL((k_update32<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
+                       op->w, op->dx, op->part);
+    hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G6, (double *)nullptr);
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+// sum of a device partial array (fixed order on the host)
+static int host_sum(plsqr_op *op, const double *part, int count, double *out) {
+    std::vector<double> h((size_t)count);
+    HIP_TRY(op, hipMemcpyAsync(h.data(), part, sizeof(double) * count, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    double s = 0.0;
+    for (int i = 0; i < count; ++i) s += h[(size_t)i];
+    *out = s;
+    return PLSQR_OK;
+}
+
+// Mixed precision with FP64 outer refinement: r = [b;0] - A x in FP64, an
+// inner FP32 LSQR on min ||A dx - r|| for refine_every iterations, x += dx;
+// stops when ||A^T r|| / (anorm ||r||) <= tol in FP64 (anorm: the largest
+// LSQR estimate of ||A|| seen by the inner runs) or after itn inner steps.
+static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out) {
+    using namespace plsqr;
+    int rc;
+    if ((rc = ensure_fp32(op))) return rc;
+    const int itn = std::max(0, o->itn);
+    const int period = o->refine_every > 0 ? o->refine_every : itn;
+    const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->x);
+    LAUNCH_CHECK(op);
+    hipEvent_t e0, e1;
+    HIP_TRY(op, hipEventCreate(&e0));
+    HIP_TRY(op, hipEventCreate(&e1));
+    HIP_TRY(op, hipEventRecord(e0, op->stream));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
+    const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
+    double anorm_max = 0.0, rn = 0.0, test2 = INFINITY, an = 0.0;
+    int total = 0, outer = 0;
+    Scal h;
+    for (;;) {
+        // FP64 residual of the current x
+        hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                           op->m, op->rp, op->ci, op->rv, op->x, b_dev, op->hy);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, op->hy, op->part);
+        LAUNCH_CHECK(op);
+        if ((rc = host_sum(op, op->part, gs, &rn))) return rc;
+        rn = std::sqrt(rn);
+        if (outer > 0) {
+            hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->hx);
+            LAUNCH_CHECK(op);
+            if ((rc = aprod2_dev(op, op->hx, op->hy))) return rc;
+            hipLaunchKernelGGL((k_ssq<BS>), dim3(gn), dim3(BS), 0, op->stream, (long long)op->n, op->hx, op->part);
+            LAUNCH_CHECK(op);
+            if ((rc = host_sum(op, op->part, gn, &an))) return rc;
+            an = std::sqrt(an);
+            test2 = (rn > 0.0 && anorm_max > 0.0) ? an / (anorm_max * rn) : 0.0;
+            if (test2 <= o->tol || total >= itn) break;
+        }
+        if (rn == 0.0 || itn == 0) {
+            test2 = 0.0;
+            break;
+        }
+        // inner FP32 LSQR on r
+        const int k = std::min(period, itn - total);
+        Scal z;
+        memset(&z, 0, sizeof(z));
+        z.damp = 0.0;
+        z.atol = o->atol;
+        z.btol = o->btol;
+        z.ctol = o->conlim > 0.0 ? 1.0 / o->conlim : 0.0;
+        z.tol = o->tol;
+        z.itnlim = o->itnlim;
+        z.itn_max = k;
+        z.cs2 = -1.0;
+        HIP_TRY(op, hipMemcpyAsync(op->sc, &z, sizeof(z), hipMemcpyHostToDevice, op->stream));
+        hipLaunchKernelGGL(k_to_f32, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, op->hy, op->u32);
+        hipLaunchKernelGGL(k_fill32, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, op->v32);
+        hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->dx);
+        hipLaunchKernelGGL((k_ssq2<BS>), dim3(gs), dim3(BS), 0, op->stream, op->m_k, op->m, op->hy, op->part);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
+        hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
+                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32,
+                           op->t_sseg, op->u32, (const Scal *)op->sc, op->P);
+        hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
+                           op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
+        LAUNCH_CHECK(op);
+        hipLaunchKernelGGL(k_s_init_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4,
+                           (double *)nullptr);
+        hipLaunchKernelGGL(k_init_vw32, dim3(gz), dim3(256), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
+                           op->w);
+        LAUNCH_CHECK(op);
+        if (!exec) {
+            HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_iteration32(op);
+            hipError_t ce = hipStreamEndCapture(op->stream, &graph);
+            if (rc) return rc;
+            if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+            HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < k; ++i) HIP_TRY(op, hipGraphLaunch(exec, op->stream));
+        HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        anorm_max = std::max(anorm_max, h.anorm);
+        hipLaunchKernelGGL(k_axpy1, dim3(gz), dim3(256), 0, op->stream, op->n, op->dx, op->x);
+        LAUNCH_CHECK(op);
+        total += std::max(h.itn, 0);
+        ++outer;
+        if (h.itn == 0) {  // the inner run could not start (r orthogonal to the range): converged
+            test2 = 0.0;
+            break;
+        }
+    }
+    HIP_TRY(op, hipEventRecord(e1, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    float ms = 0.f;
+    HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (out) {
+        memset(out, 0, sizeof(*out));
+        out->itn = total;
+        out->inner_total = total;
+        out->refinements = outer;
+        out->anorm = anorm_max;
+        out->rnorm = rn;
+        out->arnorm = test2;
+        out->arnorm_true = an;
+        out->elapsed_ms = ms;
+        out->istop = test2 <= o->tol ? 2 : 5;
+    }
+    return PLSQR_OK;
+}
+
 int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *opts, double *x_dev,
                             double *w_dev, plsqr_solve_out *out, double *trace) {
     if (!op) return PLSQR_ERR_ARG;
@@ -1017,9 +1195,15 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
     if (!b_dev || !opts || !x_dev) return set_err(op, PLSQR_ERR_ARG, "b, opts and x are required");
     plsqr_solve_opts o;
     default_opts(opts, &o);
-    if (o.precision != PLSQR_PREC_FP64) return set_err(op, PLSQR_ERR_ARG, "precision not supported by this build");
-    // the loop works on the handle's x and w (device column order)
-    int rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, trace);
+    int rc;
+    if (o.precision == PLSQR_PREC_MIXED) {
+        rc = solve_mixed(op, b_dev, &o, out);
+    } else if (o.precision == PLSQR_PREC_FP64) {
+        // the loop works on the handle's x and w (device column order)
+        rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, trace);
+    } else {
+        return set_err(op, PLSQR_ERR_ARG, "unknown precision");
+    }
     if (rc) return rc;
     if (op->permuted) {
         if ((rc = to_external(op, op->x, x_dev))) return rc;

@@ -1,0 +1,277 @@
+// Mixed-precision kernels (included once, by plsqr_api.hip).
+//
+// Matrix values and the Golub-Kahan vectors u, v are stored and multiplied in
+// FP32; every row / column dot product is accumulated with the compensated
+// FP32 scheme of Ogita, Rump and Oishi ("Accurate sum and dot product", SIAM
+// J. Sci. Comput. 26, 2005, Algorithm Dot2): TwoProduct via fmaf and
+// Knuth's TwoSum carry the rounding errors in a second FP32 word, so each dot
+// product is as accurate as if computed in twice the working precision.
+// alpha, beta, the rotations and the estimates stay FP64 (the Scal kernels),
+// and so do x, w and the outer residual r = [b;0] - A x.
+#pragma once
+
+#include "plsqr_kernels.h"
+
+namespace plsqr {
+
+struct F2 {  // s + c, |c| << |s|
+    float s, c;
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void dot2_add(F2 &acc, float a, float x) {
+    const float p = a * x;
+    const float ep = fmaf(a, x, -p);  // exact error of the product
+    float s, es;
+    two_sum(acc.s, p, s, es);
+    acc.s = s;
+    acc.c += ep + es;
+}
+
+__device__ __forceinline__ F2 f2_merge(F2 a, F2 b) {
+    F2 r;
+    float e;
+    two_sum(a.s, b.s, r.s, e);
+    r.c = (a.c + b.c) + e;
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ F2 group_sum_f2(F2 v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        F2 t;
+        t.s = __shfl_xor(v.s, o, W);
+        t.c = __shfl_xor(v.c, o, W);
+        v = f2_merge(v, t);
+    }
+    return v;
+}
+
+// Mode 1 in FP32: y <- fl32(fl32(y*s1)*s2) + dot2(row, x); FP64 partial sums
+// of squares of the new y, pair layout (kernel rows, damping rows).
+template <int L, int BS>
+__global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__restrict__ rp,
+                                               const int *__restrict__ ci, const float *__restrict__ av,
+                                               const float *__restrict__ x, float *__restrict__ y,
+                                               const Scal *__restrict__ sc, double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    if (sc->done) return;
+    const float s1 = (float)sc->ib, s2 = (float)(-sc->alpha);
+    constexpr int R = BS / L;
+    const int lane = threadIdx.x % L;
+    const int slot = threadIdx.x / L;
+    double ssq_k = 0.0, ssq_d = 0.0;
+    const int groups = (m_k + R - 1) / R;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int r = g * R + slot;
+        F2 acc{0.f, 0.f};
+        if (r < m_k) {
+            const int b = rp[r], e = rp[r + 1];
+            int k = b + lane;
+            for (; k + 3 * L < e; k += 4 * L) {
+                const int c0 = ci[k], c1 = ci[k + L], c2 = ci[k + 2 * L], c3 = ci[k + 3 * L];
+                const float a0 = av[k], a1 = av[k + L], a2 = av[k + 2 * L], a3 = av[k + 3 * L];
+                const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+                dot2_add(acc, a0, x0);
+                dot2_add(acc, a1, x1);
+                dot2_add(acc, a2, x2);
+                dot2_add(acc, a3, x3);
+            }
+            for (; k < e; k += L) dot2_add(acc, av[k], x[ci[k]]);
+        }
+        acc = group_sum_f2<L>(acc);
+        if (lane == 0 && r < m_k) {
+            const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+            y[r] = yn;
+            ssq_k += (double)yn * (double)yn;
+        }
+    }
+    for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
+         r += (long long)gridDim.x * BS) {
+        F2 acc{0.f, 0.f};
+        for (int k = rp[r]; k < rp[r + 1]; ++k) dot2_add(acc, av[k], x[ci[k]]);
+        const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+        y[r] = yn;
+        ssq_d += (double)yn * (double)yn;
+    }
+    const double a = block_sum<BS>(ssq_k, sh);
+    const double d = block_sum<BS>(ssq_d, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = d;
+    }
+}
+
+// Mode 2 in FP32 on the tiled layout (same item structure as k_cols_tiled).
+template <int TW, int BS, int CAP>
+__global__ __launch_bounds__(BS) void k_cols_tiled32(
+    int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+    const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
+    const int *__restrict__ seg_ray, const float *__restrict__ sval, const uint16_t *__restrict__ sseg,
+    const float *__restrict__ u, const Scal *__restrict__ sc, double *__restrict__ P) {
+    constexpr int NW = BS / TW;
+    __shared__ float g[CAP];
+    __shared__ F2 part[NW][TW];
+    if (sc->done || !sc->beta_pos) return;
+    const float ib = (float)sc->ib;
+    const int it = blockIdx.x;
+    const int s = it % S;
+    const int t = it / S;
+    const int c0 = tile_start[t];
+    const int tw = tile_start[t + 1] - c0;
+    const int v = threadIdx.x % TW;
+    const int w = threadIdx.x / TW;
+    const long long base = slot_base[it];
+    const long long s0 = seg_start[it];
+    const int ns = (int)(seg_start[it + 1] - s0);
+    const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
+    F2 acc{0.f, 0.f};
+    int j = w;
+    for (int r0 = 0; r0 < ns; r0 += CAP) {
+        const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
+        constexpr int PER = CAP / BS;
+        int rr[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            rr[q] = i < cnt ? seg_ray[s0 + r0 + i] : 0;
+        }
+        float gv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            gv[q] = i < cnt ? u[rr[q]] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * BS;
+            if (i < cnt) g[i] = gv[q] * ib;
+        }
+        __syncthreads();
+        const int hi = r0 + cnt;
+        if (ns <= CAP) {
+            const float *sv = sval + base + v;
+            const uint16_t *ss = sseg + base + v;
+            for (; j + 3 * NW < len; j += 4 * NW) {
+                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
+                const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                dot2_add(acc, a0, g[g0]);
+                dot2_add(acc, a1, g[g1]);
+                dot2_add(acc, a2, g[g2]);
+                dot2_add(acc, a3, g[g3]);
+            }
+            for (; j < len; j += NW) {
+                const long long o = (long long)j * TW;
+                dot2_add(acc, sv[o], g[ss[o]]);
+            }
+        } else {
+            while (j < len) {
+                const long long slot = base + (long long)j * TW + v;
+                const int sg = sseg[slot];
+                if (sg >= hi) break;
+                dot2_add(acc, sval[slot], g[sg - r0]);
+                j += NW;
+            }
+        }
+        __syncthreads();
+    }
+    part[w][v] = acc;
+    __syncthreads();
+    if (w == 0 && v < tw) {
+        F2 tot = part[0][v];
+#pragma unroll
+        for (int k = 1; k < NW; ++k) tot = f2_merge(tot, part[k][v]);
+        P[(size_t)s * n + c0 + v] = (double)tot.s + (double)tot.c;
+    }
+}
+
+// v32[c] <- fl32(v32*vscale) + (tk + td): tk the FP64 slice partials, td the
+// damping column (FP64 values, FP32 u); FP64 partial sums of squares.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, const double *__restrict__ P,
+                                                       const int *__restrict__ dcp, const int *__restrict__ dri,
+                                                       const double *__restrict__ dav, const float *__restrict__ u,
+                                                       const Scal *__restrict__ sc, float *__restrict__ v,
+                                                       double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    if (sc->done || !sc->beta_pos) return;
+    const double ib = sc->ib;
+    const float vs = (float)sc->vscale;
+    const float *ud = u + m_k;
+    double ssq = 0.0;
+    for (int c = blockIdx.x * BS + threadIdx.x; c < n; c += gridDim.x * BS) {
+        double tk = P[c];
+        for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
+        double td = 0.0;
+        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * ((double)ud[dri[k]] * ib);
+        const float vn = v[c] * vs + (float)(tk + td);
+        v[c] = vn;
+        ssq += (double)vn * (double)vn;
+    }
+    const double a = block_sum<BS>(ssq, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// x/w update with FP32 v and FP64 x, w (lsqr.c:590-597, 685-690)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_update32(int n, const Scal *__restrict__ sc, float *__restrict__ v,
+                                                 double *__restrict__ w, double *__restrict__ x,
+                                                 double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    if (sc->done) return;
+    const bool scl = sc->scale_v != 0;
+    const float ia = (float)sc->inv_alpha;
+    const double t1 = sc->t1, t2 = sc->t2, t3 = sc->t3;
+    double dk = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        float vi = v[i];
+        if (scl) {
+            vi = vi * ia;
+            v[i] = vi;
+        }
+        const double t = w[i];
+        x[i] = t1 * t + x[i];
+        w[i] = t2 * t + (double)vi;
+        const double q = t3 * t;
+        dk += q * q;
+    }
+    const double a = block_sum<BS>(dk, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void k_init_vw32(int n, const Scal *__restrict__ sc, float *__restrict__ v, double *__restrict__ w) {
+    if (sc->done) return;
+    const bool scl = sc->scale_v != 0;
+    const float ia = (float)sc->inv_alpha;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float vi = v[i];
+        if (scl) {
+            vi = vi * ia;
+            v[i] = vi;
+        }
+        w[i] = (double)vi;
+    }
+}
+
+__global__ void k_to_f32(long long len, const double *__restrict__ a, float *__restrict__ b) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_fill32(long long len, float *__restrict__ a) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256) a[i] = 0.f;
+}
+
+// x += dx
+__global__ void k_axpy1(int n, const double *__restrict__ dx, double *__restrict__ x) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] = x[i] + dx[i];
+}
+
+}  // namespace plsqr

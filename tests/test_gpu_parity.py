@@ -192,3 +192,26 @@ def test_residual_norms_confirm_test2(P, MG, gpu_available):
     np.testing.assert_allclose(an, np.linalg.norm(A.T @ r), rtol=1e-10)
     np.testing.assert_allclose(af, sp.linalg.norm(A), rtol=1e-12)
     np.testing.assert_allclose(rn, info["rnorm"], rtol=1e-8)
+
+
+def test_mixed_precision_refinement(P, O, MG, gpu_available):
+    """FP32 values and u, v with compensated accumulation, FP64 scalars and an
+    FP64 residual refinement every 25 inner iterations, run to test2 <= 1e-6
+    (checked in FP64 from the true residual)."""
+    announce("mixed precision")
+    import scipy.sparse as sp
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    x_tight, _, _ = O.Oracle(s).lsqr(s.b, 300)
+    with P.Operator(s) as op:
+        x64, i64 = op.lsqr(s.b, 2000, tol=1e-6)
+        xm, im = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)
+        xm2, _ = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)
+    assert np.array_equal(xm, xm2)                       # reproducible
+    assert im["refinements"] >= 2 and im["arnorm"] <= 1e-6
+    r = np.concatenate([s.b, np.zeros(s.m_d)]) - A @ xm
+    assert np.linalg.norm(A.T @ r) / (im["anorm"] * np.linalg.norm(r)) <= 1e-6 * 1.001
+    # as close to the converged solution as the FP64 solve stopped at the same test2
+    e_m, e_64 = rel(xm, x_tight), rel(x64, x_tight)
+    assert e_m < 3 * e_64 + 1e-7, (e_m, e_64)
