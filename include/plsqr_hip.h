@@ -279,6 +279,78 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
 int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_host,
                          double *rnorm, double *arnorm, double *anorm_f);
 
+/* ---------------------------------------------------------------------- */
+/* Batched solves: many independent systems, up to 4 right-hand sides each */
+/* ---------------------------------------------------------------------- */
+/* The reference solves one system per process run (plsqr3.c main ->
+ * call_lsqr, lsqr.c:34-184); a batch runs lsqr.c's iteration for every
+ * (system, right-hand side) pair at once.  Each pair has its own scalar
+ * state and stops on its own test (tol, itn); a stopped pair keeps its x
+ * while the others go on, and a system whose pairs have all stopped costs
+ * no further matrix traffic.  Every iteration of the whole batch is one
+ * captured HIP graph of six launches, whatever the number of systems. */
+typedef struct plsqr_batch plsqr_batch;
+
+#define PLSQR_BATCH_MAX_RHS 4
+enum {
+    PLSQR_BATCH_NO_TUNE = 1,     /* keep the default launch shapes at create      */
+    PLSQR_BATCH_TUNE_MIXED = 2   /* also tune the FP32 kernels at create           */
+};
+
+/* Launch shapes the batch's autotuner chooses among (mode 1: y += A x over
+ * rows; mode 2: x += A^T y over 4 x 4 x 8 voxel tiles). */
+typedef struct plsqr_batch_tuning {
+    int    rows_per_wave;   /* mode 1: rows per 64-lane wavefront: 2, 4, 8, 16  */
+    int    nnz_per_lane;    /* mode 1: nonzeros each lane keeps in flight: 2, 4 */
+    int    block_rows;      /* mode 1 workgroup size: 128 or 256               */
+    int    block_cols;      /* mode 2 workgroup size: 128, 256 or 512          */
+    double ms_rows, ms_cols; /* measured time per launch of the chosen shapes  */
+    int    candidates;      /* shapes timed (0: defaults, not measured)        */
+} plsqr_batch_tuning;
+
+/* systems[p]: the p-th system in the loaded layout (b and x_true unused).
+ * grids: NULL or n_problems entries (a zero-sized entry: no grid).  n_rhs:
+ * right-hand sides per system (1..4).  Validates every system on the host,
+ * builds the block-diagonal device layout and, unless PLSQR_BATCH_NO_TUNE,
+ * times the candidate launch shapes on it and keeps the fastest. */
+int  plsqr_batch_create(const plsqr_system *const *systems, const plsqr_grid *grids,
+                        int n_problems, int n_rhs, int flags, plsqr_batch **out);
+void plsqr_batch_destroy(plsqr_batch *h);
+/* last error of h; with h == NULL the last plsqr_batch_create failure */
+const char *plsqr_batch_last_error(const plsqr_batch *h);
+void plsqr_batch_clear_error(plsqr_batch *h);
+int  plsqr_batch_dims(const plsqr_batch *h, int *n_problems, int *n_rhs,
+                      long long *rows, long long *cols, long long *nnz);
+size_t plsqr_batch_device_bytes(const plsqr_batch *h);
+
+/* Times every candidate shape (reps launches each, median) for one
+ * precision and keeps the fastest; *out (may be NULL) receives it. */
+int plsqr_batch_autotune(plsqr_batch *h, int precision, int reps, plsqr_batch_tuning *out);
+int plsqr_batch_get_tuning(const plsqr_batch *h, int precision, plsqr_batch_tuning *out);
+/* Force a shape (its ms_* and candidates fields are ignored). */
+int plsqr_batch_set_tuning(plsqr_batch *h, int precision, const plsqr_batch_tuning *t);
+
+/* Array layout of every batch vector: system after system, and inside a
+ * system right-hand side after right-hand side; x parts hold n_p values in
+ * the caller's column order, y parts m_k + m_d (kernel rows, then damping
+ * rows), b parts m_k.
+ * aprod of every system and right-hand side at once, host arrays:
+ *   mode 1: y <- y + A x      mode 2: x <- x + A^T y
+ * precision PLSQR_PREC_MIXED runs the FP32 kernels (operands rounded to FP32). */
+int plsqr_batch_aprod(plsqr_batch *h, int mode, int precision, double *x, double *y);
+
+/* LSQR from x = 0 for every (system, right-hand side): opts as for
+ * plsqr_lsqr_solve (precision FP64 or MIXED; MIXED refines in FP64 every
+ * refine_every inner iterations and stops on the FP64 test2 of the true
+ * residual).  out: n_problems * n_rhs results in the array order above;
+ * out[i].elapsed_ms is the device time from the start of the solve until
+ * pair i stopped, and *total_ms (may be NULL) the time of the whole batch. */
+int plsqr_batch_solve(plsqr_batch *h, const double *b_host, const plsqr_solve_opts *opts,
+                      double *x_host, plsqr_solve_out *out, double *total_ms);
+/* Same on device arrays (b_dev and x_dev in the array layout above). */
+int plsqr_batch_solve_device(plsqr_batch *h, const double *b_dev, const plsqr_solve_opts *opts,
+                             double *x_dev, plsqr_solve_out *out, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

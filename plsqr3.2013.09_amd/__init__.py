@@ -16,10 +16,11 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (  # noqa: F401
-    GridC, CONFIGS, ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT,
-    OP_EXACT_ORDER, OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH,
-    PlsqrError, SolveOptsC, SolveOutC, System, dptr, gen_rhs, generate, generate_config, load,
-    read_x, write_x, _check,
+    BATCH_CONFIGS, BATCH_MAX_RHS, BATCH_NO_TUNE, BATCH_TUNE_MIXED, BatchTuningC, GridC, CONFIGS,
+    ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT, OP_EXACT_ORDER,
+    OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH, PlsqrError,
+    SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, gen_rhs, generate, generate_batch,
+    generate_config, load, read_x, write_x, _check,
 )
 
 
@@ -160,3 +161,137 @@ class Operator:
                                              dptr(np.ascontiguousarray(x, np.float64)),
                                              C.byref(rn), C.byref(an), C.byref(af)), self.h)
         return rn.value, an.value, af.value
+
+
+class Batch:
+    """Independent systems solved together (plsqr_batch), n_rhs right-hand
+    sides each.  Vectors cross as lists with one (n_rhs, length) array per
+    system; the library packs them system after system."""
+
+    def __init__(self, systems, n_rhs, grids="auto", flags=0):
+        lib = load()
+        self.lib = lib
+        self.systems = list(systems)
+        P = len(self.systems)
+        arr = (C.POINTER(SystemC) * P)(*[C.pointer(s.c) for s in self.systems])
+        if grids == "auto":
+            grids = [getattr(s, "grid", None) for s in self.systems]
+        g = None
+        if grids is not None:
+            g = (GridC * P)()
+            for i, gr in enumerate(grids):
+                if gr is not None:
+                    g[i].nx, g[i].ny, g[i].nz = (int(v) for v in gr)
+        h = C.c_void_p()
+        rc = lib.plsqr_batch_create(arr, g, P, int(n_rhs), int(flags), C.byref(h))
+        if rc != 0:
+            raise PlsqrError(rc, (lib.plsqr_batch_last_error(None) or b"").decode(errors="replace"))
+        self.h = h
+        self.P, self.K = P, int(n_rhs)
+        self.m_k = [s.m_k for s in self.systems]
+        self.m = [s.m_k + s.m_d for s in self.systems]
+        self.n = [s.n for s in self.systems]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.plsqr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = (self.lib.plsqr_batch_last_error(self.h) or b"").decode(errors="replace")
+            self.lib.plsqr_batch_clear_error(self.h)
+            raise PlsqrError(rc, msg)
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.plsqr_batch_device_bytes(self.h))
+
+    def dims(self):
+        p, k = C.c_int(), C.c_int()
+        r, c, z = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._check(self.lib.plsqr_batch_dims(self.h, C.byref(p), C.byref(k), C.byref(r), C.byref(c),
+                                              C.byref(z)))
+        return dict(n_problems=p.value, n_rhs=k.value, rows=r.value, cols=c.value, nnz=z.value)
+
+    def _pack(self, arrs, lengths):
+        out = []
+        for a, ln in zip(arrs, lengths):
+            a = np.asarray(a, dtype=np.float64).reshape(self.K, ln)
+            out.append(a.ravel())
+        return np.ascontiguousarray(np.concatenate(out))
+
+    def _unpack(self, flat, lengths):
+        res, off = [], 0
+        for ln in lengths:
+            res.append(flat[off:off + self.K * ln].reshape(self.K, ln).copy())
+            off += self.K * ln
+        return res
+
+    def aprod(self, mode, X, Y, precision=PREC_FP64):
+        """mode 1: Y[p] += A_p X[p]; mode 2: X[p] += A_p^T Y[p] (lists of (K, len) arrays);
+        returns the updated (X, Y)."""
+        x = self._pack(X, self.n)
+        y = self._pack(Y, self.m)
+        self._check(self.lib.plsqr_batch_aprod(self.h, int(mode), int(precision), dptr(x), dptr(y)))
+        return self._unpack(x, self.n), self._unpack(y, self.m)
+
+    def solve(self, B, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+        """B: list of (K, m_k) arrays.  Returns (X list of (K, n), infos [p][k] dicts, total_ms)."""
+        b = self._pack(B, self.m_k)
+        x = np.empty(sum(self.n) * self.K, np.float64)
+        out = (SolveOutC * (self.P * self.K))()
+        tot = C.c_double()
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp)
+        self._check(self.lib.plsqr_batch_solve(self.h, dptr(b), C.byref(o), dptr(x), out, C.byref(tot)))
+        infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]
+        return self._unpack(x, self.n), infos, tot.value
+
+    def solve_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
+                     damp=0.0):
+        """b_ptr, x_ptr: device arrays in the library's batch layout.  Returns (infos, total_ms)."""
+        out = (SolveOutC * (self.P * self.K))()
+        tot = C.c_double()
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp)
+        self._check(self.lib.plsqr_batch_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
+                                                      C.c_void_p(x_ptr), out, C.byref(tot)))
+        infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]
+        return infos, tot.value
+
+    def pack_b(self, B):
+        return self._pack(B, self.m_k)
+
+    def autotune(self, precision=PREC_FP64, reps=5):
+        t = BatchTuningC()
+        self._check(self.lib.plsqr_batch_autotune(self.h, int(precision), int(reps), C.byref(t)))
+        return t.as_dict()
+
+    def get_tuning(self, precision=PREC_FP64):
+        t = BatchTuningC()
+        self._check(self.lib.plsqr_batch_get_tuning(self.h, int(precision), C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, precision=PREC_FP64, rows_per_wave=4, nnz_per_lane=4, block_rows=256,
+                   block_cols=256):
+        t = BatchTuningC()
+        t.rows_per_wave, t.nnz_per_lane, t.block_rows, t.block_cols = (
+            int(rows_per_wave), int(nnz_per_lane), int(block_rows), int(block_cols))
+        self._check(self.lib.plsqr_batch_set_tuning(self.h, int(precision), C.byref(t)))
+
+
+# launch shapes the batch autotuner chooses among
+BATCH_SHAPES = [dict(rows_per_wave=r, nnz_per_lane=u, block_rows=b, block_cols=256)
+                for r in (16, 8, 4, 2) for b in (128, 256) for u in (2, 4)] + \
+               [dict(rows_per_wave=4, nnz_per_lane=4, block_rows=256, block_cols=c) for c in (128, 512)]

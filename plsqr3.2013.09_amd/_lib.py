@@ -87,6 +87,21 @@ class SolveOutC(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchTuningC(C.Structure):
+    _fields_ = [
+        ("rows_per_wave", C.c_int), ("nnz_per_lane", C.c_int), ("block_rows", C.c_int),
+        ("block_cols", C.c_int), ("ms_rows", C.c_double), ("ms_cols", C.c_double),
+        ("candidates", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+BATCH_MAX_RHS = 4
+BATCH_NO_TUNE = 1
+BATCH_TUNE_MIXED = 2
+
 APROD_REF_T = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                           c_double_p, c_double_p, C.c_void_p, C.c_int, C.c_int)
 
@@ -125,6 +140,22 @@ SIGNATURES = {
                                           c_double_p]),
     "plsqr_residual_norms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p,
                                        c_double_p, c_double_p]),
+    "plsqr_batch_create": (C.c_int, [C.POINTER(C.POINTER(SystemC)), C.POINTER(GridC), C.c_int,
+                                     C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "plsqr_batch_destroy": (None, [C.c_void_p]),
+    "plsqr_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "plsqr_batch_clear_error": (None, [C.c_void_p]),
+    "plsqr_batch_dims": (C.c_int, [C.c_void_p, c_int_p, c_int_p, C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "plsqr_batch_device_bytes": (C.c_size_t, [C.c_void_p]),
+    "plsqr_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(BatchTuningC)]),
+    "plsqr_batch_get_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchTuningC)]),
+    "plsqr_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchTuningC)]),
+    "plsqr_batch_aprod": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_double_p]),
+    "plsqr_batch_solve": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p,
+                                    C.POINTER(SolveOutC), c_double_p]),
+    "plsqr_batch_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveOptsC),
+                                           C.c_void_p, C.POINTER(SolveOutC), c_double_p]),
 }
 
 _lib = None
@@ -292,6 +323,53 @@ CONFIGS = {
     "grid67_rays2m": dict(nx=67, ny=67, nz=67, n_rays=2_000_000, geometry=GEOM_TOP_SOURCES,
                           seed=2014, sigma=0.02, lam=0.05, itn=100),
 }
+
+
+# Batches of independent problems: grids of edge U{lo..hi} (cubic), rays =
+# rays_per_col x columns between opposite faces, lambda 0.1 damping, n_rhs
+# right-hand sides per problem from their own blob fields and noise levels.
+BATCH_CONFIGS = {
+    "batch24_grid20to48_rhs4": dict(n_problems=24, edge_lo=20, edge_hi=48, rays_per_col=8, n_rhs=4,
+                                    geometry=GEOM_OPPOSITE_FACES, seed=2015, sigma_lo=0.01,
+                                    sigma_hi=0.05, lam=0.1),
+}
+
+
+def batch_specs(name=None, **override):
+    """Per-problem generation parameters of a batch workload (deterministic)."""
+    cfg = dict(BATCH_CONFIGS[name]) if name else {}
+    cfg.update(override)
+    rng = np.random.default_rng(cfg["seed"])
+    P, K = cfg["n_problems"], cfg["n_rhs"]
+    edges = rng.integers(cfg["edge_lo"], cfg["edge_hi"] + 1, size=P)
+    sigmas = rng.uniform(cfg["sigma_lo"], cfg["sigma_hi"], size=(P, K))
+    specs = []
+    for p in range(P):
+        e = int(edges[p])
+        sys_seed = int(cfg["seed"]) * 1000 + p
+        specs.append(dict(grid=(e, e, e), n_rays=int(cfg["rays_per_col"]) * e ** 3,
+                          geometry=cfg["geometry"], seed=sys_seed, lam=cfg["lam"],
+                          rhs=[(sys_seed * 16 + k + 1, float(sigmas[p, k])) for k in range(K)]))
+    return specs
+
+
+def generate_batch(name=None, specs=None, **override):
+    """(systems, B, X_true): B[p] and X_true[p] are (n_rhs, m_k) and (n_rhs, n) arrays."""
+    if specs is None:
+        specs = batch_specs(name, **override)
+    systems, B, XT = [], [], []
+    for sp in specs:
+        nx, ny, nz = sp["grid"]
+        s = generate(nx, ny, nz, sp["n_rays"], sp["geometry"], sp["seed"], sp["rhs"][0][1], sp["lam"])
+        bs, xs = [], []
+        for seed, sigma in sp["rhs"]:
+            b, xt = gen_rhs(s, nx, ny, nz, seed, sigma)
+            bs.append(b)
+            xs.append(xt)
+        systems.append(s)
+        B.append(np.stack(bs))
+        XT.append(np.stack(xs))
+    return systems, B, XT
 
 
 def generate_config(name, **override):

@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -31,20 +32,24 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
+SOURCES = (("plsqr_api.hip", DEVICE), ("plsqr_batch.hip", DEVICE), ("tomo_io.cpp", []),
+           ("tile_build.cpp", []), ("batch_build.cpp", []))
+
+
 def build_lib(out=LIB):
     tmp = out + ".tmp"
-    objs = []
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
-    for src, extra in (("plsqr_api.hip", DEVICE), ("plsqr_mixed.hip", DEVICE),
-                       ("tomo_io.cpp", []), ("tile_build.cpp", [])):
+    jobs = []
+    for src, extra in SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(obj_dir, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "c++"]
-        run([HIPCC] + COMMON + extra + lang + ["-c", path, "-o", obj])
-        objs.append(obj)
+        jobs.append(([HIPCC] + COMMON + extra + lang + ["-c", path, "-o", obj], obj))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda j: run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, out)
     return out

@@ -1,0 +1,264 @@
+// Host-side layout of a batch of independent systems; see batch_build.h.
+#include "batch_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
+
+namespace plsqr {
+
+namespace {
+
+int check_ptr(const char *what, const int *p, int len, long long nnz, std::string *err) {
+    if (!p) {
+        *err = std::string(what) + " is NULL";
+        return PLSQR_ERR_ARG;
+    }
+    if (p[0] != 0) {
+        *err = std::string(what) + "[0] != 0";
+        return PLSQR_ERR_FORMAT;
+    }
+    for (int i = 0; i < len; ++i)
+        if (p[i + 1] < p[i]) {
+            *err = std::string(what) + " decreases at " + std::to_string(i);
+            return PLSQR_ERR_FORMAT;
+        }
+    if (p[len] != nnz) {
+        *err = std::string(what) + " does not end at nnz";
+        return PLSQR_ERR_FORMAT;
+    }
+    return PLSQR_OK;
+}
+
+int check_idx(const char *what, const int *idx, const double *val, long long nnz, int bound, std::string *err) {
+    if (nnz > 0 && (!idx || !val)) {
+        *err = std::string(what) + " arrays are NULL";
+        return PLSQR_ERR_ARG;
+    }
+    std::atomic<long long> bad_idx{LLONG_MAX}, bad_val{LLONG_MAX};
+    par_for(nnz, [&](long long b, long long e) {
+        for (long long i = b; i < e; ++i) {
+            if (idx[i] < 0 || idx[i] >= bound) {
+                long long cur = bad_idx.load();
+                while (i < cur && !bad_idx.compare_exchange_weak(cur, i)) {
+                }
+                return;
+            }
+            if (!std::isfinite(val[i])) {
+                long long cur = bad_val.load();
+                while (i < cur && !bad_val.compare_exchange_weak(cur, i)) {
+                }
+                return;
+            }
+        }
+    });
+    if (bad_idx.load() != LLONG_MAX) {
+        *err = std::string(what) + " index out of range at entry " + std::to_string(bad_idx.load());
+        return PLSQR_ERR_BOUNDS;
+    }
+    if (bad_val.load() != LLONG_MAX) {
+        *err = std::string(what) + " value not finite at entry " + std::to_string(bad_val.load());
+        return PLSQR_ERR_NONFINITE;
+    }
+    return PLSQR_OK;
+}
+
+// CSR of [A_k ; D] in the new column numbering.  Kernel rows list their
+// columns ascending (the CSC is walked in new column order); damping rows keep
+// their entry order.  Row ranges are filled by independent host threads.
+void build_csr(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &inv, ProblemLayout *L) {
+    const int m_k = s->m_k, m_d = s->m_d, n = s->n;
+    L->rp.assign((size_t)m_k + m_d + 1, 0);
+    std::vector<int> cnt((size_t)m_k, 0);
+    for (long long e = 0; e < s->nnz_k; ++e) ++cnt[(size_t)s->k_row_idx[e]];
+    for (int r = 0; r < m_k; ++r) L->rp[(size_t)r + 1] = L->rp[(size_t)r] + cnt[(size_t)r];
+    for (int r = 0; r < m_d; ++r)
+        L->rp[(size_t)m_k + r + 1] = L->rp[(size_t)m_k + r] + (s->d_row_ptr[r + 1] - s->d_row_ptr[r]);
+    const long long nnz = (long long)L->rp[(size_t)m_k + m_d];
+    L->ci.assign((size_t)nnz, 0);
+    L->rv.assign((size_t)nnz, 0.0);
+    par_for(m_k, [&](long long r0, long long r1) {
+        std::vector<int> next(L->rp.begin() + r0, L->rp.begin() + r1);
+        for (int c = 0; c < n; ++c) {
+            const int o = inv[(size_t)c];
+            const int *rb = s->k_row_idx + s->k_col_ptr[o];
+            const int *re = s->k_row_idx + s->k_col_ptr[o + 1];
+            const int *lo = std::lower_bound(rb, re, (int)r0);
+            for (const int *q = lo; q != re && *q < r1; ++q) {
+                const int pos = next[(size_t)(*q - r0)]++;
+                L->ci[(size_t)pos] = c;
+                L->rv[(size_t)pos] = s->k_val[q - s->k_row_idx];
+            }
+        }
+    });
+    for (int r = 0; r < m_d; ++r) {
+        int pos = L->rp[(size_t)m_k + r];
+        for (int e = s->d_row_ptr[r]; e < s->d_row_ptr[r + 1]; ++e, ++pos) {
+            L->ci[(size_t)pos] = perm[(size_t)s->d_col_idx[e]];
+            L->rv[(size_t)pos] = s->d_val[e];
+        }
+    }
+    // damping columns in the new order
+    L->dcp.assign((size_t)n + 1, 0);
+    for (int c = 0; c < n; ++c) {
+        const int o = inv[(size_t)c];
+        L->dcp[(size_t)c + 1] = L->dcp[(size_t)c] + (s->dc_col_ptr[o + 1] - s->dc_col_ptr[o]);
+    }
+    L->dri.assign((size_t)L->dcp[(size_t)n], 0);
+    L->dcv.assign((size_t)L->dcp[(size_t)n], 0.0);
+    for (int c = 0; c < n; ++c) {
+        const int o = inv[(size_t)c];
+        int pos = L->dcp[(size_t)c];
+        for (int e = s->dc_col_ptr[o]; e < s->dc_col_ptr[o + 1]; ++e, ++pos) {
+            L->dri[(size_t)pos] = s->dc_row_idx[e];
+            L->dcv[(size_t)pos] = s->dc_val[e];
+        }
+    }
+}
+
+}  // namespace
+
+int validate_system(const plsqr_system *s, std::string *err) {
+    if (!s) {
+        *err = "system is NULL";
+        return PLSQR_ERR_ARG;
+    }
+    if (s->m_k <= 0 || s->n <= 0 || s->m_d < 0 || s->nnz_k < 0 || s->nnz_d < 0) {
+        *err = "system dimensions must be positive";
+        return PLSQR_ERR_ARG;
+    }
+    if (s->nnz_k > INT_MAX || s->nnz_d > INT_MAX) {
+        *err = "more than 2^31-1 nonzeros in one system";
+        return PLSQR_ERR_ARG;
+    }
+    int rc;
+    if ((rc = check_ptr("kernel col_ptr", s->k_col_ptr, s->n, s->nnz_k, err)) ||
+        (rc = check_idx("kernel row_idx", s->k_row_idx, s->k_val, s->nnz_k, s->m_k, err)))
+        return rc;
+    for (int c = 0; c < s->n; ++c)
+        for (int e = s->k_col_ptr[c] + 1; e < s->k_col_ptr[c + 1]; ++e)
+            if (s->k_row_idx[e] <= s->k_row_idx[e - 1]) {
+                *err = "kernel rows not ascending in column " + std::to_string(c);
+                return PLSQR_ERR_FORMAT;
+            }
+    if (s->m_d > 0) {
+        if ((rc = check_ptr("damping row_ptr", s->d_row_ptr, s->m_d, s->nnz_d, err)) ||
+            (rc = check_idx("damping col_idx", s->d_col_idx, s->d_val, s->nnz_d, s->n, err)) ||
+            (rc = check_ptr("damping col_ptr", s->dc_col_ptr, s->n, s->nnz_d, err)) ||
+            (rc = check_idx("damping-by-column row_idx", s->dc_row_idx, s->dc_val, s->nnz_d, s->m_d, err)))
+            return rc;
+    } else if (s->nnz_d != 0) {
+        *err = "damping nonzeros without damping rows";
+        return PLSQR_ERR_FORMAT;
+    }
+    return PLSQR_OK;
+}
+
+int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids, int P, int TW, int row_chunk,
+                     int damp_chunk, int col_chunk, BatchPlan *plan, std::string *err) {
+    if (P <= 0 || !systems) {
+        *err = "no systems";
+        return PLSQR_ERR_ARG;
+    }
+    plan->P = P;
+    plan->TW = TW;
+    plan->prob.assign((size_t)P, ProblemLayout());
+    int rc;
+    for (int p = 0; p < P; ++p) {
+        const plsqr_system *s = systems[p];
+        if ((rc = validate_system(s, err))) {
+            *err = "system " + std::to_string(p) + ": " + *err;
+            return rc;
+        }
+        const plsqr_grid *g = grids ? &grids[p] : nullptr;
+        if (g && g->nx * (long long)g->ny * g->nz == 0) g = nullptr;
+        if (g && (long long)g->nx * g->ny * g->nz != s->n) {
+            *err = "system " + std::to_string(p) + ": grid dimensions do not multiply to the number of columns";
+            return PLSQR_ERR_ARG;
+        }
+        ProblemLayout &L = plan->prob[(size_t)p];
+        L.m_k = s->m_k;
+        L.m_d = s->m_d;
+        L.n = s->n;
+        std::vector<int> perm, tiles;
+        if (g) {
+            brick_order(g->nx, g->ny, g->nz, 4, 4, 8, true, perm, tiles);
+        } else {
+            perm.resize((size_t)s->n);
+            for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
+            for (int c = 0; c < s->n; c += TW) tiles.push_back(c);
+            tiles.push_back(s->n);
+        }
+        std::vector<int> slices{0, s->m_k};
+        if (!build_tile_layout(s, perm, tiles, slices, TW, &L.tiles, err)) {
+            *err = "system " + std::to_string(p) + ": " + *err;
+            return PLSQR_ERR_FORMAT;
+        }
+        L.inv = L.tiles.inv;
+        L.tiles.perm.clear();
+        L.tiles.perm.shrink_to_fit();
+        build_csr(s, perm, L.inv, &L);
+    }
+    // offsets and pieces
+    plan->row0.assign((size_t)P + 1, 0);
+    plan->col0.assign((size_t)P + 1, 0);
+    plan->nnz0.assign((size_t)P + 1, 0);
+    plan->dnnz0.assign((size_t)P + 1, 0);
+    plan->slot0.assign((size_t)P + 1, 0);
+    plan->seg0.assign((size_t)P + 1, 0);
+    plan->tile0.assign((size_t)P + 1, 0);
+    plan->rch0.assign((size_t)P + 1, 0);
+    plan->cch0.assign((size_t)P + 1, 0);
+    plan->rch_row.clear();
+    plan->rch_prob.clear();
+    plan->rch_kind.clear();
+    plan->cch_col.clear();
+    plan->cch_prob.clear();
+    int mx = 0;
+    for (int p = 0; p < P; ++p) {
+        const ProblemLayout &L = plan->prob[(size_t)p];
+        const long long r0 = plan->row0[(size_t)p], c0 = plan->col0[(size_t)p];
+        plan->row0[(size_t)p + 1] = r0 + L.m_k + L.m_d;
+        plan->col0[(size_t)p + 1] = c0 + L.n;
+        plan->nnz0[(size_t)p + 1] = plan->nnz0[(size_t)p] + (long long)L.rp.back();
+        plan->dnnz0[(size_t)p + 1] = plan->dnnz0[(size_t)p] + (long long)L.dcp.back();
+        plan->slot0[(size_t)p + 1] = plan->slot0[(size_t)p] + L.tiles.n_slot;
+        plan->seg0[(size_t)p + 1] = plan->seg0[(size_t)p] + L.tiles.n_seg;
+        plan->tile0[(size_t)p + 1] = plan->tile0[(size_t)p] + L.tiles.T;
+        mx = std::max(mx, L.tiles.max_seg_per_item);
+        for (int r = 0; r < L.m_k; r += row_chunk) {
+            plan->rch_row.push_back((int)(r0 + r));
+            plan->rch_prob.push_back(p);
+            plan->rch_kind.push_back(0);
+        }
+        for (int r = 0; r < L.m_d; r += damp_chunk) {
+            plan->rch_row.push_back((int)(r0 + L.m_k + r));
+            plan->rch_prob.push_back(p);
+            plan->rch_kind.push_back(1);
+        }
+        plan->rch0[(size_t)p + 1] = (int)plan->rch_prob.size();
+        for (int c = 0; c < L.n; c += col_chunk) {
+            plan->cch_col.push_back((int)(c0 + c));
+            plan->cch_prob.push_back(p);
+        }
+        plan->cch0[(size_t)p + 1] = (int)plan->cch_prob.size();
+    }
+    plan->rows = plan->row0[(size_t)P];
+    plan->cols = plan->col0[(size_t)P];
+    plan->nnz = plan->nnz0[(size_t)P];
+    plan->dnnz = plan->dnnz0[(size_t)P];
+    plan->slots = plan->slot0[(size_t)P];
+    plan->segs = plan->seg0[(size_t)P];
+    plan->tiles = plan->tile0[(size_t)P];
+    plan->max_seg_per_tile = mx;
+    plan->rch_row.push_back((int)plan->rows);
+    plan->cch_col.push_back((int)plan->cols);
+    if (plan->rows * 4 > INT_MAX || plan->cols * 4 > INT_MAX || plan->nnz > INT_MAX || plan->dnnz > INT_MAX) {
+        *err = "the batch exceeds 32-bit row, column or nonzero indexing";
+        return PLSQR_ERR_ARG;
+    }
+    return PLSQR_OK;
+}
+
+}  // namespace plsqr

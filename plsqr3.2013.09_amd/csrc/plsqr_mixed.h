@@ -14,45 +14,6 @@
 
 namespace plsqr {
 
-struct F2 {  // s + c, |c| << |s|
-    float s, c;
-};
-
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-__device__ __forceinline__ void dot2_add(F2 &acc, float a, float x) {
-    const float p = a * x;
-    const float ep = fmaf(a, x, -p);  // exact error of the product
-    float s, es;
-    two_sum(acc.s, p, s, es);
-    acc.s = s;
-    acc.c += ep + es;
-}
-
-__device__ __forceinline__ F2 f2_merge(F2 a, F2 b) {
-    F2 r;
-    float e;
-    two_sum(a.s, b.s, r.s, e);
-    r.c = (a.c + b.c) + e;
-    return r;
-}
-
-template <int W>
-__device__ __forceinline__ F2 group_sum_f2(F2 v) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        F2 t;
-        t.s = __shfl_xor(v.s, o, W);
-        t.c = __shfl_xor(v.c, o, W);
-        v = f2_merge(v, t);
-    }
-    return v;
-}
-
 // Mode 1 in FP32: y <- fl32(fl32(y*s1)*s2) + dot2(row, x); FP64 partial sums
 // of squares of the new y, pair layout (kernel rows, damping rows).
 template <int L, int BS>

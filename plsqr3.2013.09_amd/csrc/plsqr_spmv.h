@@ -544,217 +544,42 @@ __global__ void k_se_final(int n, double t, double *__restrict__ se) {
 }
 
 // ---------------------------------------------------------------------------
-// Scalar recurrences (one block; thread 0 does the scalar work)
+// Scalar recurrences (one block; thread 0 runs the device functions of
+// plsqr_kernels.h on the block-reduced sums)
 // ---------------------------------------------------------------------------
-__device__ void trace_row(double *trace, const Scal &s, double alpha) {
-    if (!trace) return;
-    double *t = trace + (size_t)s.itn * 8;
-    t[0] = alpha;
-    t[1] = s.beta;
-    t[2] = s.rnorm;
-    t[3] = s.test1;
-    t[4] = s.test2;
-    t[5] = s.anorm;
-    t[6] = s.acond;
-    t[7] = s.xnorm;
-}
-
-// Start: beta = ||[b;0]|| (lsqr.c:343); u is scaled lazily (ib).
 __global__ __launch_bounds__(256) void k_s_init_beta(Scal *sc, const double *part, int count) {
     __shared__ double sh[4];
     const double sk = reduce_partials<256>(part, count, 2, sh);
     const double sd = reduce_partials<256>(part + 1, count, 2, sh);
-    if (threadIdx.x != 0) return;
-    Scal &s = *sc;
-    double beta;
-    if (s.exact) {
-        const double pk = s.pk, pd = sqrt(s.pd * s.pd);
-        beta = sqrt(pk * pk + pd * pd);
-    } else {
-        beta = sqrt(sk + sd);
-    }
-    s.beta = beta;
-    s.alpha = 0.0;
-    if (beta > 0.0) {
-        s.ib = 1.0 / beta;
-        s.beta_pos = 1;
-    } else {
-        s.ib = 1.0;
-        s.beta_pos = 0;
-    }
-    s.vscale = 1.0;  // first v += A^T u starts from v = 0 (lsqr.c:311,366)
+    if (threadIdx.x == 0) s_init_beta(*sc, sk, sd);
 }
 
-// After the first v += A^T u: alpha, w = v / alpha, initial estimates
-// (lsqr.c:375-445).
-__global__ __launch_bounds__(256) void k_s_init_alpha(Scal *sc, const double *part, int count,
-                                                      double *trace) {
+__global__ __launch_bounds__(256) void k_s_init_alpha(Scal *sc, const double *part, int count, double *trace) {
     __shared__ double sh[4];
     const double sv = reduce_partials<256>(part, count, 1, sh);
-    if (threadIdx.x != 0) return;
-    Scal &s = *sc;
-    double alpha = 0.0;
-    if (s.beta_pos) alpha = s.exact ? sqrt(s.pv * s.pv) : sqrt(sv);
-    s.alpha = alpha;
-    if (alpha > 0.0) {
-        s.inv_alpha = 1.0 / alpha;
-        s.scale_v = 1;
-    } else {
-        s.inv_alpha = 1.0;
-        s.scale_v = 0;
-    }
-    s.arnorm = alpha * s.beta;
-    if (s.arnorm == 0.0) {  // goto_800
-        s.done = 1;
-        s.init_zero = 1;
-        s.itn = 0;
-        s.rnorm = 0.0;
-        s.test1 = 0.0;
-        s.test2 = 0.0;
-        trace_row(trace, s, alpha);
-        return;
-    }
-    s.rhobar = alpha;
-    s.phibar = s.beta;
-    s.bnorm = s.beta;
-    s.rnorm = s.beta;
-    s.test1 = 1.0;
-    s.test2 = alpha / s.beta;
-    s.itn = 0;
-    trace_row(trace, s, alpha);
-    if (s.itn_max <= 0) s.done = 1;
+    if (threadIdx.x == 0) s_init_alpha(*sc, sv, trace);
 }
 
-// After u <- u*(-alpha) + A v: beta, anorm (lsqr.c:487, 516-528, 530-538, 558).
 __global__ __launch_bounds__(256) void k_s_beta(Scal *sc, const double *part, int count) {
     __shared__ double sh[4];
     if (sc->done) return;  // uniform across the block
     const double sk = reduce_partials<256>(part, count, 2, sh);
     const double sd = reduce_partials<256>(part + 1, count, 2, sh);
-    if (threadIdx.x != 0) return;
-    Scal &s = *sc;
-    s.itn = s.itn + 1;
-    double beta;
-    if (s.exact) {
-        const double pk = s.pk, pd = sqrt(s.pd * s.pd);
-        beta = sqrt(pk * pk + pd * pd);
-    } else {
-        beta = sqrt(sk + sd);
-    }
-    s.beta = beta;
-    double temp = d2norm(s.alpha, beta);
-    temp = d2norm(temp, s.damp);
-    s.anorm = d2norm(s.anorm, temp);
-    if (beta > 0.0) {
-        s.ib = 1.0 / beta;
-        s.vscale = -beta;
-        s.beta_pos = 1;
-    } else {
-        s.ib = 1.0;
-        s.beta_pos = 0;
-    }
+    if (threadIdx.x == 0) s_beta(*sc, sk, sd);
 }
 
-// After v <- v*(-beta) + A^T u: alpha and the two plane rotations
-// (lsqr.c:577-637).
 __global__ __launch_bounds__(256) void k_s_alpha(Scal *sc, const double *part, int count) {
     __shared__ double sh[4];
     if (sc->done) return;
     const double sv = reduce_partials<256>(part, count, 1, sh);
-    if (threadIdx.x != 0) return;
-    Scal &s = *sc;
-    s.scale_v = 0;
-    if (s.beta_pos) {
-        const double alpha = s.exact ? sqrt(s.pv * s.pv) : sqrt(sv);
-        s.alpha = alpha;
-        if (alpha > 0.0) {
-            s.inv_alpha = 1.0 / alpha;
-            s.scale_v = 1;
-        }
-    }
-    const double alpha = s.alpha, beta = s.beta;
-    double rhbar1 = s.rhobar;
-    if (s.damp > 0.0) {
-        rhbar1 = d2norm(s.rhobar, s.damp);
-        const double cs1 = s.rhobar / rhbar1;
-        const double sn1 = s.damp / rhbar1;
-        s.psi = sn1 * s.phibar;
-        s.phibar = cs1 * s.phibar;
-    }
-    const double rho = d2norm(rhbar1, beta);
-    const double cs = rhbar1 / rho;
-    const double sn = beta / rho;
-    const double theta = sn * alpha;
-    s.rhobar = -cs * alpha;
-    const double phi = cs * s.phibar;
-    s.phibar = sn * s.phibar;
-    const double tau = sn * phi;
-    s.phi = phi;
-    s.rho = rho;
-    s.t1 = phi / rho;
-    s.t2 = -theta / rho;
-    s.t3 = 1.0 / rho;
-    s.theta = theta;
-    s.arnorm = alpha * fabs(tau);  // lsqr.c:735 (alpha and tau are final here)
+    if (threadIdx.x == 0) s_alpha(*sc, sv);
 }
 
-// After the x/w update: dknorm, norm and condition estimates, test1..3,
-// istop and the stop decision (lsqr.c:703-775, 820-834).
-__global__ __launch_bounds__(256) void k_s_est(Scal *sc, const double *part, int count,
-                                               double *trace) {
+__global__ __launch_bounds__(256) void k_s_est(Scal *sc, const double *part, int count, double *trace) {
     __shared__ double sh[4];
     if (sc->done) return;
     const double sk = reduce_partials<256>(part, count, 1, sh);
-    if (threadIdx.x != 0) return;
-    Scal &s = *sc;
-    const double theta = s.theta;
-    double dknorm = s.exact ? s.dk_seq : sk;
-    dknorm = sqrt(dknorm);
-    s.dnorm = d2norm(s.dnorm, dknorm);
-    const double dxk = fabs(s.phi * dknorm);
-    if (s.dxmax < dxk) {
-        s.dxmax = dxk;
-        s.maxdx = s.itn;
-    }
-    const double rho = s.rho;
-    const double delta = s.sn2 * rho;
-    const double gambar = -s.cs2 * rho;
-    const double rhs = s.phi - delta * s.z;
-    const double zbar = rhs / gambar;
-    s.xnorm = d2norm(s.xnorm1, zbar);
-    const double gamma = d2norm(gambar, theta);
-    s.cs2 = gambar / gamma;
-    s.sn2 = theta / gamma;
-    s.z = rhs / gamma;
-    s.xnorm1 = d2norm(s.xnorm1, s.z);
-    s.acond = s.anorm * s.dnorm;
-    s.res2 = d2norm(s.res2, s.psi);
-    s.rnorm = d2norm(s.res2, s.phibar);
-    s.test1 = s.rnorm / s.bnorm;
-    s.test2 = 0.0;
-    if (s.rnorm > 0.0) s.test2 = s.arnorm / (s.anorm * s.rnorm);
-    s.test3 = 1.0 / s.acond;
-    double t1 = s.test1 / (1.0 + s.anorm * s.xnorm / s.bnorm);
-    const double rtol = s.btol + s.atol * s.anorm * s.xnorm / s.bnorm;
-    const double t3 = 1.0 + s.test3;
-    const double t2 = 1.0 + s.test2;
-    t1 = 1.0 + t1;
-    if (s.itn >= s.itnlim) s.istop = 5;
-    if (t3 <= 1.0) s.istop = 4;
-    if (t2 <= 1.0) s.istop = 2;
-    if (t1 <= 1.0) s.istop = 1;
-    if (s.test3 <= s.ctol) s.istop = 4;
-    if (s.test2 <= s.atol) s.istop = 2;
-    if (s.test1 <= rtol) s.istop = 1;
-    if (s.istop == 0) {
-        s.nstop = 0;
-    } else {
-        s.nstop = s.nstop + 1;
-        if (s.nstop < 1 && s.itn < s.itnlim) s.istop = 0;
-    }
-    trace_row(trace, s, s.alpha);
-    if (s.itn >= s.itn_max) s.done = 1;
-    if (s.tol > 0.0 && s.test2 <= s.tol) s.done = 1;
+    if (threadIdx.x == 0) s_est(*sc, sk, trace);
 }
 
 // ---------------------------------------------------------------------------

@@ -10,22 +10,6 @@ namespace plsqr {
 
 namespace {
 
-template <class F>
-void par_for(long long n, F &&f) {
-    unsigned nt = std::thread::hardware_concurrency();
-    if (const char *e = std::getenv("PLSQR_HOST_THREADS")) nt = (unsigned)std::max(1, atoi(e));
-    nt = std::max(1u, std::min(nt, 64u));
-    if (n < 64) nt = 1;
-    std::vector<std::thread> th;
-    const long long chunk = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        const long long b = (long long)t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        th.emplace_back([&f, b, e] { f(b, e); });
-    }
-    for (auto &x : th) x.join();
-}
-
 // rows of old column o inside [r0, r1): index range in the CSC arrays
 inline void slice_range(const plsqr_system *s, int o, int r0, int r1, int *b, int *e) {
     const int *rb = s->k_row_idx + s->k_col_ptr[o];

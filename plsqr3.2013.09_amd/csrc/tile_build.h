@@ -13,13 +13,34 @@
 // padding that the kernel never reads.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "plsqr_hip.h"
 
 namespace plsqr {
+
+// Runs f(b, e) over a split of [0, n) on up to PLSQR_HOST_THREADS (default:
+// all hardware threads, at most 64) host threads.
+template <class F>
+void par_for(long long n, F &&f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("PLSQR_HOST_THREADS")) nt = (unsigned)std::max(1, atoi(e));
+    nt = std::max(1u, std::min(nt, 64u));
+    if (n < 64) nt = 1;
+    std::vector<std::thread> th;
+    const long long chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const long long b = (long long)t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto &x : th) x.join();
+}
 
 struct TileLayout {
     int S = 1;   // row slices

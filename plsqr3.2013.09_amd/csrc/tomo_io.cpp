@@ -497,11 +497,24 @@ int plsqr_gen_rhs(const plsqr_system *sys, const plsqr_gen_params *p, double *b,
     if ((long long)p->nx * p->ny * p->nz != sys->n) return fail(PLSQR_ERR_ARG, "grid does not match the system's columns");
     Grid g{{p->nx, p->ny, p->nz}};
     make_blobs(g, p->seed, p->n_blobs, p->blob_width, x_true);
-    std::vector<long long> rp;
-    std::vector<int> col;
-    std::vector<double> val;
-    csc_to_csr(sys->m_k, sys->n, sys->k_col_ptr, sys->k_row_idx, sys->k_val, rp, col, val);
-    kernel_rhs(sys->m_k, rp.data(), col.data(), val.data(), x_true, p->seed, p->sigma, b);
+    // b[r] = sum over ascending columns of val * x + noise, as kernel_rhs
+    // forms it from the CSR; each host thread owns a range of rows and walks
+    // the columns in order, so no CSR copy is needed
+    const int *cp = sys->k_col_ptr, *ri = sys->k_row_idx;
+    const double *kv = sys->k_val;
+    parallel_for(sys->m_k, [&](long long lo, long long hi, unsigned) {
+        std::vector<double> acc((size_t)(hi - lo), 0.0);
+        for (int c = 0; c < sys->n; ++c) {
+            const int *rb = ri + cp[c], *re = ri + cp[c + 1];
+            const double xc = x_true[c];
+            for (const int *q = std::lower_bound(rb, re, (int)lo); q != re && *q < hi; ++q)
+                acc[(size_t)(*q - lo)] += kv[q - ri] * xc;
+        }
+        for (long long r = lo; r < hi; ++r) {
+            Stream st(p->seed, STREAM_NOISE, (uint64_t)r);
+            b[r] = acc[(size_t)(r - lo)] + p->sigma * st.normal();
+        }
+    });
     return PLSQR_OK;
 }
 
