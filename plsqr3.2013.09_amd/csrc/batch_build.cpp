@@ -5,6 +5,7 @@
 #include <atomic>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 
 namespace plsqr {
 
@@ -195,6 +196,11 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
             *err = "system " + std::to_string(p) + ": " + *err;
             return PLSQR_ERR_FORMAT;
         }
+        if (std::getenv("PLSQR_TILE_CHECK") && std::atoi(std::getenv("PLSQR_TILE_CHECK")) != 0 &&
+            !tile_layout_self_check(s, L.tiles, err)) {
+            *err = "system " + std::to_string(p) + ": " + *err;
+            return PLSQR_ERR_STATE;
+        }
         L.inv = L.tiles.inv;
         L.tiles.perm.clear();
         L.tiles.perm.shrink_to_fit();
@@ -207,6 +213,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     plan->dnnz0.assign((size_t)P + 1, 0);
     plan->slot0.assign((size_t)P + 1, 0);
     plan->seg0.assign((size_t)P + 1, 0);
+    plan->cut0.assign((size_t)P + 1, 0);
     plan->tile0.assign((size_t)P + 1, 0);
     plan->rch0.assign((size_t)P + 1, 0);
     plan->cch0.assign((size_t)P + 1, 0);
@@ -225,6 +232,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         plan->dnnz0[(size_t)p + 1] = plan->dnnz0[(size_t)p] + (long long)L.dcp.back();
         plan->slot0[(size_t)p + 1] = plan->slot0[(size_t)p] + L.tiles.n_slot;
         plan->seg0[(size_t)p + 1] = plan->seg0[(size_t)p] + L.tiles.n_seg;
+        plan->cut0[(size_t)p + 1] = plan->cut0[(size_t)p] + L.tiles.cut_base.back();
         plan->tile0[(size_t)p + 1] = plan->tile0[(size_t)p] + L.tiles.T;
         mx = std::max(mx, L.tiles.max_seg_per_item);
         for (int r = 0; r < L.m_k; r += row_chunk) {
@@ -250,6 +258,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     plan->dnnz = plan->dnnz0[(size_t)P];
     plan->slots = plan->slot0[(size_t)P];
     plan->segs = plan->seg0[(size_t)P];
+    plan->cuts = plan->cut0[(size_t)P];
     plan->tiles = plan->tile0[(size_t)P];
     plan->max_seg_per_tile = mx;
     plan->rch_row.push_back((int)plan->rows);

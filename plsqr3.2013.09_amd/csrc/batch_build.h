@@ -41,13 +41,13 @@ struct BatchPlan {
     int P = 0, TW = 128;
     std::vector<ProblemLayout> prob;
     // offsets (P + 1 entries each)
-    std::vector<long long> row0, col0, nnz0, dnnz0, slot0, seg0;
+    std::vector<long long> row0, col0, nnz0, dnnz0, slot0, seg0, cut0;
     std::vector<int> tile0, rch0, cch0;
     // pieces, global numbering
     std::vector<int> rch_row, rch_prob;  // row chunks: rch_row has count + 1 entries
     std::vector<uint8_t> rch_kind;       // 0 kernel rows, 1 damping rows
     std::vector<int> cch_col, cch_prob;  // column chunks: cch_col has count + 1 entries
-    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0;
+    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0;
     int tiles = 0;
     int max_seg_per_tile = 0;
 };

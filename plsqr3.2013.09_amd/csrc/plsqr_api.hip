@@ -621,26 +621,8 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         std::string why;
         if (!plsqr::build_tile_layout(s, perm, tiles, hb, TILE_W, L.get(), &why)) {
             L.reset();  // falls back to the sliced CSC
-        } else if (env_int("PLSQR_TILE_CHECK", 0) != 0) {
-            // compare the layout's per-slice column sums with the CSC's on the host
-            std::vector<double> u((size_t)s->m_k), Pt((size_t)L->S * s->n), Pr((size_t)L->S * s->n, 0.0);
-            for (int i = 0; i < s->m_k; ++i) u[(size_t)i] = std::sin(0.37 * i + 1.0);
-            plsqr::tile_layout_apply(*L, s->n, u.data(), Pt.data());
-            for (int o = 0; o < s->n; ++o) {
-                int sl = 0;
-                for (int e = s->k_col_ptr[o]; e < s->k_col_ptr[o + 1]; ++e) {
-                    const int r = s->k_row_idx[e];
-                    while (r >= hb[(size_t)sl + 1]) ++sl;
-                    Pr[(size_t)sl * s->n + L->perm[(size_t)o]] += s->k_val[e] * u[(size_t)r];
-                }
-            }
-            double dmax = 0.0, nrm = 0.0;
-            for (size_t i = 0; i < Pt.size(); ++i) {
-                dmax = std::max(dmax, std::fabs(Pt[i] - Pr[i]));
-                nrm = std::max(nrm, std::fabs(Pr[i]));
-            }
-            if (!(dmax <= 1e-12 * (nrm + 1.0)))
-                return fail(set_err(op, PLSQR_ERR_STATE, "tiled layout self-check failed: max diff " + std::to_string(dmax)));
+        } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(s, *L, &why)) {
+            return fail(set_err(op, PLSQR_ERR_STATE, why));
         }
     }
     hipError_t e = hipGetDevice(&op->dev);

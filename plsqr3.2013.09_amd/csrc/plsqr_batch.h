@@ -242,18 +242,20 @@ __global__ __launch_bounds__(BS) void kb_rows(int nch, const int *__restrict__ c
 // Mode 2 on tiles: v[c] <- fl(v*vscale) + (A_k^T u' + D^T u')[c] for the
 // columns of one brick per workgroup, u' = u*ib (the pending 1/beta).  The
 // rays crossing the brick are gathered once each (all K values) into LDS,
-// CAP at a time, and reused for every voxel they cross.  Right-hand sides
+// CAP at a time, and reused for every voxel they cross; the round cuts of the
+// layout give each voxel's slot range per round.  Right-hand sides
 // with beta = 0 keep v (lsqr.c:530, 577).  part[t*K + k] = the tile's sum of
 // squares of the new v.
 // ---------------------------------------------------------------------------
-template <class T, int K, int TW, int BS, int CAP>
+template <class T, int K, int TW, int BS, int CAP, int CUT>
 __global__ __launch_bounds__(BS) void kb_cols(
     const int *__restrict__ t_prob, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len, const int *__restrict__ seg_ray,
-    const T *__restrict__ sval, const uint16_t *__restrict__ sseg, const int *__restrict__ dcp,
+    const T *__restrict__ sval, const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
+    const uint16_t *__restrict__ vox_cut, const int *__restrict__ dcp,
     const int *__restrict__ dri, const double *__restrict__ dcv, const T *__restrict__ u, T *__restrict__ v,
     const Scal *__restrict__ sc, double *__restrict__ part) {
-    static_assert(BS % TW == 0 && CAP % BS == 0, "tile shape");
+    static_assert(BS % TW == 0 && CAP % BS == 0 && CAP % CUT == 0, "tile shape");
     constexpr int NW = BS / TW;
     constexpr int B = (CAP / BS) < 8 ? (CAP / BS) : 8;
     __shared__ T g[CAP * K];
@@ -317,39 +319,30 @@ __global__ __launch_bounds__(BS) void kb_cols(
             }
         }
         __syncthreads();
-        if (ns <= CAP) {
-            const T *sv = sval + base + vx;
-            const uint16_t *ss = sseg + base + vx;
-            for (; j + 3 * NW < len; j += 4 * NW) {
-                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
-                const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
-                const T a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+        // this round's slots of voxel vx: [j, jend), segments r0 .. hi-1
+        const int hi = r0 + cnt;
+        int jend = len;
+        if (hi < ns && vx < tw) jend = (int)vox_cut[(size_t)(cut_base[it] + hi / CUT - 1) * TW + vx];
+        const T *sv = sval + base + vx;
+        const uint16_t *ss = sseg + base + vx;
+        for (; j + 3 * NW < jend; j += 4 * NW) {
+            const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+            const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
+            const T a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    acc[k].add(a0, g[g0 * K + k]);
-                    acc[k].add(a1, g[g1 * K + k]);
-                    acc[k].add(a2, g[g2 * K + k]);
-                    acc[k].add(a3, g[g3 * K + k]);
-                }
+            for (int k = 0; k < K; ++k) {
+                acc[k].add(a0, g[g0 * K + k]);
+                acc[k].add(a1, g[g1 * K + k]);
+                acc[k].add(a2, g[g2 * K + k]);
+                acc[k].add(a3, g[g3 * K + k]);
             }
-            for (; j < len; j += NW) {
-                const long long o = (long long)j * TW;
-                const int g0 = ss[o];
-                const T a0 = sv[o];
+        }
+        for (; j < jend; j += NW) {
+            const long long o = (long long)j * TW;
+            const int g0 = ss[o] - r0;
+            const T a0 = sv[o];
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k].add(a0, g[g0 * K + k]);
-            }
-        } else {
-            const int hi = r0 + cnt;
-            while (j < len) {
-                const long long sl = base + (long long)j * TW + vx;
-                const int sg = sseg[sl];
-                if (sg >= hi) break;
-                const T a0 = sval[sl];
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k].add(a0, g[(sg - r0) * K + k]);
-                j += NW;
-            }
+            for (int k = 0; k < K; ++k) acc[k].add(a0, g[g0 * K + k]);
         }
         __syncthreads();
     }

@@ -44,7 +44,7 @@ struct plsqr_batch {
     hipStream_t stream = nullptr;
     int cu = 256;
     int P = 0, K = 1, Kr = 1;
-    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0;
+    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0;
     int tiles = 0, nrch = 0, ncch = 0;
     // caller layout offsets (host)
     std::vector<int> mk, md, np;
@@ -56,8 +56,8 @@ struct plsqr_batch {
     int *dcp = nullptr, *dri = nullptr;
     double *dcv = nullptr;
     int *t_prob = nullptr, *t_start = nullptr, *seg_ray = nullptr;
-    long long *slot_base = nullptr, *seg_start = nullptr;
-    uint16_t *vox_len = nullptr, *sseg = nullptr;
+    long long *slot_base = nullptr, *seg_start = nullptr, *cut_base = nullptr;
+    uint16_t *vox_len = nullptr, *sseg = nullptr, *vox_cut = nullptr;
     double *sval = nullptr;
     float *sval32 = nullptr;
     // pieces and ranges
@@ -179,9 +179,9 @@ int cols_k(plsqr_batch *b, const Shape &s, const T *sv, const T *uv, T *vv, cons
     const dim3 g(b->tiles);
 #define COLS_CASE(BS_)                                                                                        \
     if (s.bs2 == BS_) {                                                                                      \
-        hipLaunchKernelGGL((kb_cols<T, K, TW, BS_, CAP>), g, dim3(BS_), 0, b->stream, b->t_prob, b->t_start,    \
-                           b->slot_base, b->seg_start, b->vox_len, b->seg_ray, sv, b->sseg, b->dcp, b->dri,     \
-                           b->dcv, uv, vv, sc, b->part);                                                      \
+        hipLaunchKernelGGL((kb_cols<T, K, TW, BS_, CAP, plsqr::TileLayout::CUT>), g, dim3(BS_), 0, b->stream,   \
+                           b->t_prob, b->t_start, b->slot_base, b->seg_start, b->vox_len, b->seg_ray, sv,       \
+                           b->sseg, b->cut_base, b->vox_cut, b->dcp, b->dri, b->dcv, uv, vv, sc, b->part);       \
         BT_LAUNCH(b);                                                                                         \
         return PLSQR_OK;                                                                                      \
     }
@@ -734,6 +734,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     b->dnnz = plan.dnnz;
     b->slots = plan.slots;
     b->segs = plan.segs;
+    b->cuts = plan.cuts;
     b->tiles = plan.tiles;
     b->nrch = (int)plan.rch_prob.size();
     b->ncch = (int)plan.cch_prob.size();
@@ -762,6 +763,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->slot_base, (size_t)b->tiles + 1)) || (rc = balloc(b, &b->seg_start, (size_t)b->tiles + 1)) ||
         (rc = balloc(b, &b->vox_len, (size_t)b->tiles * TW)) || (rc = balloc(b, &b->seg_ray, (size_t)b->segs)) ||
         (rc = balloc(b, &b->sval, (size_t)b->slots)) || (rc = balloc(b, &b->sseg, (size_t)b->slots)) ||
+        (rc = balloc(b, &b->cut_base, (size_t)b->tiles + 1)) || (rc = balloc(b, &b->vox_cut, (size_t)b->cuts * TW)) ||
         (rc = balloc(b, &b->rch_row, (size_t)b->nrch + 1)) || (rc = balloc(b, &b->rch_prob, (size_t)b->nrch)) ||
         (rc = balloc(b, &b->rch_kind, (size_t)b->nrch)) || (rc = balloc(b, &b->cch_col, (size_t)b->ncch + 1)) ||
         (rc = balloc(b, &b->cch_prob, (size_t)b->ncch)) || (rc = balloc(b, &b->r_rch, (size_t)P + 1)) ||
@@ -796,17 +798,20 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         const plsqr::TileLayout &T = L.tiles;
         const int t0 = plan.tile0[(size_t)p];
         std::vector<int> tp((size_t)T.T, p), ts((size_t)T.T);
-        std::vector<long long> sb((size_t)T.T), sg((size_t)T.T);
+        std::vector<long long> sb((size_t)T.T), sg((size_t)T.T), cb((size_t)T.T);
         for (int t = 0; t < T.T; ++t) {
             ts[(size_t)t] = (int)(c0 + T.tile_start[(size_t)t]);
             sb[(size_t)t] = plan.slot0[(size_t)p] + T.slot_base[(size_t)t];
             sg[(size_t)t] = plan.seg0[(size_t)p] + T.seg_start[(size_t)t];
+            cb[(size_t)t] = plan.cut0[(size_t)p] + T.cut_base[(size_t)t];
         }
         std::vector<int> ray(T.seg_ray.size());
         for (size_t i = 0; i < ray.size(); ++i) ray[i] = (int)(r0 + T.seg_ray[i]);
         if ((rc = upload(b, b->t_prob + t0, tp.data(), tp.size())) || (rc = upload(b, b->t_start + t0, ts.data(), ts.size())) ||
             (rc = upload(b, b->slot_base + t0, sb.data(), sb.size())) ||
             (rc = upload(b, b->seg_start + t0, sg.data(), sg.size())) ||
+            (rc = upload(b, b->cut_base + t0, cb.data(), cb.size())) ||
+            (rc = upload(b, b->vox_cut + (size_t)plan.cut0[(size_t)p] * TW, T.vox_cut.data(), T.vox_cut.size())) ||
             (rc = upload(b, b->vox_len + (size_t)t0 * TW, T.vox_len.data(), T.vox_len.size())) ||
             (rc = upload(b, b->seg_ray + plan.seg0[(size_t)p], ray.data(), ray.size())) ||
             (rc = upload(b, b->sval + plan.slot0[(size_t)p], T.sval.data(), T.sval.size())) ||
@@ -825,13 +830,14 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     const int rows_i = (int)b->rows, cols_i = (int)b->cols;
     const int nnz_i = (int)b->nnz, dnnz_i = (int)b->dnnz;
     std::vector<int> tail_t{(int)b->cols};
-    std::vector<long long> tail_sb{b->slots}, tail_sg{b->segs};
+    std::vector<long long> tail_sb{b->slots}, tail_sg{b->segs}, tail_cb{b->cuts};
     std::vector<int> mrow((size_t)P);
     for (int p = 0; p < P; ++p) mrow[(size_t)p] = b->mk[(size_t)p] + b->md[(size_t)p];
     if ((rc = upload(b, b->rp + rows_i, &nnz_i, 1)) || (rc = upload(b, b->dcp + cols_i, &dnnz_i, 1)) ||
         (rc = upload(b, b->t_start + b->tiles, tail_t.data(), 1)) ||
         (rc = upload(b, b->slot_base + b->tiles, tail_sb.data(), 1)) ||
         (rc = upload(b, b->seg_start + b->tiles, tail_sg.data(), 1)) ||
+        (rc = upload(b, b->cut_base + b->tiles, tail_cb.data(), 1)) ||
         (rc = upload(b, b->rch_row, plan.rch_row.data(), plan.rch_row.size())) ||
         (rc = upload(b, b->rch_prob, plan.rch_prob.data(), plan.rch_prob.size())) ||
         (rc = upload(b, b->rch_kind, plan.rch_kind.data(), plan.rch_kind.size())) ||

@@ -2,6 +2,7 @@
 #include "tile_build.h"
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdlib>
 #include <thread>
@@ -100,12 +101,16 @@ bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
     }
     L->seg_start.assign((size_t)items + 1, 0);
     L->slot_base.assign((size_t)items + 1, 0);
+    L->cut_base.assign((size_t)items + 1, 0);
     int mx = 0;
     for (long long it = 0; it < items; ++it) {
         L->seg_start[(size_t)it + 1] = L->seg_start[(size_t)it] + nseg[(size_t)it];
         L->slot_base[(size_t)it + 1] = L->slot_base[(size_t)it] + (long long)depth[(size_t)it] * TW;
+        const int ncut = nseg[(size_t)it] > 0 ? (nseg[(size_t)it] - 1) / TileLayout::CUT : 0;
+        L->cut_base[(size_t)it + 1] = L->cut_base[(size_t)it] + ncut;
         mx = std::max(mx, nseg[(size_t)it]);
     }
+    L->vox_cut.assign((size_t)L->cut_base[(size_t)items] * TW, 0);
     L->max_seg_per_item = mx;
     if (mx > 65535) {
         *err = "an item needs more than 65535 segments";
@@ -128,12 +133,18 @@ bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
                 for (int v = 0; v < tw; ++v) {
                     int b, e;
                     slice_range(s, L->inv[(size_t)(c0 + v)], slice_bounds[(size_t)sl], slice_bounds[(size_t)sl + 1], &b, &e);
+                    const long long cb = L->cut_base[(size_t)it], ncut = L->cut_base[(size_t)it + 1] - cb;
+                    long long q = 0;
                     for (int k = b; k < e; ++k) {
                         const long long slot = base + (long long)(k - b) * TW + v;
                         const int seg = (int)(std::lower_bound(rows.begin(), rows.end(), s->k_row_idx[k]) - rows.begin());
                         L->sval[(size_t)slot] = s->k_val[k];
                         L->sseg[(size_t)slot] = (uint16_t)seg;
+                        // slots ascend in segment: close every cut this slot passes
+                        for (; q < ncut && seg >= (q + 1) * TileLayout::CUT; ++q)
+                            L->vox_cut[(size_t)((cb + q) * TW + v)] = (uint16_t)(k - b);
                     }
+                    for (; q < ncut; ++q) L->vox_cut[(size_t)((cb + q) * TW + v)] = (uint16_t)(e - b);
                 }
                 std::vector<int>().swap(rows_of[(size_t)it]);
             }
@@ -142,8 +153,9 @@ bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
     return true;
 }
 
-void tile_layout_apply(const TileLayout &L, int n, const double *u, double *P) {
+bool tile_layout_apply(const TileLayout &L, int n, const double *u, double *P) {
     const int S = L.S, TW = L.TW;
+    std::atomic<bool> ok{true};
     par_for(L.T, [&](long long tb, long long te) {
         std::vector<double> g;
         for (long long t = tb; t < te; ++t) {
@@ -151,20 +163,60 @@ void tile_layout_apply(const TileLayout &L, int n, const double *u, double *P) {
             for (int sl = 0; sl < S; ++sl) {
                 const long long it = t * S + sl;
                 const long long s0 = L.seg_start[(size_t)it], ns = L.seg_start[(size_t)it + 1] - s0;
+                const long long cb = L.cut_base[(size_t)it], ncut = L.cut_base[(size_t)it + 1] - cb;
                 g.resize((size_t)ns);
                 for (long long i = 0; i < ns; ++i) g[(size_t)i] = u[L.seg_ray[(size_t)(s0 + i)]];
                 for (int v = 0; v < tw; ++v) {
                     double acc = 0.0;
                     const int len = L.vox_len[(size_t)it * TW + v];
-                    for (int j = 0; j < len; ++j) {
-                        const long long slot = L.slot_base[(size_t)it] + (long long)j * TW + v;
-                        acc += L.sval[(size_t)slot] * g[L.sseg[(size_t)slot]];
+                    int j = 0;
+                    for (long long q = 0; q <= ncut; ++q) {
+                        const long long lo = q * TileLayout::CUT, hi = (q + 1) * TileLayout::CUT;
+                        const int jend = q < ncut ? L.vox_cut[(size_t)((cb + q) * TW + v)] : len;
+                        if (jend < j || jend > len) ok.store(false);
+                        for (; j < jend && j < len; ++j) {
+                            const long long slot = L.slot_base[(size_t)it] + (long long)j * TW + v;
+                            const int sg = L.sseg[(size_t)slot];
+                            if (sg < lo || (q < ncut && sg >= hi)) ok.store(false);
+                            acc += L.sval[(size_t)slot] * g[(size_t)sg];
+                        }
                     }
+                    if (j != len) ok.store(false);
                     P[(size_t)sl * n + c0 + v] = acc;
                 }
             }
         }
     });
+    return ok.load();
+}
+
+bool tile_layout_self_check(const plsqr_system *s, const TileLayout &L, std::string *err) {
+    const int n = s->n;
+    const std::vector<int> &hb = L.slice_bounds;
+    std::vector<double> u((size_t)s->m_k), Pt((size_t)L.S * n), Pr((size_t)L.S * n, 0.0);
+    for (int i = 0; i < s->m_k; ++i) u[(size_t)i] = std::sin(0.37 * i + 1.0);
+    if (!tile_layout_apply(L, n, u.data(), Pt.data())) {
+        *err = "tiled layout self-check failed: round cuts disagree with the slots";
+        return false;
+    }
+    for (int o = 0; o < n; ++o) {
+        int sl = 0;
+        for (int e = s->k_col_ptr[o]; e < s->k_col_ptr[o + 1]; ++e) {
+            const int r = s->k_row_idx[e];
+            while (r >= hb[(size_t)sl + 1]) ++sl;
+            Pr[(size_t)sl * n + L.perm[(size_t)o]] += s->k_val[e] * u[(size_t)r];
+        }
+    }
+    double dmax = 0.0, nrm = 0.0;
+    for (size_t i = 0; i < Pt.size(); ++i) {
+        dmax = std::max(dmax, std::fabs(Pt[i] - Pr[i]));
+        nrm = std::max(nrm, std::fabs(Pr[i]));
+    }
+    if (!(dmax <= 1e-12 * (nrm + 1.0))) {
+        *err = "tiled layout self-check failed: max diff " + std::to_string(dmax);
+        return false;
+    }
+    return true;
 }
 
 }  // namespace plsqr

@@ -56,6 +56,13 @@ struct TileLayout {
     std::vector<int> seg_ray;          // segments
     std::vector<double> sval;          // slots
     std::vector<uint16_t> sseg;        // slots
+    // Round cuts for items with more than CUT segments: the kernel stages the
+    // segments CUT (or a multiple) at a time, and vox_cut[(cut_base[item] +
+    // q) * TW + v] = the number of voxel v's slots whose segment is below
+    // (q + 1) * CUT, so every round is a plain loop over a known slot range.
+    static constexpr int CUT = 1024;
+    std::vector<long long> cut_base;   // items+1
+    std::vector<uint16_t> vox_cut;     // cut_base[items] * TW
     long long n_seg = 0, n_slot = 0;
     int max_seg_per_item = 0;
 };
@@ -79,6 +86,13 @@ bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
 
 // Host evaluation of the per-slice column sums P[s*n + new_col] from the
 // layout, used to check the layout against the CSC before it is uploaded.
-void tile_layout_apply(const TileLayout &L, int n, const double *u, double *P);
+// The slots are visited round by round (CUT segments per round) through the
+// round cuts, as the batch kernel does; returns false if a cut disagrees with
+// the segments of the slots it bounds.
+bool tile_layout_apply(const TileLayout &L, int n, const double *u, double *P);
+
+// Compares tile_layout_apply with the CSC of s on a fixed test vector (the
+// layout's own permutation, slices and cuts); false with *err on a mismatch.
+bool tile_layout_self_check(const plsqr_system *s, const TileLayout &L, std::string *err);
 
 }  // namespace plsqr

@@ -82,6 +82,21 @@ def test_batch_create_validates_on_the_host(P):
         assert e.code == 4, str(e)
 
 
+def test_batch_layout_self_check_on_host(P, monkeypatch):
+    """With PLSQR_TILE_CHECK=1 every problem's brick layout (one row slice,
+    so bricks crossed by more than 1024 rays are staged in rounds) is
+    evaluated round by round through its cuts on the host and compared with
+    the CSC before any device call; without a GPU creation then stops at
+    hipGetDevice (code 4), which means the layout passed."""
+    monkeypatch.setenv("PLSQR_TILE_CHECK", "1")
+    systems, _ = make_batch_inputs(P, SPECS, 1)
+    big = P.generate(16, 16, 16, 60000, 0, 7, 0.01, 0.1)   # ~3000 segments per brick
+    try:
+        P.Batch(systems + [big], 1).close()
+    except P.PlsqrError as e:
+        assert e.code == 4 and "self-check" not in str(e), str(e)
+
+
 # -- GPU -------------------------------------------------------------------------
 
 def announce(msg):
