@@ -229,6 +229,41 @@ def bench_batch(P, hip, a):
     hip.free(x_dev)
 
 
+def bench_learned(P, hip, a):
+    """Learned warm start on the 67^3 / 2M-ray system: train the network on
+    (A^T b, x*) pairs of fresh blob fields and noise, then time LSQR to tol on
+    held-out right-hand sides from zero (plain FP64, the reference method)
+    and from the network's x0."""
+    import importlib
+    W = importlib.import_module(P.__name__ + ".warmstart")
+    cfg = P.CONFIGS[a.workload]
+    t0 = time.perf_counter()
+    s = P.generate_config(a.workload)
+    gen_s = time.perf_counter() - t0
+    grid = (cfg["nx"], cfg["ny"], cfg["nz"])
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    log(f"[bench] generated {a.workload} ({gen_s:.1f} s); {a.train_pairs} training pairs, {a.eval_pairs} held out")
+    res, net = W.run(s, grid, a.train_pairs, a.eval_pairs, cfg["sigma"], 500000, a.target_tol, a.epochs, a.tol,
+                     a.refine_every, log=log)
+    if a.save_params:
+        np.save(a.save_params, net.get_params())
+    net.close()
+    S = res["summary"]
+    best = min(("mixed_warm", "fp64_warm", "mixed_zero"), key=lambda k: S[k]["ms_mean"])
+    out = {
+        "metric": METRIC, "value": round(S["mixed_warm"]["ms_mean"] / 1000.0, 6), "unit": "s",
+        "n_gpus": 1, "steps": a.eval_pairs, "warmup": 0, "ms_per_step": round(S["mixed_warm"]["ms_mean"], 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 + bf16 network",
+        "data": "synthetic (seeded straight-ray generator; fixed ray geometry, fresh blob fields and noise "
+                "per pair)",
+        "config": {"workload": a.workload + "+learned_warm_start", "train_pairs": a.train_pairs,
+                   "eval_pairs": a.eval_pairs, "target_tol": a.target_tol, "epochs": a.epochs, "tol": a.tol,
+                   "refine_every": a.refine_every, "parallelism": "single GPU"},
+        "summary": S, "best_method": best, "times": res["times"], "loss": res["loss"], "targets": res["targets"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +283,15 @@ def main():
                          "(the workload for rocprofv3 counter passes)")
     ap.add_argument("--exact-order", action="store_true",
                     help="operator summing every row/column in the reference's order")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned warm start: train the network, then time LSQR from its x0 against "
+                         "plain FP64 LSQR from zero on held-out right-hand sides")
+    ap.add_argument("--train-pairs", type=int, default=512)
+    ap.add_argument("--eval-pairs", type=int, default=8)
+    ap.add_argument("--target-tol", type=float, default=1e-3,
+                    help="test2 the FP64 training targets x* are solved to")
+    ap.add_argument("--epochs", type=float, default=10)
+    ap.add_argument("--save-params", default=None, help="write the trained parameters (.npy)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -257,6 +301,9 @@ def main():
     hip = Hip()
     if a.workload in P.BATCH_CONFIGS:
         bench_batch(P, hip, a)
+        return
+    if a.learned:
+        bench_learned(P, hip, a)
         return
     t0 = time.perf_counter()
     s = P.generate_config(a.workload)

@@ -279,6 +279,75 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
 int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_host,
                          double *rnorm, double *arnorm, double *anorm_f);
 
+/* Warm start: solve from x0 (host, n doubles, caller's column order; NULL =
+ * zero start).  x0 is kept only when ||[b;0] - A x0|| <= ||[b;0]||, the
+ * residual of the zero start; otherwise the solve starts from 0.  *used_x0
+ * (may be NULL) reports which.  FP64: LSQR on r0 = [b;0] - A x0, x = x0 +
+ * its solution (test2 is that of the full problem, since the residuals are
+ * the same); MIXED: the FP64 refinement loop starts from x0.  out->bnorm is
+ * then ||r0||.  No counterpart in the reference (lsqr.c:311-312 always
+ * starts from zero); used with the warm-start network below. */
+int plsqr_lsqr_solve_x0(plsqr_op *op, const double *b_host, const double *x0_host,
+                        const plsqr_solve_opts *opts, double *x_host, plsqr_solve_out *out,
+                        int *used_x0);
+int plsqr_lsqr_solve_x0_device(plsqr_op *op, const double *b_dev, const double *x0_dev,
+                               const plsqr_solve_opts *opts, double *x_dev, plsqr_solve_out *out,
+                               int *used_x0);
+
+/* x_true field of the generator alone (the blob sum of plsqr_gen_system for
+ * p's grid, seed, n_blobs and blob_width): nx*ny*nz doubles. */
+int plsqr_gen_field(const plsqr_gen_params *p, double *x_true);
+
+/* ---------------------------------------------------------------------- */
+/* Warm-start network (no counterpart in the reference)                    */
+/* ---------------------------------------------------------------------- */
+/* A 3-D convolutional network on the nx*ny*nz voxel grid maps the
+ * back-projection A^T b to a starting point x0: 4 layers of 3x3x3
+ * convolutions with zero padding, 32 channels, ReLU between layers, bf16
+ * weights and activations with FP32 accumulation on the matrix cores, FP32
+ * master weights trained with Adam on the GPU.  Each sample is scaled by the
+ * RMS s of its A^T b (input A^T b / s, prediction s * output), so the map
+ * commutes with scaling b.  Parameters (plsqr_cnn_param_count() floats):
+ * for each layer l = 0..3, weights w[t][n][k] (tap t = (dz+1)*9 + (dy+1)*3
+ * + (dx+1) reading voxel (x+dx, y+dy, z+dz), output channel n, input
+ * channel k) then bias[n]; the input has one real channel (k = 0) and the
+ * output one (n = 0), the other entries of the first and last layer are 0.
+ * The last parameter is the output gain g (not trained; plsqr_cnn_set_data
+ * sets it to the RMS of the scaled targets xstar / s): x0 = s * g * output. */
+typedef struct plsqr_cnn plsqr_cnn;
+int  plsqr_cnn_param_count(void);
+int  plsqr_cnn_create(int nx, int ny, int nz, int max_batch, unsigned long long seed,
+                      plsqr_cnn **out);
+void plsqr_cnn_destroy(plsqr_cnn *h);
+/* last error of h; with h == NULL the last failure of a call without one */
+const char *plsqr_cnn_last_error(const plsqr_cnn *h);
+size_t plsqr_cnn_device_bytes(const plsqr_cnn *h);
+int  plsqr_cnn_get_params(plsqr_cnn *h, float *params);
+int  plsqr_cnn_set_params(plsqr_cnn *h, const float *params);
+/* Training pairs (host, count * nx*ny*nz doubles each): A^T b and the LSQR
+ * solution x*; kept on the device, scaled per sample. */
+int  plsqr_cnn_set_data(plsqr_cnn *h, int count, const double *atb, const double *xstar);
+/* steps Adam steps on minibatches of `batch` pairs drawn without
+ * replacement per epoch (seeded shuffle); loss_out (may be NULL, steps
+ * doubles) receives each step's loss, the mean over the minibatch of
+ * ||x0 - x*||^2 / ||x*||^2 in the scaled variables. */
+int  plsqr_cnn_train(plsqr_cnn *h, int steps, int batch, float lr, unsigned long long seed,
+                     double *loss_out);
+/* x0 for count samples of A^T b (host arrays). */
+int  plsqr_cnn_predict(plsqr_cnn *h, int count, const double *atb, double *x0);
+/* Loss and its gradient with respect to every parameter (params layout) for
+ * one minibatch (batch <= max_batch), without updating the parameters. */
+int  plsqr_cnn_gradient(plsqr_cnn *h, int batch, const double *atb, const double *xstar,
+                        float *grads, double *loss);
+/* One 32 -> 32 layer on host arrays in[g][k], out[g][n] (g = sample*V + voxel):
+ * operands are rounded to bf16, products summed in FP32.  backward: din is
+ * the data gradient of dout (no activation derivative), dw and dbias the
+ * parameter gradients. */
+int  plsqr_cnn_conv_forward(int nx, int ny, int nz, int batch, const float *in, const float *w,
+                            const float *bias, int relu, float *out);
+int  plsqr_cnn_conv_backward(int nx, int ny, int nz, int batch, const float *in, const float *w,
+                             const float *dout, float *din, float *dw, float *dbias);
+
 /* ---------------------------------------------------------------------- */
 /* Batched solves: many independent systems, up to 4 right-hand sides each */
 /* ---------------------------------------------------------------------- */

@@ -19,8 +19,8 @@ from ._lib import (  # noqa: F401
     BATCH_CONFIGS, BATCH_MAX_RHS, BATCH_NO_TUNE, BATCH_TUNE_MIXED, BatchTuningC, GridC, CONFIGS,
     ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT, OP_EXACT_ORDER,
     OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH, PlsqrError,
-    SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, gen_rhs, generate, generate_batch,
-    generate_config, load, read_x, write_x, _check,
+    SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, fptr, gen_field, gen_rhs, generate,
+    generate_batch, generate_config, load, read_x, write_x, _check,
 )
 
 
@@ -154,6 +154,31 @@ class Operator:
                                                 C.c_void_p(w_ptr) if w_ptr else None,
                                                 C.byref(out), None), self.h)
         return out.as_dict()
+
+    def lsqr_x0(self, b, x0, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+        """Warm start from x0 (None = zero start); returns (x, info) with info["used_x0"]."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        x = np.empty(self.n, np.float64)
+        out = SolveOutC()
+        used = C.c_int(0)
+        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        _check(self.lib.plsqr_lsqr_solve_x0(self.h, dptr(b), dptr(x0), C.byref(o), dptr(x), C.byref(out),
+                                            C.byref(used)), self.h)
+        info = out.as_dict()
+        info["used_x0"] = bool(used.value)
+        return x, info
+
+    def lsqr_x0_device(self, b_ptr, x0_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25):
+        out = SolveOutC()
+        used = C.c_int(0)
+        o = self.make_opts(itn, tol, precision, refine_every, 0.0)
+        _check(self.lib.plsqr_lsqr_solve_x0_device(self.h, C.c_void_p(b_ptr),
+                                                   C.c_void_p(x0_ptr) if x0_ptr else None, C.byref(o),
+                                                   C.c_void_p(x_ptr), C.byref(out), C.byref(used)), self.h)
+        info = out.as_dict()
+        info["used_x0"] = bool(used.value)
+        return info
 
     def residual_norms(self, b, x):
         rn, an, af = C.c_double(), C.c_double(), C.c_double()
@@ -295,3 +320,107 @@ class Batch:
 BATCH_SHAPES = [dict(rows_per_wave=r, nnz_per_lane=u, block_rows=b, block_cols=256)
                 for r in (16, 8, 4, 2) for b in (128, 256) for u in (2, 4)] + \
                [dict(rows_per_wave=4, nnz_per_lane=4, block_rows=256, block_cols=c) for c in (128, 512)]
+
+
+class WarmStartNet:
+    """The warm-start network (plsqr_cnn): A^T b on the voxel grid -> x0."""
+
+    def __init__(self, grid, max_batch=4, seed=1):
+        lib = load()
+        self.lib = lib
+        h = C.c_void_p()
+        nx, ny, nz = (int(v) for v in grid)
+        rc = lib.plsqr_cnn_create(nx, ny, nz, int(max_batch), int(seed), C.byref(h))
+        if rc != 0:
+            raise PlsqrError(rc, (lib.plsqr_cnn_last_error(None) or b"").decode(errors="replace"))
+        self.h = h
+        self.grid = (nx, ny, nz)
+        self.V = nx * ny * nz
+        self.max_batch = int(max_batch)
+        self.n_params = int(lib.plsqr_cnn_param_count())
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.plsqr_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PlsqrError(rc, (self.lib.plsqr_cnn_last_error(self.h) or b"").decode(errors="replace"))
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        self._check(self.lib.plsqr_cnn_get_params(self.h, fptr(p)))
+        return p
+
+    def set_params(self, p):
+        self._check(self.lib.plsqr_cnn_set_params(self.h, fptr(np.ascontiguousarray(p, np.float32))))
+
+    def set_data(self, atb, xstar):
+        atb = np.ascontiguousarray(atb, np.float64).reshape(-1, self.V)
+        xstar = np.ascontiguousarray(xstar, np.float64).reshape(-1, self.V)
+        assert atb.shape == xstar.shape
+        self._check(self.lib.plsqr_cnn_set_data(self.h, atb.shape[0], dptr(atb), dptr(xstar)))
+
+    def train(self, steps, batch, lr=1e-3, seed=7):
+        loss = np.empty(max(int(steps), 1), np.float64)
+        self._check(self.lib.plsqr_cnn_train(self.h, int(steps), int(batch), float(lr), int(seed), dptr(loss)))
+        return loss[:int(steps)]
+
+    def predict(self, atb):
+        atb = np.ascontiguousarray(atb, np.float64).reshape(-1, self.V)
+        x0 = np.empty_like(atb)
+        self._check(self.lib.plsqr_cnn_predict(self.h, atb.shape[0], dptr(atb), dptr(x0)))
+        return x0
+
+    def gradient(self, atb, xstar):
+        atb = np.ascontiguousarray(atb, np.float64).reshape(-1, self.V)
+        xstar = np.ascontiguousarray(xstar, np.float64).reshape(-1, self.V)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.plsqr_cnn_gradient(self.h, atb.shape[0], dptr(atb), dptr(xstar), fptr(g),
+                                                C.byref(loss)))
+        return loss.value, g
+
+
+def conv_forward(grid, inp, w, bias=None, relu=False):
+    """One 32 -> 32 layer through the MFMA kernel: inp (B, V, 32), w (27, 32, 32) [t][n][k]."""
+    lib = load()
+    nx, ny, nz = grid
+    inp = np.ascontiguousarray(inp, np.float32)
+    B = inp.shape[0]
+    out = np.empty_like(inp)
+    rc = lib.plsqr_cnn_conv_forward(nx, ny, nz, B, fptr(inp), fptr(np.ascontiguousarray(w, np.float32)),
+                                    fptr(None if bias is None else np.ascontiguousarray(bias, np.float32)),
+                                    int(bool(relu)), fptr(out))
+    if rc:
+        raise PlsqrError(rc, (lib.plsqr_cnn_last_error(None) or b"").decode(errors="replace"))
+    return out
+
+
+def conv_backward(grid, inp, w, dout):
+    lib = load()
+    nx, ny, nz = grid
+    inp = np.ascontiguousarray(inp, np.float32)
+    dout = np.ascontiguousarray(dout, np.float32)
+    B = inp.shape[0]
+    din = np.empty_like(inp)
+    dw = np.empty((27, 32, 32), np.float32)
+    db = np.empty(32, np.float32)
+    rc = lib.plsqr_cnn_conv_backward(nx, ny, nz, B, fptr(inp), fptr(np.ascontiguousarray(w, np.float32)),
+                                     fptr(dout), fptr(din), fptr(dw), fptr(db))
+    if rc:
+        raise PlsqrError(rc, (lib.plsqr_cnn_last_error(None) or b"").decode(errors="replace"))
+    return din, dw, db

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libplsqr_hip.so")
 
 c_int_p = C.POINTER(C.c_int)
 c_double_p = C.POINTER(C.c_double)
+c_float_p = C.POINTER(C.c_float)
 
 PLSQR_OK = 0
 ERR_NAMES = {0: "OK", 1: "ARG", 2: "BOUNDS", 3: "IO", 4: "HIP", 5: "NOMEM", 6: "STATE",
@@ -140,6 +141,30 @@ SIGNATURES = {
                                           c_double_p]),
     "plsqr_residual_norms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p,
                                        c_double_p, c_double_p]),
+    "plsqr_lsqr_solve_x0": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.POINTER(SolveOptsC),
+                                      c_double_p, C.POINTER(SolveOutC), c_int_p]),
+    "plsqr_lsqr_solve_x0_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.POINTER(SolveOptsC), C.c_void_p,
+                                             C.POINTER(SolveOutC), c_int_p]),
+    "plsqr_gen_field": (C.c_int, [C.POINTER(GenParamsC), c_double_p]),
+    "plsqr_cnn_param_count": (C.c_int, []),
+    "plsqr_cnn_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                                   C.POINTER(C.c_void_p)]),
+    "plsqr_cnn_destroy": (None, [C.c_void_p]),
+    "plsqr_cnn_last_error": (C.c_char_p, [C.c_void_p]),
+    "plsqr_cnn_device_bytes": (C.c_size_t, [C.c_void_p]),
+    "plsqr_cnn_get_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "plsqr_cnn_set_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "plsqr_cnn_set_data": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
+    "plsqr_cnn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong,
+                                  c_double_p]),
+    "plsqr_cnn_predict": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
+    "plsqr_cnn_gradient": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_float_p,
+                                     c_double_p]),
+    "plsqr_cnn_conv_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p,
+                                         c_float_p, C.c_int, c_float_p]),
+    "plsqr_cnn_conv_backward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p,
+                                          c_float_p, c_float_p, c_float_p, c_float_p]),
     "plsqr_batch_create": (C.c_int, [C.POINTER(C.POINTER(SystemC)), C.POINTER(GridC), C.c_int,
                                      C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "plsqr_batch_destroy": (None, [C.c_void_p]),
@@ -192,6 +217,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(c_double_p)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_float_p)
 
 
 def iptr(a):
@@ -302,6 +334,14 @@ def generate(nx, ny, nz, n_rays, geometry, seed, sigma, lam, w=1.0, n_blobs=5, b
     p = gen_params(nx, ny, nz, n_rays, geometry, seed, sigma, lam, w, n_blobs, blob_width)
     _check(lib.plsqr_gen_system(C.byref(p), C.byref(s)))
     return System(s, grid=(int(nx), int(ny), int(nz)))
+
+
+def gen_field(nx, ny, nz, seed, n_blobs=5, blob_width=3.0):
+    """The generator's x_true blob field alone."""
+    p = gen_params(nx, ny, nz, 1, 0, seed, 0.0, 0.0, 1.0, n_blobs, blob_width)
+    x = np.empty(int(nx) * int(ny) * int(nz), np.float64)
+    _check(load().plsqr_gen_field(C.byref(p), dptr(x)))
+    return x
 
 
 def gen_rhs(system, nx, ny, nz, seed, sigma, n_blobs=5, blob_width=3.0):

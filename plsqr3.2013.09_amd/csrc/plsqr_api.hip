@@ -69,6 +69,8 @@ struct plsqr_op {
     // FP32 copies for mixed-precision solves (allocated on first use)
     float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr;
     double *dx = nullptr;
+    // warm starts: x0 in the device column order and r0 = [b;0] - A x0
+    double *x0i = nullptr, *r0 = nullptr;
     int tiled_persistent = 0;  // > 0: workgroups of the persistent tiled kernel
     std::vector<void *> allocs;
     size_t bytes = 0;
@@ -852,8 +854,10 @@ static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, dou
     return PLSQR_OK;
 }
 
+// u_full: NULL for u = [b;0], else the full m-vector to start from (a warm
+// start's residual r0; the solve then returns the correction).
 static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, double *x, double *w,
-                      plsqr_solve_out *out, double *trace_host) {
+                      plsqr_solve_out *out, double *trace_host, const double *u_full = nullptr) {
     using namespace plsqr;
     int rc;
     const int itn = std::max(0, o->itn);
@@ -874,9 +878,13 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
     h.cs2 = -1.0;  // lsqr.c:293
     HIP_TRY(op, hipMemcpyAsync(op->sc, &h, sizeof(h), hipMemcpyHostToDevice, op->stream));
     // u = [b ; 0], v = x = w = 0 (tridiagonal.c:33-57, lsqr.c:311-312)
-    HIP_TRY(op, hipMemcpyAsync(op->u, b_dev, sizeof(double) * op->m_k, hipMemcpyDeviceToDevice, op->stream));
     const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
-    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m_d, 0.0, op->u + op->m_k);
+    if (u_full) {
+        HIP_TRY(op, hipMemcpyAsync(op->u, u_full, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
+    } else {
+        HIP_TRY(op, hipMemcpyAsync(op->u, b_dev, sizeof(double) * op->m_k, hipMemcpyDeviceToDevice, op->stream));
+        hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m_d, 0.0, op->u + op->m_k);
+    }
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->v);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, x);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, w);
@@ -1048,18 +1056,56 @@ static int host_sum(plsqr_op *op, const double *part, int count, double *out) {
     return PLSQR_OK;
 }
 
-// Mixed precision with FP64 outer refinement: r = [b;0] - A x in FP64, an
-// inner FP32 LSQR on min ||A dx - r|| for refine_every iterations, x += dx;
-// stops when ||A^T r|| / (anorm ||r||) <= tol in FP64 (anorm: the largest
-// LSQR estimate of ||A|| seen by the inner runs) or after itn inner steps.
-static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out) {
+// ||A^T r|| and ||r|| in FP64 for r = [b;0] - A xt (r into op->hy, A^T r
+// into op->hx)
+static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, double *rn, double *an) {
+    using namespace plsqr;
+    int rc;
+    const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
+    const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
+    const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k, op->m,
+                       op->rp, op->ci, op->rv, xt, b_dev, op->hy);
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, (const double *)op->hy,
+                       op->part);
+    LAUNCH_CHECK(op);
+    if ((rc = host_sum(op, op->part, gs, rn))) return rc;
+    *rn = std::sqrt(*rn);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->hx);
+    LAUNCH_CHECK(op);
+    if ((rc = aprod2_dev(op, op->hx, op->hy))) return rc;
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(gn), dim3(BS), 0, op->stream, (long long)op->n, (const double *)op->hx,
+                       op->part);
+    LAUNCH_CHECK(op);
+    if ((rc = host_sum(op, op->part, gn, an))) return rc;
+    *an = std::sqrt(*an);
+    return PLSQR_OK;
+}
+
+// Mixed precision: an inner LSQR in FP32 (values, u, v; Dot2 sums; FP64
+// scalars, w and correction dx) solves min ||A dx - r|| for r = [b;0] - A x.
+// Every refine_every inner iterations the FP64 residual of xt = x + dx is
+// formed and its test2 = ||A^T r|| / (anorm ||r||) checked in FP64 (anorm:
+// the largest LSQR estimate of ||A|| so far); the solve stops there when it
+// is <= tol.  The inner run is not interrupted by the check: it keeps its
+// Krylov basis, and is restarted on the refined residual (x = xt) only when
+// its own recurrence stops (its estimate of test2 reached tol) while the FP64
+// test2 has not.  itn bounds the inner iterations in total.
+// x_init: NULL for x = 0, else the starting x in the device column order.
+static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out,
+                       const double *x_init = nullptr) {
     using namespace plsqr;
     int rc;
     if ((rc = ensure_fp32(op))) return rc;
+    if (!op->x0i && ((rc = dalloc(op, &op->x0i, op->n)) || (rc = dalloc(op, &op->r0, op->m)))) return rc;
+    double *xt = op->x0i == x_init ? op->r0 : op->x0i;  // n doubles not aliasing x_init
     const int itn = std::max(0, o->itn);
-    const int period = o->refine_every > 0 ? o->refine_every : itn;
+    const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
-    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->x);
+    if (x_init)
+        HIP_TRY(op, hipMemcpyAsync(op->x, x_init, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    else
+        hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->x);
     LAUNCH_CHECK(op);
     hipEvent_t e0, e1;
     HIP_TRY(op, hipEventCreate(&e0));
@@ -1068,50 +1114,37 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
-    const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
-    double anorm_max = 0.0, rn = 0.0, test2 = INFINITY, an = 0.0;
-    int total = 0, outer = 0;
+    int *done_h = nullptr;
+    HIP_TRY(op, hipHostMalloc((void **)&done_h, sizeof(int)));
+    double anorm_max = 0.0, rn = 0.0, an = 0.0, test2 = INFINITY;
+    int total = 0, starts = 0, checks = 0;
+    bool converged = false;
     Scal h;
-    for (;;) {
-        // FP64 residual of the current x
-        hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
-                           op->m, op->rp, op->ci, op->rv, op->x, b_dev, op->hy);
-        LAUNCH_CHECK(op);
-        hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, op->hy, op->part);
-        LAUNCH_CHECK(op);
-        if ((rc = host_sum(op, op->part, gs, &rn))) return rc;
-        rn = std::sqrt(rn);
-        if (outer > 0) {
-            hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->hx);
-            LAUNCH_CHECK(op);
-            if ((rc = aprod2_dev(op, op->hx, op->hy))) return rc;
-            hipLaunchKernelGGL((k_ssq<BS>), dim3(gn), dim3(BS), 0, op->stream, (long long)op->n, op->hx, op->part);
-            LAUNCH_CHECK(op);
-            if ((rc = host_sum(op, op->part, gn, &an))) return rc;
-            an = std::sqrt(an);
-            test2 = (rn > 0.0 && anorm_max > 0.0) ? an / (anorm_max * rn) : 0.0;
-            if (test2 <= o->tol || total >= itn) break;
-        }
-        if (rn == 0.0 || itn == 0) {
-            test2 = 0.0;
-            break;
-        }
-        // inner FP32 LSQR on r
-        const int k = std::min(period, itn - total);
+    // r = [b;0] - A x for the first start
+    double rn0 = 0.0, an0 = 0.0;
+    if ((rc = true_norms(op, b_dev, op->x, &rn0, &an0))) return rc;
+    rn = rn0;
+    an = an0;
+    if (rn0 == 0.0 || itn == 0) {
+        test2 = 0.0;
+        converged = rn0 == 0.0;
+    }
+    while (!converged && total < itn) {
+        // (re)start the inner FP32 LSQR on r (in op->hy)
         Scal z;
         memset(&z, 0, sizeof(z));
-        z.damp = 0.0;
         z.atol = o->atol;
         z.btol = o->btol;
         z.ctol = o->conlim > 0.0 ? 1.0 / o->conlim : 0.0;
         z.tol = o->tol;
         z.itnlim = o->itnlim;
-        z.itn_max = k;
+        z.itn_max = itn - total;
         z.cs2 = -1.0;
         HIP_TRY(op, hipMemcpyAsync(op->sc, &z, sizeof(z), hipMemcpyHostToDevice, op->stream));
         hipLaunchKernelGGL(k_to_f32, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, op->hy, op->u32);
         hipLaunchKernelGGL(k_fill32, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, op->v32);
         hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->dx);
+        hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->w);
         hipLaunchKernelGGL((k_ssq2<BS>), dim3(gs), dim3(BS), 0, op->stream, op->m_k, op->m, op->hy, op->part);
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
@@ -1126,6 +1159,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         hipLaunchKernelGGL(k_init_vw32, dim3(gz), dim3(256), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
                            op->w);
         LAUNCH_CHECK(op);
+        ++starts;
         if (!exec) {
             HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
             rc = enqueue_iteration32(op);
@@ -1134,17 +1168,43 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
             if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
             HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
-        for (int i = 0; i < k; ++i) HIP_TRY(op, hipGraphLaunch(exec, op->stream));
-        HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
-        HIP_TRY(op, hipStreamSynchronize(op->stream));
-        anorm_max = std::max(anorm_max, h.anorm);
-        hipLaunchKernelGGL(k_axpy1, dim3(gz), dim3(256), 0, op->stream, op->n, op->dx, op->x);
-        LAUNCH_CHECK(op);
-        total += std::max(h.itn, 0);
-        ++outer;
-        if (h.itn == 0) {  // the inner run could not start (r orthogonal to the range): converged
-            test2 = 0.0;
-            break;
+        int inner_prev = 0;
+        for (;;) {
+            // a chunk of inner iterations, polling the inner stop every 8
+            const int k = std::min(period, itn - total);
+            int launched = 0;
+            for (int i = 0; i < k; ++i) {
+                HIP_TRY(op, hipGraphLaunch(exec, op->stream));
+                ++launched;
+                if ((i + 1) % 8 == 0 && i + 1 < k) {
+                    HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+                    HIP_TRY(op, hipStreamSynchronize(op->stream));
+                    if (*done_h) break;
+                }
+            }
+            HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
+            HIP_TRY(op, hipStreamSynchronize(op->stream));
+            const int inner = std::max(h.itn, 0);
+            total += inner - inner_prev;
+            inner_prev = inner;
+            anorm_max = std::max(anorm_max, h.anorm);
+            // FP64 check of xt = x + dx
+            hipLaunchKernelGGL(k_add3, dim3(gz), dim3(256), 0, op->stream, op->n, (const double *)op->x,
+                               (const double *)op->dx, xt);
+            LAUNCH_CHECK(op);
+            if ((rc = true_norms(op, b_dev, xt, &rn, &an))) return rc;
+            ++checks;
+            test2 = (rn > 0.0 && anorm_max > 0.0) ? an / (anorm_max * rn) : 0.0;
+            if (test2 <= o->tol || rn == 0.0) converged = true;
+            if (converged || total >= itn || h.done || launched == 0) {
+                // x = xt; op->hy now holds its residual for a restart
+                HIP_TRY(op, hipMemcpyAsync(op->x, xt, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+                if (inner == 0 && !converged) {  // the inner run could not start: nothing left to gain
+                    converged = test2 <= o->tol;
+                    total = itn;
+                }
+                break;
+            }
         }
     }
     HIP_TRY(op, hipEventRecord(e1, op->stream));
@@ -1153,19 +1213,22 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipHostFree(done_h);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (out) {
         memset(out, 0, sizeof(*out));
         out->itn = total;
         out->inner_total = total;
-        out->refinements = outer;
+        out->refinements = checks;
         out->anorm = anorm_max;
         out->rnorm = rn;
         out->arnorm = test2;
         out->arnorm_true = an;
+        out->bnorm = rn0;
         out->elapsed_ms = ms;
         out->istop = test2 <= o->tol ? 2 : 5;
+        (void)starts;
     }
     return PLSQR_OK;
 }
@@ -1221,6 +1284,83 @@ int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts 
         }
         HIP_TRY(op, hipStreamSynchronize(op->stream));
     }
+    return PLSQR_OK;
+}
+
+int plsqr_lsqr_solve_x0_device(plsqr_op *op, const double *b_dev, const double *x0_dev,
+                               const plsqr_solve_opts *opts, double *x_dev, plsqr_solve_out *out, int *used_x0) {
+    using namespace plsqr;
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_dev || !opts || !x_dev) return set_err(op, PLSQR_ERR_ARG, "b, opts and x are required");
+    if (op->exact) return set_err(op, PLSQR_ERR_ARG, "warm starts run on the default operator");
+    plsqr_solve_opts o;
+    default_opts(opts, &o);
+    int rc;
+    if (!op->x0i && ((rc = dalloc(op, &op->x0i, op->n)) || (rc = dalloc(op, &op->r0, op->m)))) return rc;
+    int use = 0;
+    if (x0_dev) {
+        // r0 = [b;0] - A x0; the warm start is kept only if ||r0|| <= ||[b;0]||,
+        // the residual of the zero start
+        if (op->permuted) {
+            if ((rc = to_internal(op, x0_dev, op->x0i))) return rc;
+        } else {
+            HIP_TRY(op, hipMemcpyAsync(op->x0i, x0_dev, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+        }
+        hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                           op->m, op->rp, op->ci, op->rv, (const double *)op->x0i, b_dev, op->r0);
+        LAUNCH_CHECK(op);
+        const int g = std::min(op->part_len, grid_for(op->m, BS, op->cu * 4));
+        double rn2 = 0.0, bn2 = 0.0;
+        hipLaunchKernelGGL((k_ssq<BS>), dim3(g), dim3(BS), 0, op->stream, (long long)op->m, (const double *)op->r0,
+                           op->part);
+        LAUNCH_CHECK(op);
+        if ((rc = host_sum(op, op->part, g, &rn2))) return rc;
+        hipLaunchKernelGGL((k_ssq<BS>), dim3(g), dim3(BS), 0, op->stream, (long long)op->m_k, b_dev, op->part);
+        LAUNCH_CHECK(op);
+        if ((rc = host_sum(op, op->part, g, &bn2))) return rc;
+        use = std::isfinite(rn2) && rn2 <= bn2;
+    }
+    if (used_x0) *used_x0 = use;
+    if (o.precision == PLSQR_PREC_MIXED) {
+        rc = solve_mixed(op, b_dev, &o, out, use ? op->x0i : nullptr);
+    } else if (o.precision == PLSQR_PREC_FP64) {
+        rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, nullptr, use ? op->r0 : nullptr);
+        if (!rc && use) {  // x = x0 + correction
+            hipLaunchKernelGGL(k_axpy1, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                               (const double *)op->x0i, op->x);
+            LAUNCH_CHECK(op);
+        }
+    } else {
+        return set_err(op, PLSQR_ERR_ARG, "unknown precision");
+    }
+    if (rc) return rc;
+    if (op->permuted) {
+        if ((rc = to_external(op, op->x, x_dev))) return rc;
+    } else if (x_dev != op->x) {
+        HIP_TRY(op, hipMemcpyAsync(x_dev, op->x, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    }
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_lsqr_solve_x0(plsqr_op *op, const double *b_host, const double *x0_host, const plsqr_solve_opts *opts,
+                        double *x_host, plsqr_solve_out *out, int *used_x0) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_host || !opts || !x_host) return set_err(op, PLSQR_ERR_ARG, "b, opts and x are required");
+    for (int i = 0; i < op->m_k; ++i)
+        if (!std::isfinite(b_host[i])) return set_err(op, PLSQR_ERR_NONFINITE, "b holds a NaN or Inf at " + std::to_string(i));
+    if (x0_host)
+        for (int i = 0; i < op->n; ++i)
+            if (!std::isfinite(x0_host[i]))
+                return set_err(op, PLSQR_ERR_NONFINITE, "x0 holds a NaN or Inf at " + std::to_string(i));
+    HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
+    if (x0_host) HIP_TRY(op, hipMemcpyAsync(op->hx, x0_host, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
+    int rc = plsqr_lsqr_solve_x0_device(op, op->bst, x0_host ? op->hx : nullptr, opts, op->hx, out, used_x0);
+    if (rc) return rc;
+    HIP_TRY(op, hipMemcpyAsync(x_host, op->hx, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
     return PLSQR_OK;
 }
 

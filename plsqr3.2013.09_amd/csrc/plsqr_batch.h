@@ -666,6 +666,13 @@ __global__ void kb_fill(long long len, T val, T *__restrict__ a) {
         a[i] = val;
 }
 
+// xt = x + dx
+__global__ void kb_add3(long long len, const double *__restrict__ x, const double *__restrict__ dx,
+                        double *__restrict__ xt) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
+        xt[i] = x[i] + dx[i];
+}
+
 // x += dx
 __global__ void kb_axpy(long long len, const double *__restrict__ dx, double *__restrict__ x) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)

@@ -69,7 +69,7 @@ struct plsqr_batch {
     long long *d_row0 = nullptr, *d_col0 = nullptr, *d_xb = nullptr, *d_yb = nullptr, *d_bb = nullptr;
     int *d_mk = nullptr, *d_mrow = nullptr, *d_ncol = nullptr;
     // vectors (interleaved K)
-    double *u64 = nullptr, *v64 = nullptr, *b0 = nullptr, *x = nullptr, *w = nullptr, *dx = nullptr;
+    double *u64 = nullptr, *v64 = nullptr, *b0 = nullptr, *x = nullptr, *w = nullptr, *dx = nullptr, *xt = nullptr;
     float *u32 = nullptr, *v32 = nullptr;
     double *part = nullptr, *sums = nullptr, *stage = nullptr;
     long long stage_len = 0;
@@ -267,12 +267,13 @@ int launch_state_sums(plsqr_batch *b, const int *rng, double *out) {
     return PLSQR_OK;
 }
 
-int launch_residual(plsqr_batch *b) {
+// u64 = [b;0] - A xv, per-chunk sums of squares in part
+int launch_residual(plsqr_batch *b, const double *xv) {
     const dim3 g(b->nrch), blk(256);
     switch (b->K) {
-        case 1: hipLaunchKernelGGL((kb_residual<1, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, b->x, b->b0, b->u64, b->part); break;
-        case 2: hipLaunchKernelGGL((kb_residual<2, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, b->x, b->b0, b->u64, b->part); break;
-        default: hipLaunchKernelGGL((kb_residual<4, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, b->x, b->b0, b->u64, b->part); break;
+        case 1: hipLaunchKernelGGL((kb_residual<1, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, xv, b->b0, b->u64, b->part); break;
+        case 2: hipLaunchKernelGGL((kb_residual<2, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, xv, b->b0, b->u64, b->part); break;
+        default: hipLaunchKernelGGL((kb_residual<4, 256>), g, blk, 0, b->stream, b->nrch, b->rch_row, b->rp, b->ci, b->rv, xv, b->b0, b->u64, b->part); break;
     }
     BT_LAUNCH(b);
     return PLSQR_OK;
@@ -499,69 +500,81 @@ int solve_fp64(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, p
     return PLSQR_OK;
 }
 
-// Outer FP64 residual refinement around inner FP32 LSQR runs of at most
-// refine_every iterations, per state; a state stops when the FP64 test2 of
-// its true residual, ||A^T r|| / (anorm ||r||) with anorm the largest inner
-// estimate, reaches tol.
+// per-state ||r|| and ||A^T r|| in FP64 for r = [b;0] - A xt (r into u64,
+// A^T r into v64); host arrays rn2[S], an2[S] receive the squares
+int true_norms(plsqr_batch *b, const double *xv, std::vector<double> &sums) {
+    int rc;
+    const int S = b->P * b->K;
+    if ((rc = launch_residual(b, xv)) || (rc = launch_state_sums(b, b->r_rch, b->sums)) || (rc = fill(b, b->v64, b->cols * b->K, 0.0)) ||
+        (rc = launch_cols<double>(b, b->shape[0], b->sval, b->u64, b->v64, b->sc_plain)) ||
+        (rc = launch_state_sums(b, b->r_tile, b->sums + S)))
+        return rc;
+    sums.resize((size_t)S * 2);
+    BT_TRY(b, hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, b->stream));
+    BT_TRY(b, hipStreamSynchronize(b->stream));
+    return PLSQR_OK;
+}
+
+// Mixed precision per (system, right-hand side), as the single-system solve:
+// a continuous inner FP32 LSQR on r = [b;0] - A x for every live pair, an
+// FP64 check of xt = x + dx every refine_every launches (a pair stops there
+// once its FP64 test2 = ||A^T r|| / (anorm ||r||) <= tol, anorm the largest
+// inner estimate), and a restart on the refined residuals (x = xt) when
+// every pair still live has stopped its inner run without meeting tol.
 int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out, double *total_ms) {
     int rc;
     if ((rc = ensure_fp32(b))) return rc;
+    if (!b->xt && (rc = balloc(b, &b->xt, (size_t)b->cols * b->K))) return rc;
     const int itn = std::max(0, o->itn);
-    const int period = o->refine_every;
+    const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int S = b->P * b->K;
     std::vector<Scal> h((size_t)S);
     std::vector<double> anorm_max((size_t)S, 0.0), rn((size_t)S, 0.0), an((size_t)S, 0.0), t2((size_t)S, INFINITY);
-    std::vector<double> sums((size_t)S * 2, 0.0);
-    std::vector<int> odone((size_t)S, 0), inner((size_t)S, 0), refs((size_t)S, 0), fin_ev((size_t)S, -1);
+    std::vector<double> rn0((size_t)S, 0.0), sums;
+    std::vector<int> odone((size_t)S, 0), inner((size_t)S, 0), prev((size_t)S, 0), checks((size_t)S, 0),
+        fin_ev((size_t)S, -1);
     for (int p = 0; p < b->P; ++p)
         for (int k = b->Kr; k < b->K; ++k) odone[(size_t)p * b->K + k] = 1;
     hipEvent_t e0;
     BT_TRY(b, hipEventCreate(&e0));
     BT_TRY(b, hipEventRecord(e0, b->stream));
-    if ((rc = launch_load_b<double>(b, b_dev, b->b0)) || (rc = fill(b, b->x, b->cols * b->K, 0.0))) return rc;
-    std::vector<hipEvent_t> outer_ev;
-    int total = 0, outer = 0;
+    if ((rc = launch_load_b<double>(b, b_dev, b->b0)) || (rc = fill(b, b->x, b->cols * b->K, 0.0)) ||
+        (rc = true_norms(b, b->x, sums)))
+        return rc;
+    std::vector<hipEvent_t> check_ev;
     size_t ev_next = 0;
+    auto record = [&](int *idx) -> int {
+        int r2;
+        if ((r2 = ensure_events(b, ev_next + 1))) return r2;
+        BT_TRY(b, hipEventRecord(b->evs[ev_next], b->stream));
+        check_ev.push_back(b->evs[ev_next]);
+        ++ev_next;
+        *idx = (int)check_ev.size() - 1;
+        return PLSQR_OK;
+    };
+    int ev_idx;
+    if ((rc = record(&ev_idx))) return rc;
+    for (int st = 0; st < S; ++st) {
+        rn0[(size_t)st] = rn[(size_t)st] = std::sqrt(sums[(size_t)st]);
+        if (odone[(size_t)st]) continue;
+        if (rn[(size_t)st] == 0.0 || itn == 0) {
+            odone[(size_t)st] = 1;
+            t2[(size_t)st] = rn[(size_t)st] == 0.0 ? 0.0 : INFINITY;
+            fin_ev[(size_t)st] = ev_idx;
+        }
+    }
+    int total = 0;
     hipGraphExec_t exec = nullptr;
+    const int one = 1;
     for (;;) {
-        // FP64 residual r = [b;0] - A x (in u64) and ||r|| per state
-        if ((rc = launch_residual(b)) || (rc = launch_state_sums(b, b->r_rch, b->sums))) return rc;
-        if (outer > 0) {  // ||A^T r|| per state, into v64 from zero
-            if ((rc = fill(b, b->v64, b->cols * b->K, 0.0)) ||
-                (rc = launch_cols<double>(b, b->shape[0], b->sval, b->u64, b->v64, b->sc_plain)) ||
-                (rc = launch_state_sums(b, b->r_tile, b->sums + S)))
-                return rc;
-        }
-        if ((rc = ensure_events(b, ev_next + 1))) return rc;
-        hipEvent_t ev = b->evs[ev_next++];
-        BT_TRY(b, hipEventRecord(ev, b->stream));
-        outer_ev.push_back(ev);
-        BT_TRY(b, hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, b->stream));
-        BT_TRY(b, hipStreamSynchronize(b->stream));
         int live = 0;
+        for (int st = 0; st < S; ++st) live += odone[(size_t)st] ? 0 : 1;
+        if (live == 0 || total >= itn) break;
+        // (re)start the inner runs of the live pairs on r (in u64)
         for (int st = 0; st < S; ++st) {
-            if (odone[(size_t)st]) continue;
-            rn[(size_t)st] = std::sqrt(sums[(size_t)st]);
-            if (outer > 0) {
-                an[(size_t)st] = std::sqrt(sums[(size_t)S + st]);
-                t2[(size_t)st] = (rn[(size_t)st] > 0.0 && anorm_max[(size_t)st] > 0.0)
-                                     ? an[(size_t)st] / (anorm_max[(size_t)st] * rn[(size_t)st])
-                                     : 0.0;
-            }
-            if (rn[(size_t)st] == 0.0 || itn == 0) t2[(size_t)st] = 0.0;
-            if (t2[(size_t)st] <= o->tol || (outer > 0 && total >= itn) || rn[(size_t)st] == 0.0 || itn == 0) {
-                odone[(size_t)st] = 1;
-                fin_ev[(size_t)st] = outer;
-            } else {
-                ++live;
-            }
-        }
-        if (live == 0) break;
-        // inner FP32 LSQR on r for the live states
-        const int k = std::min(period, itn - total);
-        for (int st = 0; st < S; ++st) {
-            h[(size_t)st] = fresh_state(o, k);
+            h[(size_t)st] = fresh_state(o, itn - total);
             if (odone[(size_t)st]) h[(size_t)st].done = 1;
+            prev[(size_t)st] = 0;
         }
         if ((rc = upload(b, b->sc, h.data(), h.size()))) return rc;
         hipLaunchKernelGGL((kb_convert<float>), dim3(grid_for(b->rows * b->K, 256, b->cu * 8)), dim3(256), 0, b->stream,
@@ -572,38 +585,55 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             (rc = enqueue_start<float>(b, b->shape[1], b->sval32, b->u32, b->v32)))
             return rc;
         if (!exec && (rc = get_graph<float>(b, 1, b->rv32, b->sval32, b->u32, b->v32, b->dx, &exec))) return rc;
-        int launched = 0;
-        if ((rc = run_launches(b, exec, k, ev_next, &launched))) return rc;
-        ev_next += (size_t)launched;
-        BT_TRY(b, hipMemcpyAsync(h.data(), b->sc, sizeof(Scal) * h.size(), hipMemcpyDeviceToHost, b->stream));
-        hipLaunchKernelGGL(kb_axpy, dim3(grid_for(b->cols * b->K, 256, b->cu * 8)), dim3(256), 0, b->stream,
-                           b->cols * b->K, (const double *)b->dx, b->x);
-        BT_LAUNCH(b);
-        BT_TRY(b, hipStreamSynchronize(b->stream));
-        total += launched;
-        ++outer;
-        for (int st = 0; st < S; ++st) {
-            if (odone[(size_t)st]) continue;
-            anorm_max[(size_t)st] = std::max(anorm_max[(size_t)st], h[(size_t)st].anorm);
-            inner[(size_t)st] += std::max(h[(size_t)st].itn, 0);
-            refs[(size_t)st] += 1;
-            if (h[(size_t)st].itn == 0) {  // could not start: r orthogonal to the range, converged
-                odone[(size_t)st] = 1;
-                t2[(size_t)st] = 0.0;
-                fin_ev[(size_t)st] = -2;  // finished with this round's launches
+        bool restart = false;
+        while (!restart) {
+            const int k = std::min(period, itn - total);
+            int launched = 0;
+            if ((rc = run_launches(b, exec, k, ev_next, &launched))) return rc;
+            ev_next += (size_t)launched;
+            total += launched;
+            BT_TRY(b, hipMemcpyAsync(h.data(), b->sc, sizeof(Scal) * h.size(), hipMemcpyDeviceToHost, b->stream));
+            hipLaunchKernelGGL(kb_add3, dim3(grid_for(b->cols * b->K, 256, b->cu * 8)), dim3(256), 0, b->stream,
+                               b->cols * b->K, (const double *)b->x, (const double *)b->dx, b->xt);
+            BT_LAUNCH(b);
+            if ((rc = true_norms(b, b->xt, sums)) || (rc = record(&ev_idx))) return rc;
+            bool all_inner_done = true;
+            live = 0;
+            for (int st = 0; st < S; ++st) {
+                if (odone[(size_t)st]) continue;
+                const Scal &q = h[(size_t)st];
+                inner[(size_t)st] += std::max(q.itn, 0) - prev[(size_t)st];
+                prev[(size_t)st] = std::max(q.itn, 0);
+                anorm_max[(size_t)st] = std::max(anorm_max[(size_t)st], q.anorm);
+                rn[(size_t)st] = std::sqrt(sums[(size_t)st]);
+                an[(size_t)st] = std::sqrt(sums[(size_t)S + st]);
+                checks[(size_t)st] += 1;
+                t2[(size_t)st] = (rn[(size_t)st] > 0.0 && anorm_max[(size_t)st] > 0.0)
+                                     ? an[(size_t)st] / (anorm_max[(size_t)st] * rn[(size_t)st])
+                                     : 0.0;
+                const bool stuck = q.done && q.itn == 0;  // the inner run could not start
+                if (t2[(size_t)st] <= o->tol || total >= itn || stuck) {
+                    odone[(size_t)st] = 1;
+                    fin_ev[(size_t)st] = ev_idx;
+                    // stop its inner run where it is: x + dx is its answer
+                    BT_TRY(b, hipMemcpyAsync(&b->sc[st].done, &one, sizeof(int), hipMemcpyHostToDevice, b->stream));
+                } else {
+                    ++live;
+                    if (!q.done) all_inner_done = false;
+                }
+            }
+            if (live == 0 || total >= itn) {
+                restart = true;
+            } else if (all_inner_done || launched == 0) {
+                restart = true;  // refine: x = x + dx, r (in u64) is its residual
             }
         }
-        if (launched == 0) {
-            for (int st = 0; st < S; ++st)
-                if (!odone[(size_t)st]) {
-                    odone[(size_t)st] = 1;
-                    fin_ev[(size_t)st] = outer;
-                }
-        }
+        // x = x + dx for every pair (finished pairs keep their final x + dx)
+        BT_TRY(b, hipMemcpyAsync(b->x, b->xt, sizeof(double) * b->cols * b->K, hipMemcpyDeviceToDevice, b->stream));
     }
     BT_TRY(b, hipStreamSynchronize(b->stream));
     float t_all = 0.f;
-    for (size_t i = 0; i < outer_ev.size(); ++i) t_all = std::max(t_all, elapsed(e0, outer_ev[i]));
+    for (size_t i = 0; i < check_ev.size(); ++i) t_all = std::max(t_all, elapsed(e0, check_ev[i]));
     for (int p = 0; p < b->P; ++p)
         for (int k = 0; k < b->Kr; ++k) {
             const size_t st = (size_t)p * b->K + k;
@@ -611,14 +641,15 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             memset(&r, 0, sizeof(r));
             r.itn = inner[st];
             r.inner_total = inner[st];
-            r.refinements = refs[st];
+            r.refinements = checks[st];
             r.anorm = anorm_max[st];
             r.rnorm = rn[st];
+            r.bnorm = rn0[st];
             r.arnorm = t2[st];
             r.arnorm_true = an[st];
             r.istop = t2[st] <= o->tol ? 2 : 5;
             const int fe = fin_ev[st];
-            r.elapsed_ms = fe >= 0 && (size_t)fe < outer_ev.size() ? elapsed(e0, outer_ev[(size_t)fe]) : t_all;
+            r.elapsed_ms = fe >= 0 && (size_t)fe < check_ev.size() ? elapsed(e0, check_ev[(size_t)fe]) : t_all;
         }
     if (total_ms) *total_ms = t_all;
     (void)hipEventDestroy(e0);

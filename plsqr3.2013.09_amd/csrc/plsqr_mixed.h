@@ -230,6 +230,11 @@ __global__ void k_fill32(long long len, float *__restrict__ a) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256) a[i] = 0.f;
 }
 
+// xt = x + dx
+__global__ void k_add3(int n, const double *__restrict__ x, const double *__restrict__ dx, double *__restrict__ xt) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) xt[i] = x[i] + dx[i];
+}
+
 // x += dx
 __global__ void k_axpy1(int n, const double *__restrict__ dx, double *__restrict__ x) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] = x[i] + dx[i];

@@ -492,6 +492,14 @@ int plsqr_gen_system(const plsqr_gen_params *p, plsqr_system *out) {
     return PLSQR_OK;
 }
 
+int plsqr_gen_field(const plsqr_gen_params *p, double *x_true) {
+    if (!p || !x_true) return fail(PLSQR_ERR_ARG, "null argument");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(PLSQR_ERR_ARG, "grid must be positive");
+    Grid g{{p->nx, p->ny, p->nz}};
+    make_blobs(g, p->seed, p->n_blobs, p->blob_width, x_true);
+    return PLSQR_OK;
+}
+
 int plsqr_gen_rhs(const plsqr_system *sys, const plsqr_gen_params *p, double *b, double *x_true) {
     if (!sys || !p || !b || !x_true) return fail(PLSQR_ERR_ARG, "null argument");
     if ((long long)p->nx * p->ny * p->nz != sys->n) return fail(PLSQR_ERR_ARG, "grid does not match the system's columns");
