@@ -191,20 +191,24 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
             for (int c = 0; c < s->n; c += TW) tiles.push_back(c);
             tiles.push_back(s->n);
         }
+        RaySorted rays;
+        sort_rays(s, perm, tiles, {}, &rays);
+        L.rperm = rays.rperm;
+        const plsqr_system *ss = &rays.sys;
         std::vector<int> slices{0, s->m_k};
-        if (!build_tile_layout(s, perm, tiles, slices, TW, &L.tiles, err)) {
+        if (!build_tile_layout(ss, perm, tiles, slices, TW, &L.tiles, err)) {
             *err = "system " + std::to_string(p) + ": " + *err;
             return PLSQR_ERR_FORMAT;
         }
         if (std::getenv("PLSQR_TILE_CHECK") && std::atoi(std::getenv("PLSQR_TILE_CHECK")) != 0 &&
-            !tile_layout_self_check(s, L.tiles, err)) {
+            !tile_layout_self_check(ss, L.tiles, err)) {
             *err = "system " + std::to_string(p) + ": " + *err;
             return PLSQR_ERR_STATE;
         }
         L.inv = L.tiles.inv;
         L.tiles.perm.clear();
         L.tiles.perm.shrink_to_fit();
-        build_csr(s, perm, L.inv, &L);
+        build_csr(ss, perm, L.inv, &L);
     }
     // offsets and pieces
     plan->row0.assign((size_t)P + 1, 0);

@@ -30,6 +30,7 @@ namespace plsqr {
 struct ProblemLayout {
     int m_k = 0, m_d = 0, n = 0;
     std::vector<int> inv;         // inv[new column] = old column
+    std::vector<int> rperm;       // rperm[new kernel row] = old kernel row (rays sorted by tiles)
     std::vector<int> rp, ci;      // CSR of [A_k ; D], new columns ascending in kernel rows
     std::vector<double> rv;
     std::vector<int> dcp, dri;    // damping CSC in new column order, damping-local rows

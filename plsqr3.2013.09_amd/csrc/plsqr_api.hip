@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -71,6 +72,12 @@ struct plsqr_op {
     double *dx = nullptr;
     // warm starts: x0 in the device column order and r0 = [b;0] - A x0
     double *x0i = nullptr, *r0 = nullptr;
+    // ray order: kernel rows renumbered so that rays crossing the same tiles
+    // are adjacent; every row-space vector on the device is in the new order
+    bool rowperm = false;
+    int *rperm = nullptr, *rinv = nullptr;  // new -> old, old -> new (kernel rows)
+    double *hy2 = nullptr;                  // m, new-order staging
+    double *bint = nullptr;                 // m_k, b in the new order
     int tiled_persistent = 0;  // > 0: workgroups of the persistent tiled kernel
     std::vector<void *> allocs;
     size_t bytes = 0;
@@ -238,15 +245,70 @@ int to_external(plsqr_op *op, const double *in, double *ext) {
     return PLSQR_OK;
 }
 
-// aprod on a caller-order x (device): renumber in, compute, renumber back
-int aprod_ext_dev(plsqr_op *op, int mode, double *x, double *y) {
-    int rc;
-    if (!op->permuted) return mode == 1 ? aprod1_dev(op, x, y) : aprod2_dev(op, x, y);
-    if ((rc = to_internal(op, x, op->hx2))) return rc;
-    if ((rc = mode == 1 ? aprod1_dev(op, op->hx2, y) : aprod2_dev(op, op->hx2, y))) return rc;
-    if (mode == 2) return to_external(op, op->hx2, x);
+// row-space vectors (m = kernel rows then damping rows) between the caller's
+// order and the device order; only the kernel rows move
+int rows_to_internal(plsqr_op *op, const double *ext, double *in) {
+    if (!op->rowperm) {
+        if (in != ext) HIP_TRY(op, hipMemcpyAsync(in, ext, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
+        return PLSQR_OK;
+    }
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m_k, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                       op->rperm, ext, in);
+    LAUNCH_CHECK(op);
+    if (op->m_d > 0)
+        HIP_TRY(op, hipMemcpyAsync(in + op->m_k, ext + op->m_k, sizeof(double) * op->m_d, hipMemcpyDeviceToDevice,
+                                   op->stream));
     return PLSQR_OK;
 }
+
+int rows_to_external(plsqr_op *op, const double *in, double *ext) {
+    if (!op->rowperm) {
+        if (in != ext) HIP_TRY(op, hipMemcpyAsync(ext, in, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
+        return PLSQR_OK;
+    }
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m_k, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                       op->rinv, in, ext);
+    LAUNCH_CHECK(op);
+    if (op->m_d > 0)
+        HIP_TRY(op, hipMemcpyAsync(ext + op->m_k, in + op->m_k, sizeof(double) * op->m_d, hipMemcpyDeviceToDevice,
+                                   op->stream));
+    return PLSQR_OK;
+}
+
+// b (m_k, caller order, device) in the device row order: op->bint or b itself
+int b_internal(plsqr_op *op, const double *b, const double **out) {
+    if (!op->rowperm) {
+        *out = b;
+        return PLSQR_OK;
+    }
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m_k, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                       op->rperm, b, op->bint);
+    LAUNCH_CHECK(op);
+    *out = op->bint;
+    return PLSQR_OK;
+}
+
+// aprod on operands each either in the caller's order (renumbered in and,
+// for the output, back) or already in the device order
+int aprod_any(plsqr_op *op, int mode, double *x, bool x_ext, double *y, bool y_ext) {
+    int rc;
+    double *xi = x, *yi = y;
+    if (x_ext && op->permuted) {
+        if ((rc = to_internal(op, x, op->hx2))) return rc;
+        xi = op->hx2;
+    }
+    if (y_ext && op->rowperm) {
+        if ((rc = rows_to_internal(op, y, op->hy2))) return rc;
+        yi = op->hy2;
+    }
+    if ((rc = mode == 1 ? aprod1_dev(op, xi, yi) : aprod2_dev(op, xi, yi))) return rc;
+    if (mode == 2 && xi != x) return to_external(op, xi, x);
+    if (mode == 1 && yi != y) return rows_to_external(op, yi, y);
+    return PLSQR_OK;
+}
+
+// aprod on caller-order x and y (device)
+int aprod_ext_dev(plsqr_op *op, int mode, double *x, double *y) { return aprod_any(op, mode, x, true, y, true); }
 
 // -------------------------------------------------------------------------
 // host-side validation of the loaded arrays (the reference loaders' checks,
@@ -609,6 +671,8 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     std::vector<int> hb = slice_bounds_host(s, nsl);
     // tiled mode-2 layout on bricks of voxels (default operator only)
     std::unique_ptr<plsqr::TileLayout> L;
+    std::unique_ptr<plsqr::RaySorted> rays;
+    const plsqr_system *sys = s;  // s, or its ray-sorted renumbering
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
         std::vector<int> perm, tiles;
         if (grid) {
@@ -619,11 +683,16 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
             for (int c = 0; c < s->n; c += TILE_W) tiles.push_back(c);
             tiles.push_back(s->n);
         }
+        if (env_int("PLSQR_RAYSORT", 1) != 0) {
+            rays.reset(new plsqr::RaySorted());
+            plsqr::sort_rays(s, perm, tiles, hb, rays.get());  // within the row slices: hb stays valid
+            sys = &rays->sys;
+        }
         L.reset(new plsqr::TileLayout());
         std::string why;
-        if (!plsqr::build_tile_layout(s, perm, tiles, hb, TILE_W, L.get(), &why)) {
+        if (!plsqr::build_tile_layout(sys, perm, tiles, hb, TILE_W, L.get(), &why)) {
             L.reset();  // falls back to the sliced CSC
-        } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(s, *L, &why)) {
+        } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(sys, *L, &why)) {
             return fail(set_err(op, PLSQR_ERR_STATE, why));
         }
     }
@@ -637,8 +706,18 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     op->own_stream = true;
-    if ((rc = build(op, s, hb, L.get()))) return fail(rc);
+    if ((rc = build(op, sys, hb, L.get()))) return fail(rc);
     L.reset();
+    if (rays) {
+        op->rowperm = true;
+        if ((rc = dalloc(op, &op->rperm, op->m_k)) || (rc = dalloc(op, &op->rinv, op->m_k)) ||
+            (rc = dalloc(op, &op->hy2, op->m)) || (rc = dalloc(op, &op->bint, op->m_k)))
+            return fail(rc);
+        e = hipMemcpy(op->rperm, rays->rperm.data(), sizeof(int) * op->m_k, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(op->rinv, rays->rinv.data(), sizeof(int) * op->m_k, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
+        rays.reset();
+    }
     // vectors and scratch
     op->nt_loads = env_int("PLSQR_NT", 1) != 0;
     op->tiled_persistent = env_int("PLSQR_TILED_WG", 0);
@@ -701,7 +780,14 @@ void *plsqr_op_get_stream(const plsqr_op *op) { return op ? (void *)op->stream :
 int plsqr_op_set_vectors(plsqr_op *op, const double *u_host, const double *v_host) {
     if (!op) return PLSQR_ERR_ARG;
     if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
-    if (u_host) HIP_TRY(op, hipMemcpyAsync(op->u, u_host, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
+    if (u_host) {
+        HIP_TRY(op, hipMemcpyAsync(op->rowperm ? op->hy : op->u, u_host, sizeof(double) * op->m, hipMemcpyHostToDevice,
+                                   op->stream));
+        if (op->rowperm) {
+            int rc = rows_to_internal(op, op->hy, op->u);
+            if (rc) return rc;
+        }
+    }
     if (v_host) {
         HIP_TRY(op, hipMemcpyAsync(op->permuted ? op->hx : op->v, v_host, sizeof(double) * op->n,
                                    hipMemcpyHostToDevice, op->stream));
@@ -717,7 +803,14 @@ int plsqr_op_set_vectors(plsqr_op *op, const double *u_host, const double *v_hos
 int plsqr_op_get_vectors(plsqr_op *op, double *u_host, double *v_host) {
     if (!op) return PLSQR_ERR_ARG;
     if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
-    if (u_host) HIP_TRY(op, hipMemcpyAsync(u_host, op->u, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
+    if (u_host) {
+        if (op->rowperm) {
+            int rc = rows_to_external(op, op->u, op->hy);
+            if (rc) return rc;
+        }
+        HIP_TRY(op, hipMemcpyAsync(u_host, op->rowperm ? op->hy : op->u, sizeof(double) * op->m, hipMemcpyDeviceToHost,
+                                   op->stream));
+    }
     if (v_host) {
         if (op->permuted) {
             int rc = to_external(op, op->v, op->hx);
@@ -773,14 +866,12 @@ static int aprod_host(plsqr_op *op, int mode, int m, int n, double *x, double *y
         HIP_TRY(op, hipMemcpyAsync(op->hy, y, sizeof(double) * op->m, hipMemcpyHostToDevice, op->stream));
         yd = op->hy;
     }
-    int rc;
     if (x) {
         HIP_TRY(op, hipMemcpyAsync(op->hx, x, sizeof(double) * op->n, hipMemcpyHostToDevice, op->stream));
         xd = op->hx;
-        rc = aprod_ext_dev(op, mode, xd, yd);  // caller order
-    } else {
-        rc = mode == 1 ? aprod1_dev(op, xd, yd) : aprod2_dev(op, xd, yd);  // resident v, device order
     }
+    // caller arrays are in the caller's order, the resident u and v in the device order
+    const int rc = aprod_any(op, mode, xd, x != nullptr, yd, y != nullptr);
     if (rc) return rc;
     if (mode == 1 && y) HIP_TRY(op, hipMemcpyAsync(y, yd, sizeof(double) * op->m, hipMemcpyDeviceToHost, op->stream));
     if (mode == 2 && x) HIP_TRY(op, hipMemcpyAsync(x, xd, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
@@ -1241,6 +1332,7 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
     plsqr_solve_opts o;
     default_opts(opts, &o);
     int rc;
+    if ((rc = b_internal(op, b_dev, &b_dev))) return rc;
     if (o.precision == PLSQR_PREC_MIXED) {
         rc = solve_mixed(op, b_dev, &o, out);
     } else if (o.precision == PLSQR_PREC_FP64) {
@@ -1297,6 +1389,7 @@ int plsqr_lsqr_solve_x0_device(plsqr_op *op, const double *b_dev, const double *
     plsqr_solve_opts o;
     default_opts(opts, &o);
     int rc;
+    if ((rc = b_internal(op, b_dev, &b_dev))) return rc;
     if (!op->x0i && ((rc = dalloc(op, &op->x0i, op->n)) || (rc = dalloc(op, &op->r0, op->m)))) return rc;
     int use = 0;
     if (x0_dev) {
@@ -1378,9 +1471,14 @@ int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_hos
         if (rc0) return rc0;
         xin = op->hx2;
     }
+    const double *bi = op->bst;
+    {
+        int rcb = b_internal(op, op->bst, &bi);
+        if (rcb) return rcb;
+    }
     const int g = grid_for(op->m, 256, op->cu * 4);
     hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, xin,
-                       op->bst, op->hy);
+                       bi, op->hy);
     LAUNCH_CHECK(op);
     std::vector<double> hp((size_t)op->part_len);
     auto host_sum = [&](int count) {

@@ -544,7 +544,7 @@ __global__ void kb_count_live(int n_states, const Scal *__restrict__ sc, int *ou
 // lanes Kr..K-1 are zero.
 // ---------------------------------------------------------------------------
 struct ExtMap {
-    const int *row_prob, *col_prob, *col_old;
+    const int *row_prob, *row_old, *col_prob, *col_old;  // row_old: the caller's row of a device row
     const long long *row0, *col0, *xb, *yb, *bb;
     const int *mk, *mrow, *ncol;
 };
@@ -554,7 +554,7 @@ template <class T, int K>
 __global__ void kb_load_b(long long rows, int Kr, ExtMap m, const double *__restrict__ b, T *__restrict__ u) {
     for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long long)gridDim.x * blockDim.x) {
         const int p = m.row_prob[r];
-        const long long i = r - m.row0[p];
+        const long long i = m.row_old[r];
         VK<T, K> q;
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -568,7 +568,7 @@ template <class T, int K>
 __global__ void kb_conv_rows(long long rows, int Kr, ExtMap m, int dir, double *__restrict__ ext, T *__restrict__ y) {
     for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long long)gridDim.x * blockDim.x) {
         const int p = m.row_prob[r];
-        const long long i = r - m.row0[p];
+        const long long i = m.row_old[r];
         const long long base = m.yb[p] + i;
         if (dir == 0) {
             VK<T, K> q;

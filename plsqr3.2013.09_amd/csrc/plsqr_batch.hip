@@ -65,7 +65,7 @@ struct plsqr_batch {
     uint8_t *rch_kind = nullptr;
     int *r_rch = nullptr, *r_tile = nullptr, *r_cch = nullptr;
     // caller layout maps
-    int *row_prob = nullptr, *col_prob = nullptr, *col_old = nullptr;
+    int *row_prob = nullptr, *row_old = nullptr, *col_prob = nullptr, *col_old = nullptr;
     long long *d_row0 = nullptr, *d_col0 = nullptr, *d_xb = nullptr, *d_yb = nullptr, *d_bb = nullptr;
     int *d_mk = nullptr, *d_mrow = nullptr, *d_ncol = nullptr;
     // vectors (interleaved K)
@@ -139,6 +139,7 @@ int env_int(const char *name, int dflt) {
 ExtMap ext_map(const plsqr_batch *b) {
     ExtMap m;
     m.row_prob = b->row_prob;
+    m.row_old = b->row_old;
     m.col_prob = b->col_prob;
     m.col_old = b->col_old;
     m.row0 = b->d_row0;
@@ -799,7 +800,8 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->rch_kind, (size_t)b->nrch)) || (rc = balloc(b, &b->cch_col, (size_t)b->ncch + 1)) ||
         (rc = balloc(b, &b->cch_prob, (size_t)b->ncch)) || (rc = balloc(b, &b->r_rch, (size_t)P + 1)) ||
         (rc = balloc(b, &b->r_tile, (size_t)P + 1)) || (rc = balloc(b, &b->r_cch, (size_t)P + 1)) ||
-        (rc = balloc(b, &b->row_prob, (size_t)b->rows)) || (rc = balloc(b, &b->col_prob, (size_t)b->cols)) ||
+        (rc = balloc(b, &b->row_prob, (size_t)b->rows)) || (rc = balloc(b, &b->row_old, (size_t)b->rows)) ||
+        (rc = balloc(b, &b->col_prob, (size_t)b->cols)) ||
         (rc = balloc(b, &b->col_old, (size_t)b->cols)) || (rc = balloc(b, &b->d_row0, (size_t)P + 1)) ||
         (rc = balloc(b, &b->d_col0, (size_t)P + 1)) || (rc = balloc(b, &b->d_xb, (size_t)P + 1)) ||
         (rc = balloc(b, &b->d_yb, (size_t)P + 1)) || (rc = balloc(b, &b->d_bb, (size_t)P + 1)) ||
@@ -848,8 +850,10 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
             (rc = upload(b, b->sval + plan.slot0[(size_t)p], T.sval.data(), T.sval.size())) ||
             (rc = upload(b, b->sseg + plan.slot0[(size_t)p], T.sseg.data(), T.sseg.size())))
             return rc;
-        std::vector<int> rpb((size_t)m, p), cpb((size_t)L.n, p);
+        std::vector<int> rpb((size_t)m, p), cpb((size_t)L.n, p), rold((size_t)m);
+        for (int i = 0; i < m; ++i) rold[(size_t)i] = i < L.m_k ? L.rperm[(size_t)i] : i;
         if ((rc = upload(b, b->row_prob + r0, rpb.data(), rpb.size())) ||
+            (rc = upload(b, b->row_old + r0, rold.data(), rold.size())) ||
             (rc = upload(b, b->col_prob + c0, cpb.data(), cpb.size())) ||
             (rc = upload(b, b->col_old + c0, L.inv.data(), L.inv.size())))
             return rc;

@@ -2,6 +2,7 @@
 #include "tile_build.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <atomic>
 #include <cstdlib>
@@ -218,5 +219,54 @@ bool tile_layout_self_check(const plsqr_system *s, const TileLayout &L, std::str
     }
     return true;
 }
+
+void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
+               const std::vector<int> &slice_bounds, RaySorted *o) {
+    const int n = s->n, mk = s->m_k;
+    std::vector<int> tile_of((size_t)n, 0);
+    for (size_t t = 0; t + 1 < tile_start.size(); ++t)
+        for (int c = tile_start[t]; c < tile_start[t + 1]; ++c) tile_of[(size_t)c] = (int)t;
+    std::vector<int> lo((size_t)mk, INT_MAX), hi((size_t)mk, -1);
+    for (int c = 0; c < n; ++c) {
+        const int t = tile_of[(size_t)perm[(size_t)c]];
+        for (int e = s->k_col_ptr[c]; e < s->k_col_ptr[c + 1]; ++e) {
+            const int r = s->k_row_idx[e];
+            lo[(size_t)r] = std::min(lo[(size_t)r], t);
+            hi[(size_t)r] = std::max(hi[(size_t)r], t);
+        }
+    }
+    o->rperm.resize((size_t)mk);
+    for (int i = 0; i < mk; ++i) o->rperm[(size_t)i] = i;
+    std::vector<int> sb = slice_bounds;
+    if (sb.size() < 2) sb = {0, mk};
+    for (size_t q = 0; q + 1 < sb.size(); ++q)
+        std::stable_sort(o->rperm.begin() + sb[q], o->rperm.begin() + sb[q + 1], [&](int a, int b) {
+            return lo[(size_t)a] != lo[(size_t)b] ? lo[(size_t)a] < lo[(size_t)b] : hi[(size_t)a] < hi[(size_t)b];
+        });
+    o->rinv.resize((size_t)mk);
+    for (int i = 0; i < mk; ++i) o->rinv[(size_t)o->rperm[(size_t)i]] = i;
+    o->ri.resize((size_t)s->nnz_k);
+    o->kv.resize((size_t)s->nnz_k);
+    par_for(n, [&](long long cb, long long ce) {
+        std::vector<std::pair<int, double>> tmp;
+        for (long long c = cb; c < ce; ++c) {
+            const int b = s->k_col_ptr[c], e = s->k_col_ptr[c + 1];
+            tmp.clear();
+            for (int k = b; k < e; ++k) tmp.emplace_back(o->rinv[(size_t)s->k_row_idx[k]], s->k_val[k]);
+            std::sort(tmp.begin(), tmp.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b2) {
+                return a.first < b2.first;
+            });
+            for (int k = b; k < e; ++k) {
+                o->ri[(size_t)k] = tmp[(size_t)(k - b)].first;
+                o->kv[(size_t)k] = tmp[(size_t)(k - b)].second;
+            }
+        }
+    });
+    o->sys = *s;
+    o->sys.owned = 0;
+    o->sys.k_row_idx = o->ri.data();
+    o->sys.k_val = o->kv.data();
+}
+
 
 }  // namespace plsqr

@@ -67,6 +67,23 @@ struct TileLayout {
     int max_seg_per_item = 0;
 };
 
+// Kernel rows renumbered by the first and last tile their columns fall in
+// (a stable sort, so equal keys keep the loaded order): rays with the same
+// entry and exit tiles become neighbours, and the gathers of one wavefront
+// share cache lines.  With slice bounds (row ranges of the loaded order) the
+// sort stays inside each slice, so every slice keeps its rows and its mix of
+// ray directions.  sys gets the renumbered kernel CSC (rows ascending in
+// every column); the damping arrays and b are the caller's.  rperm[new] =
+// old kernel row, rinv[old] = new.
+struct RaySorted {
+    plsqr_system sys;
+    std::vector<int> ri, rperm, rinv;
+    std::vector<double> kv;
+};
+
+void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
+               const std::vector<int> &slice_bounds, RaySorted *out);
+
 // Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
 // voxels, voxels inside a brick x fastest.  Bricks are numbered z fastest
 // (vertical stacks of bricks, then x, then y) when z_columns is set, so that
