@@ -324,12 +324,21 @@ def main():
     hip.h2d(y_dev, np.random.default_rng(1).standard_normal(s.m_k + s.m_d) * 1e-3)
 
     if a.aprod_only:
-        # a plain device-to-device copy of 1 GiB calibrates the byte counters
-        # (FETCH_SIZE and WRITE_SIZE should read 1 GiB each for it)
+        # a plain device-to-device copy of 1 GiB and a streaming sum over
+        # 1 GiB calibrate the byte counters (each reads 1 GiB; the copy
+        # writes 1 GiB)
         cp_src, cp_dst = hip.malloc(1 << 30), hip.malloc(1 << 30)
         hip.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         hip.check(hip.lib.hipMemcpy(C.c_void_p(cp_dst), C.c_void_p(cp_src), 1 << 30, 3), "hipMemcpy")
         hip.sync()
+        try:
+            import torch
+            t = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+            float(t.sum())
+            del t
+            torch.cuda.synchronize()
+        except Exception as e:  # calibration only
+            print(f"[bench] streaming-sum calibration skipped: {e}", file=sys.stderr)
         ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
         ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
         b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)

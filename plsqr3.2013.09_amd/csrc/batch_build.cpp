@@ -191,6 +191,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
             for (int c = 0; c < s->n; c += TW) tiles.push_back(c);
             tiles.push_back(s->n);
         }
+        order_tile_columns(s, tiles, perm);
         RaySorted rays;
         sort_rays(s, perm, tiles, {}, &rays);
         L.rperm = rays.rperm;

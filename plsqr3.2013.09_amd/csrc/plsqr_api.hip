@@ -683,6 +683,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
             for (int c = 0; c < s->n; c += TILE_W) tiles.push_back(c);
             tiles.push_back(s->n);
         }
+        if (env_int("PLSQR_VOXSORT", 1) != 0) plsqr::order_tile_columns(s, tiles, perm);
         if (env_int("PLSQR_RAYSORT", 1) != 0) {
             rays.reset(new plsqr::RaySorted());
             plsqr::sort_rays(s, perm, tiles, hb, rays.get());  // within the row slices: hb stays valid

@@ -220,6 +220,21 @@ bool tile_layout_self_check(const plsqr_system *s, const TileLayout &L, std::str
     return true;
 }
 
+void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_start, std::vector<int> &perm) {
+    const int n = s->n;
+    std::vector<int> inv((size_t)n);
+    for (int o = 0; o < n; ++o) inv[(size_t)perm[(size_t)o]] = o;
+    std::vector<int> cols;
+    for (size_t t = 0; t + 1 < tile_start.size(); ++t) {
+        const int c0 = tile_start[t], c1 = tile_start[t + 1];
+        cols.assign(inv.begin() + c0, inv.begin() + c1);
+        std::stable_sort(cols.begin(), cols.end(), [&](int a, int b) {
+            return s->k_col_ptr[a + 1] - s->k_col_ptr[a] > s->k_col_ptr[b + 1] - s->k_col_ptr[b];
+        });
+        for (int i = 0; i < c1 - c0; ++i) perm[(size_t)cols[(size_t)i]] = c0 + i;
+    }
+}
+
 void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
                const std::vector<int> &slice_bounds, RaySorted *o) {
     const int n = s->n, mk = s->m_k;

@@ -67,6 +67,13 @@ struct TileLayout {
     int max_seg_per_item = 0;
 };
 
+// Reorders the columns inside every tile by descending kernel nonzero count
+// (stable), updating perm (old -> new) in place.  The tiled mode-2 kernel
+// reads a voxel-interleaved row of slots per step; with long columns first
+// the lanes still active in a late step are contiguous, so the cache lines
+// it fetches are full instead of mostly padding.
+void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_start, std::vector<int> &perm);
+
 // Kernel rows renumbered by the first and last tile their columns fall in
 // (a stable sort, so equal keys keep the loaded order): rays with the same
 // entry and exit tiles become neighbours, and the gathers of one wavefront
