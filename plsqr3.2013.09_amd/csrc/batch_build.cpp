@@ -157,7 +157,7 @@ int validate_system(const plsqr_system *s, std::string *err) {
 }
 
 int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids, int P, int TW, int row_chunk,
-                     int damp_chunk, int col_chunk, BatchPlan *plan, std::string *err) {
+                     int damp_chunk, int col_chunk, int slice_rows, int xcds, BatchPlan *plan, std::string *err) {
     if (P <= 0 || !systems) {
         *err = "no systems";
         return PLSQR_ERR_ARG;
@@ -192,11 +192,25 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
             tiles.push_back(s->n);
         }
         order_tile_columns(s, tiles, perm);
+        // row slices balanced by nonzeros, about slice_rows rows each
+        const int S = std::max(1, std::min(64, (s->m_k + std::max(slice_rows, 1) - 1) / std::max(slice_rows, 1)));
+        std::vector<int> slices((size_t)S + 1, 0);
+        {
+            std::vector<long long> cnt((size_t)s->m_k + 1, 0);
+            for (long long e = 0; e < s->nnz_k; ++e) cnt[(size_t)s->k_row_idx[e] + 1]++;
+            for (int r = 0; r < s->m_k; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
+            for (int q = 1; q < S; ++q) {
+                const long long target = s->nnz_k * q / S;
+                int b = (int)(std::lower_bound(cnt.begin(), cnt.end(), target) - cnt.begin());
+                slices[(size_t)q] = std::min(std::max(b, slices[(size_t)q - 1]), s->m_k);
+            }
+            slices[(size_t)S] = s->m_k;
+        }
+        L.S = S;
         RaySorted rays;
-        sort_rays(s, perm, tiles, {}, &rays);
+        sort_rays(s, perm, tiles, slices, &rays);
         L.rperm = rays.rperm;
         const plsqr_system *ss = &rays.sys;
-        std::vector<int> slices{0, s->m_k};
         if (!build_tile_layout(ss, perm, tiles, slices, TW, &L.tiles, err)) {
             *err = "system " + std::to_string(p) + ": " + *err;
             return PLSQR_ERR_FORMAT;
@@ -212,14 +226,11 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         build_csr(ss, perm, L.inv, &L);
     }
     // offsets and pieces
-    plan->row0.assign((size_t)P + 1, 0);
-    plan->col0.assign((size_t)P + 1, 0);
-    plan->nnz0.assign((size_t)P + 1, 0);
-    plan->dnnz0.assign((size_t)P + 1, 0);
-    plan->slot0.assign((size_t)P + 1, 0);
-    plan->seg0.assign((size_t)P + 1, 0);
-    plan->cut0.assign((size_t)P + 1, 0);
-    plan->tile0.assign((size_t)P + 1, 0);
+    for (auto *v : {&plan->row0, &plan->col0, &plan->nnz0, &plan->dnnz0, &plan->slot0, &plan->seg0, &plan->cut0,
+                    &plan->pofs})
+        v->assign((size_t)P + 1, 0);
+    plan->item0.assign((size_t)P + 1, 0);
+    plan->nslice.assign((size_t)P, 1);
     plan->rch0.assign((size_t)P + 1, 0);
     plan->cch0.assign((size_t)P + 1, 0);
     plan->rch_row.clear();
@@ -238,7 +249,9 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         plan->slot0[(size_t)p + 1] = plan->slot0[(size_t)p] + L.tiles.n_slot;
         plan->seg0[(size_t)p + 1] = plan->seg0[(size_t)p] + L.tiles.n_seg;
         plan->cut0[(size_t)p + 1] = plan->cut0[(size_t)p] + L.tiles.cut_base.back();
-        plan->tile0[(size_t)p + 1] = plan->tile0[(size_t)p] + L.tiles.T;
+        plan->item0[(size_t)p + 1] = plan->item0[(size_t)p] + L.tiles.T * L.S;
+        plan->pofs[(size_t)p + 1] = plan->pofs[(size_t)p] + (long long)L.S * L.n;
+        plan->nslice[(size_t)p] = L.S;
         mx = std::max(mx, L.tiles.max_seg_per_item);
         for (int r = 0; r < L.m_k; r += row_chunk) {
             plan->rch_row.push_back((int)(r0 + r));
@@ -257,6 +270,42 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         }
         plan->cch0[(size_t)p + 1] = (int)plan->cch_prob.size();
     }
+    // slices to XCD queues, largest first to the least loaded queue; inside a
+    // queue the items of one slice are consecutive (tile order)
+    {
+        struct Sl {
+            int p, s;
+            long long work;
+        };
+        std::vector<Sl> sl;
+        for (int p = 0; p < P; ++p) {
+            const ProblemLayout &L = plan->prob[(size_t)p];
+            for (int q = 0; q < L.S; ++q) {
+                long long w = 0;
+                for (int t = 0; t < L.tiles.T; ++t) {
+                    const long long it = (long long)t * L.S + q;
+                    w += L.tiles.slot_base[(size_t)it + 1] - L.tiles.slot_base[(size_t)it] + 4 * TW;
+                }
+                sl.push_back({p, q, w});
+            }
+        }
+        std::stable_sort(sl.begin(), sl.end(), [](const Sl &a, const Sl &b) { return a.work > b.work; });
+        const int X = std::max(1, xcds);
+        std::vector<long long> load((size_t)X, 0);
+        std::vector<std::vector<int>> queue((size_t)X);
+        for (const Sl &q : sl) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[(size_t)x] += q.work;
+            const ProblemLayout &L = plan->prob[(size_t)q.p];
+            for (int t = 0; t < L.tiles.T; ++t)
+                queue[(size_t)x].push_back((int)(plan->item0[(size_t)q.p] + (long long)t * L.S + q.s));
+        }
+        size_t len = 0;
+        for (const auto &v : queue) len = std::max(len, v.size());
+        plan->blk_item.assign(len * X, -1);
+        for (int x = 0; x < X; ++x)
+            for (size_t i = 0; i < queue[(size_t)x].size(); ++i) plan->blk_item[i * X + x] = queue[(size_t)x][i];
+    }
     plan->rows = plan->row0[(size_t)P];
     plan->cols = plan->col0[(size_t)P];
     plan->nnz = plan->nnz0[(size_t)P];
@@ -264,11 +313,13 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     plan->slots = plan->slot0[(size_t)P];
     plan->segs = plan->seg0[(size_t)P];
     plan->cuts = plan->cut0[(size_t)P];
-    plan->tiles = plan->tile0[(size_t)P];
-    plan->max_seg_per_tile = mx;
+    plan->items = plan->item0[(size_t)P];
+    plan->partials = plan->pofs[(size_t)P];
+    plan->max_seg_per_item = mx;
     plan->rch_row.push_back((int)plan->rows);
     plan->cch_col.push_back((int)plan->cols);
-    if (plan->rows * 4 > INT_MAX || plan->cols * 4 > INT_MAX || plan->nnz > INT_MAX || plan->dnnz > INT_MAX) {
+    if (plan->rows * 4 > INT_MAX || plan->cols * 4 > INT_MAX || plan->nnz > INT_MAX || plan->dnnz > INT_MAX ||
+        plan->items >= INT_MAX / TW) {
         *err = "the batch exceeds 32-bit row, column or nonzero indexing";
         return PLSQR_ERR_ARG;
     }

@@ -35,7 +35,8 @@ struct ProblemLayout {
     std::vector<double> rv;
     std::vector<int> dcp, dri;    // damping CSC in new column order, damping-local rows
     std::vector<double> dcv;
-    TileLayout tiles;             // one row slice
+    TileLayout tiles;             // S row slices, items t*S + s
+    int S = 1;
 };
 
 struct BatchPlan {
@@ -43,14 +44,21 @@ struct BatchPlan {
     std::vector<ProblemLayout> prob;
     // offsets (P + 1 entries each)
     std::vector<long long> row0, col0, nnz0, dnnz0, slot0, seg0, cut0;
-    std::vector<int> tile0, rch0, cch0;
+    std::vector<long long> pofs;         // start of each problem's slice partials (in columns)
+    std::vector<int> item0, rch0, cch0;  // items = (tile, slice) pairs
+    std::vector<int> nslice;             // row slices per problem
+    // mode-2 launch order: workgroup b runs item blk_item[b] (-1: none).
+    // Workgroup b lands on XCD b % 8; every row slice is given to one XCD
+    // and its items are queued there together, so the slice's rows of u
+    // stay in that XCD's L2 while its items run.
+    std::vector<int> blk_item;
     // pieces, global numbering
     std::vector<int> rch_row, rch_prob;  // row chunks: rch_row has count + 1 entries
     std::vector<uint8_t> rch_kind;       // 0 kernel rows, 1 damping rows
     std::vector<int> cch_col, cch_prob;  // column chunks: cch_col has count + 1 entries
-    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0;
-    int tiles = 0;
-    int max_seg_per_tile = 0;
+    long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0, partials = 0;
+    int items = 0;
+    int max_seg_per_item = 0;
 };
 
 // Checks a system's arrays on the host: pointer arrays monotone and ending at
@@ -59,7 +67,10 @@ int validate_system(const plsqr_system *s, std::string *err);
 
 // Builds the plan.  grids[p] may describe problem p's voxel grid (brick tiles
 // of 4 x 4 x 8) or be zero-sized / absent (tiles of TW consecutive columns).
+// Kernel rows are cut into slices of about slice_rows rows (balanced by
+// nonzeros); xcds is the number of L2 domains to queue the slices on.
 int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids, int P, int TW,
-                     int row_chunk, int damp_chunk, int col_chunk, BatchPlan *plan, std::string *err);
+                     int row_chunk, int damp_chunk, int col_chunk, int slice_rows, int xcds, BatchPlan *plan,
+                     std::string *err);
 
 }  // namespace plsqr

@@ -239,46 +239,41 @@ __global__ __launch_bounds__(BS) void kb_rows(int nch, const int *__restrict__ c
 }
 
 // ---------------------------------------------------------------------------
-// Mode 2 on tiles: v[c] <- fl(v*vscale) + (A_k^T u' + D^T u')[c] for the
-// columns of one brick per workgroup, u' = u*ib (the pending 1/beta).  The
-// rays crossing the brick are gathered once each (all K values) into LDS,
-// CAP at a time, and reused for every voxel they cross; the round cuts of the
-// layout give each voxel's slot range per round.  Right-hand sides
-// with beta = 0 keep v (lsqr.c:530, 577).  part[t*K + k] = the tile's sum of
-// squares of the new v.
+// Mode 2 on (tile, row slice) items: P[(pbase + c)*K + k] = sum over the
+// slice's rows r of a(r, c) * u'[r][k] for the columns c of one brick, u' =
+// u*ib (the pending 1/beta).  Workgroup b runs item blk_item[b] (-1: none):
+// the rays of the slice crossing the brick are gathered once each (all K
+// values) into LDS, CAP at a time, and reused for every voxel they cross; the
+// round cuts of the layout give each voxel's slot range per round.
+// kb_combine then adds the slices and the damping columns into v.
 // ---------------------------------------------------------------------------
 template <class T, int K, int TW, int BS, int CAP, int CUT>
 __global__ __launch_bounds__(BS) void kb_cols(
-    const int *__restrict__ t_prob, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
+    const int *__restrict__ blk_item, const int *__restrict__ i_prob, const int *__restrict__ i_c0,
+    const int *__restrict__ i_c1, const long long *__restrict__ i_pbase, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len, const int *__restrict__ seg_ray,
     const T *__restrict__ sval, const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
-    const uint16_t *__restrict__ vox_cut, const int *__restrict__ dcp,
-    const int *__restrict__ dri, const double *__restrict__ dcv, const T *__restrict__ u, T *__restrict__ v,
-    const Scal *__restrict__ sc, double *__restrict__ part) {
+    const uint16_t *__restrict__ vox_cut, const T *__restrict__ u, const Scal *__restrict__ sc,
+    double *__restrict__ P) {
     static_assert(BS % TW == 0 && CAP % BS == 0 && CAP % CUT == 0, "tile shape");
     constexpr int NW = BS / TW;
     constexpr int B = (CAP / BS) < 8 ? (CAP / BS) : 8;
     __shared__ T g[CAP * K];
     __shared__ Acc<T> red[NW][K][TW];
-    __shared__ double sh[K][BS / kWave];
-    const int it = blockIdx.x;
-    const int p = t_prob[it];
-    bool live[K];
-    T ibT[K], vsT[K];
-    double ib[K];
+    const int it = blk_item[blockIdx.x];
+    if (it < 0) return;
+    const int p = i_prob[it];
+    T ibT[K];
     bool any = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const Scal &s = sc[p * K + k];
-        live[k] = !s.done && s.beta_pos;
-        ib[k] = s.ib;
         ibT[k] = (T)s.ib;
-        vsT[k] = (T)s.vscale;
-        any = any || live[k];
+        any = any || (!s.done && s.beta_pos);
     }
     if (!any) return;  // uniform across the block, before any barrier
-    const int c0 = tile_start[it];
-    const int tw = tile_start[it + 1] - c0;
+    const int c0 = i_c0[it];
+    const int tw = i_c1[it] - c0;
     const int vx = threadIdx.x % TW;
     const int w = threadIdx.x / TW;
     const long long base = slot_base[it];
@@ -349,38 +344,87 @@ __global__ __launch_bounds__(BS) void kb_cols(
 #pragma unroll
     for (int k = 0; k < K; ++k) red[w][k][vx] = acc[k];
     __syncthreads();
-    double ssq[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) ssq[k] = 0.0;
     if (w == 0 && vx < tw) {
-        const int c = c0 + vx;
-        double td[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) td[k] = 0.0;
-        for (int e = dcp[c]; e < dcp[c + 1]; ++e) {
-            const VK<T, K> q = ldk<T, K>(u, (size_t)dri[e]);
-            const double d = dcv[e];
-#pragma unroll
-            for (int k = 0; k < K; ++k) td[k] += d * ((double)q.a[k] * ib[k]);
-        }
-        VK<T, K> vv = ldk<T, K>(v, (size_t)c);
+        VK<double, K> out;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             Acc<T> tot = red[0][k][vx];
 #pragma unroll
             for (int q = 1; q < NW; ++q) tot.merge(red[q][k][vx]);
-            if (live[k]) {
-                const T vn = vv.a[k] * vsT[k] + (T)(tot.sum64() + td[k]);
-                vv.a[k] = vn;
-                ssq[k] += (double)vn * (double)vn;
-            }
+            out.a[k] = tot.sum64();
         }
-        stk<T, K>(v, (size_t)c, vv);
+        stk<double, K>(P, (size_t)(i_pbase[it] + c0 + vx), out);
     }
-    block_sum_k<BS, K>(ssq, sh);
-    if (threadIdx.x == 0) {
+}
+
+// v[c] <- fl(v*vscale) + (sum_s P_s[c] + D^T u'[c]) over column chunks, for
+// the right-hand sides with beta > 0 (lsqr.c:530, 577): the slices summed in
+// slice order, the damping column in row order (x += (tk + td),
+// aprod_impl.c:487-489).  part[ch*K + k] = the chunk's sum of squares of the
+// new v.
+template <class T, int K, int BS>
+__global__ __launch_bounds__(BS) void kb_combine(int ncc, const int *__restrict__ cc_col,
+                                                 const int *__restrict__ cc_prob, const long long *__restrict__ pofs,
+                                                 const int *__restrict__ nslice, const long long *__restrict__ col0,
+                                                 const int *__restrict__ ncol, const double *__restrict__ P,
+                                                 const int *__restrict__ dcp, const int *__restrict__ dri,
+                                                 const double *__restrict__ dcv, const T *__restrict__ u,
+                                                 T *__restrict__ v, const Scal *__restrict__ sc,
+                                                 double *__restrict__ part) {
+    __shared__ double sh[K][BS / kWave];
+    for (int ch = blockIdx.x; ch < ncc; ch += gridDim.x) {
+        const int p = cc_prob[ch];
+        bool live[K];
+        double ib[K];
+        T vsT[K];
+        bool any = false;
 #pragma unroll
-        for (int k = 0; k < K; ++k) part[(size_t)it * K + k] = ssq[k];
+        for (int k = 0; k < K; ++k) {
+            const Scal &s = sc[p * K + k];
+            live[k] = !s.done && s.beta_pos;
+            ib[k] = s.ib;
+            vsT[k] = (T)s.vscale;
+            any = any || live[k];
+        }
+        if (!any) continue;  // uniform across the block
+        const int S = nslice[p], n = ncol[p];
+        const long long pb = pofs[p], cb = col0[p];
+        double ssq[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) ssq[k] = 0.0;
+        for (int c = cc_col[ch] + threadIdx.x; c < cc_col[ch + 1]; c += BS) {
+            const long long cl = c - cb;
+            VK<double, K> tk = ldk<double, K>(P, (size_t)(pb + cl));
+            for (int q = 1; q < S; ++q) {
+                const VK<double, K> t2 = ldk<double, K>(P, (size_t)(pb + (long long)q * n + cl));
+#pragma unroll
+                for (int k = 0; k < K; ++k) tk.a[k] += t2.a[k];
+            }
+            double td[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) td[k] = 0.0;
+            for (int e = dcp[c]; e < dcp[c + 1]; ++e) {
+                const VK<T, K> q = ldk<T, K>(u, (size_t)dri[e]);
+                const double d = dcv[e];
+#pragma unroll
+                for (int k = 0; k < K; ++k) td[k] += d * ((double)q.a[k] * ib[k]);
+            }
+            VK<T, K> vv = ldk<T, K>(v, (size_t)c);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (live[k]) {
+                    const T vn = vv.a[k] * vsT[k] + (T)(tk.a[k] + td[k]);
+                    vv.a[k] = vn;
+                    ssq[k] += (double)vn * (double)vn;
+                }
+            }
+            stk<T, K>(v, (size_t)c, vv);
+        }
+        block_sum_k<BS, K>(ssq, sh);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) part[(size_t)ch * K + k] = ssq[k];
+        }
     }
 }
 

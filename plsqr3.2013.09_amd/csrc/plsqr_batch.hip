@@ -45,7 +45,8 @@ struct plsqr_batch {
     int cu = 256;
     int P = 0, K = 1, Kr = 1;
     long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0;
-    int tiles = 0, nrch = 0, ncch = 0;
+    int items = 0, nblk = 0, nrch = 0, ncch = 0;
+    long long partials = 0;
     // caller layout offsets (host)
     std::vector<int> mk, md, np;
     std::vector<long long> row0, col0, xb, yb, bb;
@@ -55,7 +56,8 @@ struct plsqr_batch {
     float *rv32 = nullptr;
     int *dcp = nullptr, *dri = nullptr;
     double *dcv = nullptr;
-    int *t_prob = nullptr, *t_start = nullptr, *seg_ray = nullptr;
+    int *i_prob = nullptr, *i_c0 = nullptr, *i_c1 = nullptr, *blk_item = nullptr, *seg_ray = nullptr;
+    long long *i_pbase = nullptr;
     long long *slot_base = nullptr, *seg_start = nullptr, *cut_base = nullptr;
     uint16_t *vox_len = nullptr, *sseg = nullptr, *vox_cut = nullptr;
     double *sval = nullptr;
@@ -63,7 +65,10 @@ struct plsqr_batch {
     // pieces and ranges
     int *rch_row = nullptr, *rch_prob = nullptr, *cch_col = nullptr, *cch_prob = nullptr;
     uint8_t *rch_kind = nullptr;
-    int *r_rch = nullptr, *r_tile = nullptr, *r_cch = nullptr;
+    int *r_rch = nullptr, *r_cch = nullptr;
+    long long *d_pofs = nullptr;
+    int *d_nslice = nullptr;
+    double *pp = nullptr;  // slice partials of mode 2
     // caller layout maps
     int *row_prob = nullptr, *row_old = nullptr, *col_prob = nullptr, *col_old = nullptr;
     long long *d_row0 = nullptr, *d_col0 = nullptr, *d_xb = nullptr, *d_yb = nullptr, *d_bb = nullptr;
@@ -177,18 +182,24 @@ int rows_k(plsqr_batch *b, const Shape &s, const T *av, const T *xv, T *yv, cons
 template <class T, int K>
 int cols_k(plsqr_batch *b, const Shape &s, const T *sv, const T *uv, T *vv, const Scal *sc) {
     constexpr int CAP = LDS_GATHER / (K * (int)sizeof(T));
-    const dim3 g(b->tiles);
-#define COLS_CASE(BS_)                                                                                        \
-    if (s.bs2 == BS_) {                                                                                      \
-        hipLaunchKernelGGL((kb_cols<T, K, TW, BS_, CAP, plsqr::TileLayout::CUT>), g, dim3(BS_), 0, b->stream,   \
-                           b->t_prob, b->t_start, b->slot_base, b->seg_start, b->vox_len, b->seg_ray, sv,       \
-                           b->sseg, b->cut_base, b->vox_cut, b->dcp, b->dri, b->dcv, uv, vv, sc, b->part);       \
-        BT_LAUNCH(b);                                                                                         \
-        return PLSQR_OK;                                                                                      \
+    const dim3 g(b->nblk);
+    bool done = false;
+#define COLS_CASE(BS_)                                                                                          \
+    if (s.bs2 == BS_) {                                                                                        \
+        hipLaunchKernelGGL((kb_cols<T, K, TW, BS_, CAP, plsqr::TileLayout::CUT>), g, dim3(BS_), 0, b->stream,     \
+                           b->blk_item, b->i_prob, b->i_c0, b->i_c1, b->i_pbase, b->slot_base, b->seg_start,       \
+                           b->vox_len, b->seg_ray, sv, b->sseg, b->cut_base, b->vox_cut, uv, sc, b->pp);          \
+        done = true;                                                                                           \
     }
     COLS_CASE(128) COLS_CASE(256) COLS_CASE(512)
 #undef COLS_CASE
-    return set_err(b, PLSQR_ERR_ARG, "unsupported mode-2 launch shape");
+    if (!done) return set_err(b, PLSQR_ERR_ARG, "unsupported mode-2 launch shape");
+    BT_LAUNCH(b);
+    hipLaunchKernelGGL((kb_combine<T, K, 256>), dim3(b->ncch), dim3(256), 0, b->stream, b->ncch, b->cch_col,
+                       b->cch_prob, b->d_pofs, b->d_nslice, b->d_col0, b->d_ncol, (const double *)b->pp, b->dcp,
+                       b->dri, b->dcv, uv, vv, sc, b->part);
+    BT_LAUNCH(b);
+    return PLSQR_OK;
 }
 
 template <class T>
@@ -352,7 +363,7 @@ int enqueue_iteration(plsqr_batch *b, const Shape &s, const T *av, const T *sv, 
     if ((rc = launch_rows<T>(b, s, av, v, u, b->sc))) return rc;
     if ((rc = launch_scalar<PH_BETA>(b, b->r_rch))) return rc;
     if ((rc = launch_cols<T>(b, s, sv, u, v, b->sc))) return rc;
-    if ((rc = launch_scalar<PH_ALPHA>(b, b->r_tile))) return rc;
+    if ((rc = launch_scalar<PH_ALPHA>(b, b->r_cch))) return rc;
     if ((rc = launch_update<T>(b, v, xacc))) return rc;
     if ((rc = launch_scalar<PH_EST>(b, b->r_cch))) return rc;
     return PLSQR_OK;
@@ -366,7 +377,7 @@ int enqueue_start(plsqr_batch *b, const Shape &s, const T *sv, T *u, T *v) {
     if ((rc = launch_ssq_rows<T>(b, u))) return rc;
     if ((rc = launch_scalar<PH_INIT_BETA>(b, b->r_rch))) return rc;
     if ((rc = launch_cols<T>(b, s, sv, u, v, b->sc))) return rc;
-    if ((rc = launch_scalar<PH_INIT_ALPHA>(b, b->r_tile))) return rc;
+    if ((rc = launch_scalar<PH_INIT_ALPHA>(b, b->r_cch))) return rc;
     return launch_init_vw<T>(b, v);
 }
 
@@ -508,7 +519,7 @@ int true_norms(plsqr_batch *b, const double *xv, std::vector<double> &sums) {
     const int S = b->P * b->K;
     if ((rc = launch_residual(b, xv)) || (rc = launch_state_sums(b, b->r_rch, b->sums)) || (rc = fill(b, b->v64, b->cols * b->K, 0.0)) ||
         (rc = launch_cols<double>(b, b->shape[0], b->sval, b->u64, b->v64, b->sc_plain)) ||
-        (rc = launch_state_sums(b, b->r_tile, b->sums + S)))
+        (rc = launch_state_sums(b, b->r_cch, b->sums + S)))
         return rc;
     sums.resize((size_t)S * 2);
     BT_TRY(b, hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, b->stream));
@@ -767,7 +778,9 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     b->slots = plan.slots;
     b->segs = plan.segs;
     b->cuts = plan.cuts;
-    b->tiles = plan.tiles;
+    b->items = plan.items;
+    b->nblk = (int)plan.blk_item.size();
+    b->partials = plan.partials;
     b->nrch = (int)plan.rch_prob.size();
     b->ncch = (int)plan.cch_prob.size();
     b->mk.resize((size_t)P);
@@ -791,15 +804,18 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     if ((rc = balloc(b, &b->rp, (size_t)b->rows + 1)) || (rc = balloc(b, &b->ci, (size_t)b->nnz)) ||
         (rc = balloc(b, &b->rv, (size_t)b->nnz)) || (rc = balloc(b, &b->dcp, (size_t)b->cols + 1)) ||
         (rc = balloc(b, &b->dri, (size_t)b->dnnz)) || (rc = balloc(b, &b->dcv, (size_t)b->dnnz)) ||
-        (rc = balloc(b, &b->t_prob, (size_t)b->tiles)) || (rc = balloc(b, &b->t_start, (size_t)b->tiles + 1)) ||
-        (rc = balloc(b, &b->slot_base, (size_t)b->tiles + 1)) || (rc = balloc(b, &b->seg_start, (size_t)b->tiles + 1)) ||
-        (rc = balloc(b, &b->vox_len, (size_t)b->tiles * TW)) || (rc = balloc(b, &b->seg_ray, (size_t)b->segs)) ||
+        (rc = balloc(b, &b->i_prob, (size_t)b->items)) || (rc = balloc(b, &b->i_c0, (size_t)b->items)) ||
+        (rc = balloc(b, &b->i_c1, (size_t)b->items)) || (rc = balloc(b, &b->i_pbase, (size_t)b->items)) ||
+        (rc = balloc(b, &b->blk_item, (size_t)b->nblk)) || (rc = balloc(b, &b->pp, (size_t)b->partials * K)) ||
+        (rc = balloc(b, &b->d_pofs, (size_t)P + 1)) || (rc = balloc(b, &b->d_nslice, (size_t)P)) ||
+        (rc = balloc(b, &b->slot_base, (size_t)b->items + 1)) || (rc = balloc(b, &b->seg_start, (size_t)b->items + 1)) ||
+        (rc = balloc(b, &b->vox_len, (size_t)b->items * TW)) || (rc = balloc(b, &b->seg_ray, (size_t)b->segs)) ||
         (rc = balloc(b, &b->sval, (size_t)b->slots)) || (rc = balloc(b, &b->sseg, (size_t)b->slots)) ||
-        (rc = balloc(b, &b->cut_base, (size_t)b->tiles + 1)) || (rc = balloc(b, &b->vox_cut, (size_t)b->cuts * TW)) ||
+        (rc = balloc(b, &b->cut_base, (size_t)b->items + 1)) || (rc = balloc(b, &b->vox_cut, (size_t)b->cuts * TW)) ||
         (rc = balloc(b, &b->rch_row, (size_t)b->nrch + 1)) || (rc = balloc(b, &b->rch_prob, (size_t)b->nrch)) ||
         (rc = balloc(b, &b->rch_kind, (size_t)b->nrch)) || (rc = balloc(b, &b->cch_col, (size_t)b->ncch + 1)) ||
         (rc = balloc(b, &b->cch_prob, (size_t)b->ncch)) || (rc = balloc(b, &b->r_rch, (size_t)P + 1)) ||
-        (rc = balloc(b, &b->r_tile, (size_t)P + 1)) || (rc = balloc(b, &b->r_cch, (size_t)P + 1)) ||
+        (rc = balloc(b, &b->r_cch, (size_t)P + 1)) ||
         (rc = balloc(b, &b->row_prob, (size_t)b->rows)) || (rc = balloc(b, &b->row_old, (size_t)b->rows)) ||
         (rc = balloc(b, &b->col_prob, (size_t)b->cols)) ||
         (rc = balloc(b, &b->col_old, (size_t)b->cols)) || (rc = balloc(b, &b->d_row0, (size_t)P + 1)) ||
@@ -829,18 +845,25 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
             (rc = upload(b, b->dcv + plan.dnnz0[(size_t)p], L.dcv.data(), L.dcv.size())))
             return rc;
         const plsqr::TileLayout &T = L.tiles;
-        const int t0 = plan.tile0[(size_t)p];
-        std::vector<int> tp((size_t)T.T, p), ts((size_t)T.T);
-        std::vector<long long> sb((size_t)T.T), sg((size_t)T.T), cb((size_t)T.T);
-        for (int t = 0; t < T.T; ++t) {
-            ts[(size_t)t] = (int)(c0 + T.tile_start[(size_t)t]);
-            sb[(size_t)t] = plan.slot0[(size_t)p] + T.slot_base[(size_t)t];
-            sg[(size_t)t] = plan.seg0[(size_t)p] + T.seg_start[(size_t)t];
-            cb[(size_t)t] = plan.cut0[(size_t)p] + T.cut_base[(size_t)t];
-        }
+        const int t0 = plan.item0[(size_t)p];  // first item of the problem
+        const int nit = T.T * T.S;
+        std::vector<int> tp((size_t)nit, p), ic0((size_t)nit), ic1((size_t)nit);
+        std::vector<long long> sb((size_t)nit), sg((size_t)nit), cb((size_t)nit), pbase((size_t)nit);
+        for (int t = 0; t < T.T; ++t)
+            for (int q = 0; q < T.S; ++q) {
+                const size_t it = (size_t)t * T.S + q;
+                ic0[it] = (int)(c0 + T.tile_start[(size_t)t]);
+                ic1[it] = (int)(c0 + T.tile_start[(size_t)t + 1]);
+                pbase[it] = plan.pofs[(size_t)p] + (long long)q * L.n - c0;
+                sb[it] = plan.slot0[(size_t)p] + T.slot_base[it];
+                sg[it] = plan.seg0[(size_t)p] + T.seg_start[it];
+                cb[it] = plan.cut0[(size_t)p] + T.cut_base[it];
+            }
         std::vector<int> ray(T.seg_ray.size());
         for (size_t i = 0; i < ray.size(); ++i) ray[i] = (int)(r0 + T.seg_ray[i]);
-        if ((rc = upload(b, b->t_prob + t0, tp.data(), tp.size())) || (rc = upload(b, b->t_start + t0, ts.data(), ts.size())) ||
+        if ((rc = upload(b, b->i_prob + t0, tp.data(), tp.size())) || (rc = upload(b, b->i_c0 + t0, ic0.data(), ic0.size())) ||
+            (rc = upload(b, b->i_c1 + t0, ic1.data(), ic1.size())) ||
+            (rc = upload(b, b->i_pbase + t0, pbase.data(), pbase.size())) ||
             (rc = upload(b, b->slot_base + t0, sb.data(), sb.size())) ||
             (rc = upload(b, b->seg_start + t0, sg.data(), sg.size())) ||
             (rc = upload(b, b->cut_base + t0, cb.data(), cb.size())) ||
@@ -864,32 +887,35 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     }
     const int rows_i = (int)b->rows, cols_i = (int)b->cols;
     const int nnz_i = (int)b->nnz, dnnz_i = (int)b->dnnz;
-    std::vector<int> tail_t{(int)b->cols};
     std::vector<long long> tail_sb{b->slots}, tail_sg{b->segs}, tail_cb{b->cuts};
-    std::vector<int> mrow((size_t)P);
-    for (int p = 0; p < P; ++p) mrow[(size_t)p] = b->mk[(size_t)p] + b->md[(size_t)p];
+    std::vector<int> mrow((size_t)P), nsl((size_t)P);
+    for (int p = 0; p < P; ++p) {
+        mrow[(size_t)p] = b->mk[(size_t)p] + b->md[(size_t)p];
+        nsl[(size_t)p] = plan.nslice[(size_t)p];
+    }
     if ((rc = upload(b, b->rp + rows_i, &nnz_i, 1)) || (rc = upload(b, b->dcp + cols_i, &dnnz_i, 1)) ||
-        (rc = upload(b, b->t_start + b->tiles, tail_t.data(), 1)) ||
-        (rc = upload(b, b->slot_base + b->tiles, tail_sb.data(), 1)) ||
-        (rc = upload(b, b->seg_start + b->tiles, tail_sg.data(), 1)) ||
-        (rc = upload(b, b->cut_base + b->tiles, tail_cb.data(), 1)) ||
+        (rc = upload(b, b->slot_base + b->items, tail_sb.data(), 1)) ||
+        (rc = upload(b, b->seg_start + b->items, tail_sg.data(), 1)) ||
+        (rc = upload(b, b->cut_base + b->items, tail_cb.data(), 1)) ||
+        (rc = upload(b, b->blk_item, plan.blk_item.data(), plan.blk_item.size())) ||
+        (rc = upload(b, b->d_pofs, plan.pofs.data(), plan.pofs.size())) ||
         (rc = upload(b, b->rch_row, plan.rch_row.data(), plan.rch_row.size())) ||
         (rc = upload(b, b->rch_prob, plan.rch_prob.data(), plan.rch_prob.size())) ||
         (rc = upload(b, b->rch_kind, plan.rch_kind.data(), plan.rch_kind.size())) ||
         (rc = upload(b, b->cch_col, plan.cch_col.data(), plan.cch_col.size())) ||
         (rc = upload(b, b->cch_prob, plan.cch_prob.data(), plan.cch_prob.size())) ||
         (rc = upload(b, b->r_rch, plan.rch0.data(), plan.rch0.size())) ||
-        (rc = upload(b, b->r_tile, plan.tile0.data(), plan.tile0.size())) ||
         (rc = upload(b, b->r_cch, plan.cch0.data(), plan.cch0.size())) ||
         (rc = upload(b, b->d_row0, b->row0.data(), b->row0.size())) ||
         (rc = upload(b, b->d_col0, b->col0.data(), b->col0.size())) ||
         (rc = upload(b, b->d_xb, b->xb.data(), b->xb.size())) || (rc = upload(b, b->d_yb, b->yb.data(), b->yb.size())) ||
         (rc = upload(b, b->d_bb, b->bb.data(), b->bb.size())) || (rc = upload(b, b->d_mk, b->mk.data(), b->mk.size())) ||
-        (rc = upload(b, b->d_mrow, mrow.data(), mrow.size())) || (rc = upload(b, b->d_ncol, b->np.data(), b->np.size())))
+        (rc = upload(b, b->d_mrow, mrow.data(), mrow.size())) || (rc = upload(b, b->d_ncol, b->np.data(), b->np.size())) ||
+        (rc = upload(b, b->d_nslice, nsl.data(), nsl.size())))
         return rc;
     BT_TRY(b, hipStreamSynchronize(b->stream));
     // vectors and scratch
-    b->part_len = std::max(b->nrch, std::max(b->tiles, b->ncch)) * K;
+    b->part_len = std::max(b->nrch, b->ncch) * K;
     b->stage_len = std::max(b->yb[(size_t)P], b->xb[(size_t)P]);
     const int S = P * K;
     if ((rc = balloc(b, &b->u64, (size_t)b->rows * K)) || (rc = balloc(b, &b->v64, (size_t)b->cols * K)) ||
@@ -941,14 +967,18 @@ int plsqr_batch_create(const plsqr_system *const *systems, const plsqr_grid *gri
     }
     plsqr::BatchPlan plan;
     std::string why;
-    int rc = plsqr::build_batch_plan(systems, grids, n_problems, TW, ROW_CHUNK, DAMP_CHUNK, COL_CHUNK, &plan, &why);
+    const int Kp = n_rhs == 1 ? 1 : (n_rhs == 2 ? 2 : 4);
+    // rows per slice: the slice's u (Kp FP64 values per row) in 2 MB, half an XCD's L2
+    const int slice_rows = std::max(4096, env_int("PLSQR_BATCH_SLICE_BYTES", 2 << 20) / (Kp * 8));
+    int rc = plsqr::build_batch_plan(systems, grids, n_problems, TW, ROW_CHUNK, DAMP_CHUNK, COL_CHUNK, slice_rows,
+                                     8, &plan, &why);
     if (rc) {
         g_batch_err = why;
         return rc;
     }
     plsqr_batch *b = new plsqr_batch();
     b->Kr = n_rhs;
-    b->K = n_rhs == 1 ? 1 : (n_rhs == 2 ? 2 : 4);
+    b->K = Kp;
     auto fail = [&](int code) {
         g_batch_err = b->err;
         plsqr_batch_destroy(b);
