@@ -194,23 +194,29 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         order_tile_columns(s, tiles, perm);
         // row slices balanced by nonzeros, about slice_rows rows each
         const int S = std::max(1, std::min(64, (s->m_k + std::max(slice_rows, 1) - 1) / std::max(slice_rows, 1)));
-        std::vector<int> slices((size_t)S + 1, 0);
-        {
-            std::vector<long long> cnt((size_t)s->m_k + 1, 0);
-            for (long long e = 0; e < s->nnz_k; ++e) cnt[(size_t)s->k_row_idx[e] + 1]++;
-            for (int r = 0; r < s->m_k; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
+        auto cut_slices = [S](const plsqr_system *q_s) {
+            std::vector<int> sl((size_t)S + 1, 0);
+            std::vector<long long> cnt((size_t)q_s->m_k + 1, 0);
+            for (long long e = 0; e < q_s->nnz_k; ++e) cnt[(size_t)q_s->k_row_idx[e] + 1]++;
+            for (int r = 0; r < q_s->m_k; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
             for (int q = 1; q < S; ++q) {
-                const long long target = s->nnz_k * q / S;
+                const long long target = q_s->nnz_k * q / S;
                 int b = (int)(std::lower_bound(cnt.begin(), cnt.end(), target) - cnt.begin());
-                slices[(size_t)q] = std::min(std::max(b, slices[(size_t)q - 1]), s->m_k);
+                sl[(size_t)q] = std::min(std::max(b, sl[(size_t)q - 1]), q_s->m_k);
             }
-            slices[(size_t)S] = s->m_k;
-        }
+            sl[(size_t)S] = q_s->m_k;
+            return sl;
+        };
+        // rays sorted by (first tile, last tile): within the slices (default), or
+        // over all rays with the slices cut afterwards on the sorted rows
+        const bool global_sort = std::getenv("PLSQR_BATCH_RAYSORT") && std::atoi(std::getenv("PLSQR_BATCH_RAYSORT")) == 2;
+        std::vector<int> slices = global_sort ? std::vector<int>() : cut_slices(s);
         L.S = S;
         RaySorted rays;
         sort_rays(s, perm, tiles, slices, &rays);
         L.rperm = rays.rperm;
         const plsqr_system *ss = &rays.sys;
+        if (global_sort) slices = cut_slices(ss);
         if (!build_tile_layout(ss, perm, tiles, slices, TW, &L.tiles, err)) {
             *err = "system " + std::to_string(p) + ": " + *err;
             return PLSQR_ERR_FORMAT;

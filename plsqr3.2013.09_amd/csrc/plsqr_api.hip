@@ -62,8 +62,8 @@ struct plsqr_op {
     bool tiled = false;
     int tS = 1, tT = 0, tmax_seg = 0;
     int *t_start = nullptr, *t_ray = nullptr;
-    long long *t_slot = nullptr, *t_seg = nullptr;
-    uint16_t *t_vlen = nullptr, *t_sseg = nullptr;
+    long long *t_slot = nullptr, *t_seg = nullptr, *t_cut = nullptr;
+    uint16_t *t_vlen = nullptr, *t_sseg = nullptr, *t_vcut = nullptr;
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
     bool nt_loads = true;
@@ -173,15 +173,15 @@ int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double
             const int q = std::max(1, std::min(op->tT, op->tiled_persistent / op->tS));
             hipLaunchKernelGGL((k_cols_tiled_persistent<TILE_W, BS, TILE_CAP, false>), dim3(q * op->tS), dim3(BS), 0,
                                op->stream, op->n, op->tS, op->tT, op->t_start, op->t_slot, op->t_seg, op->t_vlen,
-                               op->t_ray, op->t_sval, op->t_sseg, u, sc, op->P);
+                               op->t_ray, op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
         } else if (op->nt_loads) {
             hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, true>), dim3(op->tT * op->tS), dim3(BS), 0,
                                op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
-                               op->t_sval, op->t_sseg, u, sc, op->P);
+                               op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
         } else {
             hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, false>), dim3(op->tT * op->tS), dim3(BS), 0,
                                op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
-                               op->t_sval, op->t_sseg, u, sc, op->P);
+                               op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
         }
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
@@ -505,8 +505,13 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         if ((rc = dalloc(op, &op->t_start, (size_t)L->T + 1)) || (rc = dalloc(op, &op->t_slot, (size_t)items + 1)) ||
             (rc = dalloc(op, &op->t_seg, (size_t)items + 1)) || (rc = dalloc(op, &op->t_vlen, (size_t)items * L->TW)) ||
             (rc = dalloc(op, &op->t_ray, (size_t)L->n_seg)) || (rc = dalloc(op, &op->t_sval, (size_t)L->n_slot)) ||
-            (rc = dalloc(op, &op->t_sseg, (size_t)L->n_slot)))
+            (rc = dalloc(op, &op->t_sseg, (size_t)L->n_slot)) || (rc = dalloc(op, &op->t_cut, (size_t)items + 1)) ||
+            (rc = dalloc(op, &op->t_vcut, L->vox_cut.size())))
             return rc;
+        HIP_TRY(op, hipMemcpy(op->t_cut, L->cut_base.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
+        if (!L->vox_cut.empty())
+            HIP_TRY(op, hipMemcpy(op->t_vcut, L->vox_cut.data(), sizeof(uint16_t) * L->vox_cut.size(),
+                                  hipMemcpyHostToDevice));
         HIP_TRY(op, hipMemcpy(op->t_start, L->tile_start.data(), sizeof(int) * (L->T + 1), hipMemcpyHostToDevice));
         HIP_TRY(op, hipMemcpy(op->t_slot, L->slot_base.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
         HIP_TRY(op, hipMemcpy(op->t_seg, L->seg_start.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
@@ -684,10 +689,18 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
             tiles.push_back(s->n);
         }
         if (env_int("PLSQR_VOXSORT", 1) != 0) plsqr::order_tile_columns(s, tiles, perm);
-        if (env_int("PLSQR_RAYSORT", 1) != 0) {
+        const int raysort = env_int("PLSQR_RAYSORT", 1);
+        if (raysort != 0) {
             rays.reset(new plsqr::RaySorted());
-            plsqr::sort_rays(s, perm, tiles, hb, rays.get());  // within the row slices: hb stays valid
-            sys = &rays->sys;
+            if (raysort == 2) {
+                // one sort over all rays, slices cut afterwards on the sorted rows
+                plsqr::sort_rays(s, perm, tiles, std::vector<int>(), rays.get());
+                sys = &rays->sys;
+                hb = slice_bounds_host(sys, nsl);
+            } else {
+                plsqr::sort_rays(s, perm, tiles, hb, rays.get());  // within the row slices: hb stays valid
+                sys = &rays->sys;
+            }
         }
         L.reset(new plsqr::TileLayout());
         std::string why;
@@ -1125,7 +1138,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
     hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->n,
                        op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,
-                       op->u32, (const Scal *)op->sc, op->P);
+                       op->t_cut, op->t_vcut, op->u32, (const Scal *)op->sc, op->P);
     hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
                        op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
     LAUNCH_CHECK(op);
@@ -1242,7 +1255,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
         hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
                            op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32,
-                           op->t_sseg, op->u32, (const Scal *)op->sc, op->P);
+                           op->t_sseg, op->t_cut, op->t_vcut, op->u32, (const Scal *)op->sc, op->P);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
         LAUNCH_CHECK(op);

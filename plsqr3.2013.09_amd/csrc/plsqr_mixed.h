@@ -75,7 +75,8 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const float *__restrict__ sval, const uint16_t *__restrict__ sseg,
-    const float *__restrict__ u, const Scal *__restrict__ sc, double *__restrict__ P) {
+    const long long *__restrict__ cut_base, const uint16_t *__restrict__ vox_cut, const float *__restrict__ u,
+    const Scal *__restrict__ sc, double *__restrict__ P) {
     constexpr int NW = BS / TW;
     __shared__ float g[CAP];
     __shared__ F2 part[NW][TW];
@@ -133,12 +134,22 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
                 dot2_add(acc, sv[o], g[ss[o]]);
             }
         } else {
-            while (j < len) {
-                const long long slot = base + (long long)j * TW + v;
-                const int sg = sseg[slot];
-                if (sg >= hi) break;
-                dot2_add(acc, sval[slot], g[sg - r0]);
-                j += NW;
+            int jend = len;
+            if (hi < ns && v < tw) jend = (int)vox_cut[(size_t)(cut_base[it] + hi / CUT - 1) * TW + v];
+            const float *sv = sval + base + v;
+            const uint16_t *ss = sseg + base + v;
+            for (; j + 3 * NW < jend; j += 4 * NW) {
+                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
+                const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                dot2_add(acc, a0, g[g0]);
+                dot2_add(acc, a1, g[g1]);
+                dot2_add(acc, a2, g[g2]);
+                dot2_add(acc, a3, g[g3]);
+            }
+            for (; j < jend; j += NW) {
+                const long long o = (long long)j * TW;
+                dot2_add(acc, sv[o], g[ss[o] - r0]);
             }
         }
         __syncthreads();

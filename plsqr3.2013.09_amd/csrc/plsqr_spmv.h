@@ -185,13 +185,17 @@ __global__ __launch_bounds__(BS) void k_cols_sliced(int n, int nsl, const int *_
 // nonzeros j = w, w+NW, ... of voxel v, reading the segment values from LDS;
 // the NW partial sums of a voxel are then added in a fixed order.
 // Segments beyond the LDS capacity are processed in rounds of CAP.
+constexpr int CUT = 1024;  // segments per round cut of the tiled layout (TileLayout::CUT)
+
 template <int TW, int BS, int CAP, bool NT>
 __device__ __forceinline__ void cols_tiled_item(
     int it, int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
-    const uint16_t *__restrict__ sseg, const double *__restrict__ u, double ib,
+    const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
+    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, double ib,
     double *__restrict__ P, double *g, double (*part)[TW]) {
+    static_assert(CAP % CUT == 0, "rounds end on round cuts");
     constexpr int NW = BS / TW;
     const int s = it % S;
     const int t = it / S;
@@ -248,17 +252,29 @@ __device__ __forceinline__ void cols_tiled_item(
             acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o)];
         }
     } else {
+        // rounds of CAP segments; the layout's round cuts give each voxel's
+        // slot range per round, so every round is a plain unrolled loop
         for (int r0 = 0; r0 < ns; r0 += CAP) {
             const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
             for (int i = threadIdx.x; i < cnt; i += BS) g[i] = u[seg_ray[s0 + r0 + i]] * ib;
             __syncthreads();
             const int hi = r0 + cnt;
-            while (j < len) {
-                const long long slot = base + (long long)j * TW + v;
-                const int sg = sseg[slot];
-                if (sg >= hi) break;
-                acc += sval[slot] * g[sg - r0];
-                j += NW;
+            int jend = len;
+            if (hi < ns && v < tw) jend = (int)vox_cut[(size_t)(cut_base[it] + hi / CUT - 1) * TW + v];
+            const double *sv = sval + base + v;
+            const uint16_t *ss = sseg + base + v;
+            for (; j + 3 * NW < jend; j += 4 * NW) {
+                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
+                const double a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                acc += a0 * g[g0];
+                acc += a1 * g[g1];
+                acc += a2 * g[g2];
+                acc += a3 * g[g3];
+            }
+            for (; j < jend; j += NW) {
+                const long long o = (long long)j * TW;
+                acc += sv[o] * g[ss[o] - r0];
             }
             __syncthreads();
         }
@@ -279,7 +295,8 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
     int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
-    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
+    const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
+    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, const Scal *__restrict__ sc,
     double *__restrict__ P) {
     __shared__ double g[CAP];
     __shared__ double part[BS / TW][TW];
@@ -289,7 +306,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
         ib = sc->ib;
     }
     cols_tiled_item<TW, BS, CAP, NT>(blockIdx.x, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
-                                     sval, sseg, u, ib, P, g, part);
+                                     sval, sseg, cut_base, vox_cut, u, ib, P, g, part);
 }
 
 // Persistent form: gridDim.x = S * Q workgroups; workgroup b keeps slice
@@ -299,7 +316,8 @@ __global__ __launch_bounds__(BS) void k_cols_tiled_persistent(
     int n, int S, int T, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
-    const uint16_t *__restrict__ sseg, const double *__restrict__ u, const Scal *__restrict__ sc,
+    const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
+    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, const Scal *__restrict__ sc,
     double *__restrict__ P) {
     __shared__ double g[CAP];
     __shared__ double part[BS / TW][TW];
@@ -312,7 +330,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled_persistent(
     const int Q = gridDim.x / S;
     for (int t = blockIdx.x / S; t < T; t += Q)
         cols_tiled_item<TW, BS, CAP, NT>(t * S + s, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
-                                         sval, sseg, u, ib, P, g, part);
+                                         sval, sseg, cut_base, vox_cut, u, ib, P, g, part);
 }
 
 // dst[i] = src[idx[i]]  (column renumbering at the API boundary)
