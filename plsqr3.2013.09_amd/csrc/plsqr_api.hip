@@ -66,7 +66,15 @@ struct plsqr_op {
     uint16_t *t_vlen = nullptr, *t_sseg = nullptr, *t_vcut = nullptr;
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
-    bool nt_loads = true;
+    bool nt_loads = false;
+    bool rows_nt = false;  // streaming loads for the mode-1 matrix data
+    // mode 1 on the bricked layout (segment partials + per-ray combine)
+    bool rtiled = false, rt_contig = false;
+    int rt_lc = 4;
+    int *rt_gbase = nullptr, *rt_rptr = nullptr, *rt_rseg = nullptr;
+    long long *rt_ebase = nullptr;
+    uint8_t *rt_len = nullptr, *rt_vox = nullptr;
+    double *rt_val = nullptr, *rt_segv = nullptr;
     // FP32 copies for mixed-precision solves (allocated on first use)
     float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr;
     double *dx = nullptr;
@@ -150,6 +158,35 @@ constexpr int BS = 256;
 int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double *part) {
     using namespace plsqr;
     dim3 g(op->G1), b(BS);
+    if (op->rtiled) {
+        if (op->rt_contig)
+            hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
+                               op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
+                               op->rt_segv);
+        else
+            hipLaunchKernelGGL((k_rows_tiled<BS>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
+                               op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
+                               op->rt_segv);
+        LAUNCH_CHECK(op);
+        switch (op->rt_lc) {
+            case 1: hipLaunchKernelGGL((k_rows_tiled_combine<1, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
+            case 2: hipLaunchKernelGGL((k_rows_tiled_combine<2, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
+            case 8: hipLaunchKernelGGL((k_rows_tiled_combine<8, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
+            default: hipLaunchKernelGGL((k_rows_tiled_combine<4, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
+        }
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
+    if (op->rows_nt && (op->L1 == 8 || op->L1 == 16 || op->L1 == 32)) {
+        if (op->L1 == 8)
+            hipLaunchKernelGGL((k_rows<8, BS, true>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part);
+        else if (op->L1 == 16)
+            hipLaunchKernelGGL((k_rows<16, BS, true>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part);
+        else
+            hipLaunchKernelGGL((k_rows<32, BS, true>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part);
+        LAUNCH_CHECK(op);
+        return PLSQR_OK;
+    }
     switch (op->L1) {
         case 2: hipLaunchKernelGGL((k_rows<2, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
         case 4: hipLaunchKernelGGL((k_rows<4, BS>), g, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, x, y, sc, part); break;
@@ -402,7 +439,8 @@ int exclusive_scan(plsqr_op *op, const int *cnt, int *out, long long len) {
     return PLSQR_OK;
 }
 
-int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const plsqr::TileLayout *L) {
+int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const plsqr::TileLayout *L,
+          const plsqr::RowTiles *R) {
     using namespace plsqr;
     int rc;
     const long long nk = s->nnz_k, nd = s->nnz_d;
@@ -521,6 +559,31 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         if (L->n_slot > 0) {
             HIP_TRY(op, hipMemcpy(op->t_sval, L->sval.data(), sizeof(double) * L->n_slot, hipMemcpyHostToDevice));
             HIP_TRY(op, hipMemcpy(op->t_sseg, L->sseg.data(), sizeof(uint16_t) * L->n_slot, hipMemcpyHostToDevice));
+        }
+        if (R) {
+            op->rtiled = true;
+            op->rt_contig = R->contiguous;
+            if ((rc = dalloc(op, &op->bounds, (size_t)L->S + 1))) return rc;
+            HIP_TRY(op, hipMemcpy(op->bounds, L->slice_bounds.data(), sizeof(int) * (L->S + 1), hipMemcpyHostToDevice));
+            const long long items2 = (long long)L->T * L->S;
+            if ((rc = dalloc(op, &op->rt_gbase, (size_t)items2 + 1)) ||
+                (rc = dalloc(op, &op->rt_ebase, (size_t)R->n_grp + 1)) ||
+                (rc = dalloc(op, &op->rt_len, (size_t)R->n_grp * plsqr::RowTiles::G)) ||
+                (rc = dalloc(op, &op->rt_vox, (size_t)R->n_ent)) || (rc = dalloc(op, &op->rt_val, (size_t)R->n_ent)) ||
+                (rc = dalloc(op, &op->rt_rptr, (size_t)op->m_k + 1)) || (rc = dalloc(op, &op->rt_rseg, (size_t)L->n_seg)) ||
+                (rc = dalloc(op, &op->rt_segv, (size_t)L->n_seg)))
+                return rc;
+            HIP_TRY(op, hipMemcpy(op->rt_gbase, R->grp_base.data(), sizeof(int) * (items2 + 1), hipMemcpyHostToDevice));
+            HIP_TRY(op, hipMemcpy(op->rt_ebase, R->ent_base.data(), sizeof(long long) * (R->n_grp + 1), hipMemcpyHostToDevice));
+            HIP_TRY(op, hipMemcpy(op->rt_rptr, R->ray_ptr.data(), sizeof(int) * (op->m_k + 1), hipMemcpyHostToDevice));
+            if (R->n_grp > 0)
+                HIP_TRY(op, hipMemcpy(op->rt_len, R->seg_len.data(), (size_t)R->n_grp * plsqr::RowTiles::G, hipMemcpyHostToDevice));
+            if (R->n_ent > 0) {
+                HIP_TRY(op, hipMemcpy(op->rt_vox, R->ent_vox.data(), (size_t)R->n_ent, hipMemcpyHostToDevice));
+                HIP_TRY(op, hipMemcpy(op->rt_val, R->ent_val.data(), sizeof(double) * R->n_ent, hipMemcpyHostToDevice));
+            }
+            if (L->n_seg > 0)
+                HIP_TRY(op, hipMemcpy(op->rt_rseg, R->ray_seg.data(), sizeof(int) * L->n_seg, hipMemcpyHostToDevice));
         }
         dfree(op, col, nk);
         dfree(op, op->kcp, (size_t)op->n + 1);
@@ -676,6 +739,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     std::vector<int> hb = slice_bounds_host(s, nsl);
     // tiled mode-2 layout on bricks of voxels (default operator only)
     std::unique_ptr<plsqr::TileLayout> L;
+    std::unique_ptr<plsqr::RowTiles> R;
     std::unique_ptr<plsqr::RaySorted> rays;
     const plsqr_system *sys = s;  // s, or its ray-sorted renumbering
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
@@ -709,6 +773,17 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(sys, *L, &why)) {
             return fail(set_err(op, PLSQR_ERR_STATE, why));
         }
+        if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
+            R.reset(new plsqr::RowTiles());
+            const int win = std::max(0, env_int("PLSQR_SEGWIN", 0)) / plsqr::RowTiles::G * plsqr::RowTiles::G;
+            const bool contig = env_int("PLSQR_ROWLAYOUT", 1) != 0;
+            if (!(contig ? plsqr::build_row_tiles_contiguous(*L, sys->m_k, R.get(), &why)
+                         : plsqr::build_row_tiles(*L, sys->m_k, win, R.get(), &why))) {
+                R.reset();  // mode 1 stays on the CSR kernel
+            } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::row_tiles_self_check(sys, *L, *R, &why)) {
+                return fail(set_err(op, PLSQR_ERR_STATE, why));
+            }
+        }
     }
     hipError_t e = hipGetDevice(&op->dev);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
@@ -720,7 +795,8 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     op->own_stream = true;
-    if ((rc = build(op, sys, hb, L.get()))) return fail(rc);
+    if ((rc = build(op, sys, hb, L.get(), R.get()))) return fail(rc);
+    R.reset();
     L.reset();
     if (rays) {
         op->rowperm = true;
@@ -733,11 +809,14 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         rays.reset();
     }
     // vectors and scratch
-    op->nt_loads = env_int("PLSQR_NT", 1) != 0;
+    op->nt_loads = env_int("PLSQR_NT", 0) != 0;
+    op->rows_nt = env_int("PLSQR_ROWS_NT", 0) != 0;
+    op->rt_lc = env_int("PLSQR_LC", 4);
     op->tiled_persistent = env_int("PLSQR_TILED_WG", 0);
     op->L1 = env_int("PLSQR_L1", 16);
     op->L2 = env_int("PLSQR_L2", 16);
     op->G1 = grid_for((long long)op->m_k * op->L1, BS, env_int("PLSQR_G1", op->cu * 8));
+    if (op->rtiled) op->G1 = std::max(op->tS, op->G1 / op->tS * op->tS);  // the combine deals slices by b % S
     {
         int q = grid_for((long long)op->n * op->L2, BS, std::max(1, env_int("PLSQR_G2", op->cu * 8) / op->nsl));
         op->G2 = q * op->nsl;

@@ -38,7 +38,7 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // part[2b], part[2b+1] receive block b's sums of squares of the new y over the
 // kernel rows and the damping rows (the two halves of the split norm,
 // mycblas.c:99-133).
-template <int L, int BS>
+template <int L, int BS, bool NT = false>
 __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restrict__ rp,
                                              const int *__restrict__ ci,
                                              const double *__restrict__ av,
@@ -66,15 +66,17 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
             int k = b + lane;
             // four independent gathers in flight per lane
             for (; k + 3 * L < e; k += 4 * L) {
-                const int c0 = ci[k], c1 = ci[k + L], c2 = ci[k + 2 * L], c3 = ci[k + 3 * L];
-                const double a0 = av[k], a1 = av[k + L], a2 = av[k + 2 * L], a3 = av[k + 3 * L];
+                const int c0 = ld_stream<NT>(ci + k), c1 = ld_stream<NT>(ci + k + L),
+                          c2 = ld_stream<NT>(ci + k + 2 * L), c3 = ld_stream<NT>(ci + k + 3 * L);
+                const double a0 = ld_stream<NT>(av + k), a1 = ld_stream<NT>(av + k + L),
+                             a2 = ld_stream<NT>(av + k + 2 * L), a3 = ld_stream<NT>(av + k + 3 * L);
                 const double v0 = a0 * x[c0], v1 = a1 * x[c1], v2 = a2 * x[c2], v3 = a3 * x[c3];
                 acc += v0;
                 acc += v1;
                 acc += v2;
                 acc += v3;
             }
-            for (; k < e; k += L) acc += av[k] * x[ci[k]];
+            for (; k < e; k += L) acc += ld_stream<NT>(av + k) * x[ld_stream<NT>(ci + k)];
         }
         acc = group_sum<L>(acc);
         if (lane == 0 && r < m_k) {
@@ -86,6 +88,203 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
         }
     }
     // damping rows: one lane per row, columns ascending (CPUSpMV.c:75-84)
+    for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
+         r += (long long)gridDim.x * BS) {
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) acc += av[k] * x[ci[k]];
+        double yn = y[r];
+        if (scaled) yn = (yn * s1) * s2;
+        yn = yn + acc;
+        y[r] = yn;
+        ssq_d += yn * yn;
+    }
+    if (part) {
+        const double a = block_sum<BS>(ssq_k, sh);
+        const double d = block_sum<BS>(ssq_d, sh);
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = a;
+            part[2 * blockIdx.x + 1] = d;
+        }
+    }
+}
+
+// Mode 1 on the bricked layout, pass 1: one workgroup per (tile, row slice)
+// item stages the tile of x (at most 128 values) in LDS; each wavefront
+// takes groups of 64 segments, one per lane, and sums its segment's entries
+// (ascending voxel order) against the staged values.  The lanes of a group
+// hold the item's segments in descending length, so lane 0 has the longest
+// and the step loop runs to its length.  The partial goes to
+// segv[seg_start[item] + position of the segment in that order].
+template <int BS>
+__global__ __launch_bounds__(BS) void k_rows_tiled(int S, const int *__restrict__ tile_start,
+                                                   const long long *__restrict__ seg_start,
+                                                   const int *__restrict__ grp_base,
+                                                   const long long *__restrict__ ent_base,
+                                                   const uint8_t *__restrict__ seg_len,
+                                                   const uint8_t *__restrict__ ent_vox,
+                                                   const double *__restrict__ ent_val, const double *__restrict__ x,
+                                                   const Scal *__restrict__ sc, double *__restrict__ segv) {
+    constexpr int G = 64;
+    __shared__ double xl[128];
+    if (sc && sc->done) return;
+    const int it = blockIdx.x;
+    const int g0 = grp_base[it], g1 = grp_base[it + 1];
+    if (g0 == g1) return;  // the same for the whole workgroup
+    const int t = it / S;
+    const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
+    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x % G;
+    const long long sb = seg_start[it];
+    for (int g = g0 + (int)threadIdx.x / G; g < g1; g += BS / G) {
+        const int len = seg_len[(size_t)g * G + lane];
+        const int kmax = seg_len[(size_t)g * G];  // lane 0: the group's longest segment
+        const long long o = ent_base[g] + lane;
+        double acc = 0.0;
+        int k = 0;
+        for (; k + 3 < kmax; k += 4) {
+            const long long o0 = o + (long long)k * G;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+            if (k < len) { a0 = ent_val[o0]; v0 = ent_vox[o0]; }
+            if (k + 1 < len) { a1 = ent_val[o0 + G]; v1 = ent_vox[o0 + G]; }
+            if (k + 2 < len) { a2 = ent_val[o0 + 2 * G]; v2 = ent_vox[o0 + 2 * G]; }
+            if (k + 3 < len) { a3 = ent_val[o0 + 3 * G]; v3 = ent_vox[o0 + 3 * G]; }
+            if (k < len) acc += a0 * xl[v0];
+            if (k + 1 < len) acc += a1 * xl[v1];
+            if (k + 2 < len) acc += a2 * xl[v2];
+            if (k + 3 < len) acc += a3 * xl[v3];
+        }
+        for (; k < len; ++k) {
+            const long long ok = o + (long long)k * G;
+            acc += ent_val[ok] * xl[ent_vox[ok]];
+        }
+        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = acc;
+    }
+}
+
+// Pass 1 on the contiguous layout: the item's segments in ray order, each
+// segment's entries stored together.  A wavefront takes a group of 64
+// segments (one per lane), copies the group's entries into its own LDS
+// buffer with coalesced loads, CH at a time, and every lane adds the entries
+// of its segment found in the buffer (its first entry from a wavefront
+// prefix sum of the lane lengths), in entry order.  No padding is stored.
+template <int BS, int CH>
+__global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
+                                                     const long long *__restrict__ seg_start,
+                                                     const int *__restrict__ grp_base,
+                                                     const long long *__restrict__ ent_base,
+                                                     const uint8_t *__restrict__ seg_len,
+                                                     const uint8_t *__restrict__ ent_vox,
+                                                     const double *__restrict__ ent_val, const double *__restrict__ x,
+                                                     const Scal *__restrict__ sc, double *__restrict__ segv) {
+    constexpr int G = 64;
+    constexpr int NWV = BS / G;
+    static_assert(G == kWave, "one segment group per wavefront");
+    static_assert(CH % G == 0, "whole wavefront copy steps");
+    __shared__ double xl[128];
+    __shared__ double lv[NWV][CH];
+    __shared__ uint8_t lx[NWV][CH];
+    if (sc && sc->done) return;
+    const int it = blockIdx.x;
+    const int g0 = grp_base[it], g1 = grp_base[it + 1];
+    if (g0 == g1) return;  // the same for the whole workgroup
+    const int t = it / S;
+    const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
+    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x % G;
+    const int wv = threadIdx.x / G;
+    double *bv = lv[wv];
+    uint8_t *bx = lx[wv];
+    const long long sb = seg_start[it];
+    for (int g = g0 + wv; g < g1; g += NWV) {
+        const int len = seg_len[(size_t)g * G + lane];
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int up = __shfl_up(incl, o);
+            if (lane >= o) incl += up;
+        }
+        const int tot = __shfl(incl, G - 1);  // entries of the group
+        const int beg = incl - len, end = incl;
+        const long long gb = ent_base[g];
+        double acc = 0.0;
+        for (int q0 = 0; q0 < tot; q0 += CH) {
+            const int cnt = tot - q0 < CH ? tot - q0 : CH;
+#pragma unroll 4
+            for (int i = lane; i < cnt; i += G) {
+                bv[i] = ent_val[gb + q0 + i];
+                bx[i] = ent_vox[gb + q0 + i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int e0 = beg > q0 ? beg : q0;
+            const int e1 = end < q0 + cnt ? end : q0 + cnt;
+            for (int e = e0; e < e1; ++e) acc += bv[e - q0] * xl[bx[e - q0]];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = acc;
+    }
+}
+
+// Pass 2: y = (y / beta) * (-alpha) + (the ray's segment partials added in
+// tile order, LC lanes per ray with strided partial sums), the damping rows
+// as in k_rows, and the ||y||^2 partials.  Workgroup b takes the rays of row
+// slice b % S (bounds[s] .. bounds[s+1]), the slice whose items pass 1 ran
+// on the same XCD (item t*S + s is workgroup t*S + s there), so the partials
+// it reads were written through, and stay in, that XCD's L2.  The grid is a
+// multiple of S.
+template <int LC, int BS>
+__global__ __launch_bounds__(BS) void k_rows_tiled_combine(int m_k, int m, int S, const int *__restrict__ bounds,
+                                                           const int *__restrict__ ray_ptr,
+                                                           const int *__restrict__ ray_seg,
+                                                           const double *__restrict__ segv,
+                                                           const int *__restrict__ rp, const int *__restrict__ ci,
+                                                           const double *__restrict__ av,
+                                                           const double *__restrict__ x, double *__restrict__ y,
+                                                           const Scal *__restrict__ sc, double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    double s1 = 1.0, s2 = 1.0;
+    const bool scaled = sc != nullptr;
+    if (scaled) {
+        if (sc->done) return;
+        s1 = sc->ib;
+        s2 = -sc->alpha;
+    }
+    constexpr int R = BS / LC;
+    const int lane = threadIdx.x % LC;
+    const int slot = threadIdx.x / LC;
+    double ssq_k = 0.0, ssq_d = 0.0;
+    const int q = blockIdx.x % S;
+    const int r0 = bounds[q], r1 = bounds[q + 1];
+    const int groups = (r1 - r0 + R - 1) / R;
+    for (int g = blockIdx.x / S; g < groups; g += gridDim.x / S) {
+        const int r = r0 + g * R + slot;
+        double acc = 0.0;
+        if (r < r1) {
+            const int b = ray_ptr[r], e = ray_ptr[r + 1];
+            int k = b + lane;
+            for (; k + LC < e; k += 2 * LC) {
+                const int i0 = ray_seg[k], i1 = ray_seg[k + LC];
+                const double p0 = segv[i0], p1 = segv[i1];
+                acc += p0;
+                acc += p1;
+            }
+            if (k < e) acc += segv[ray_seg[k]];
+        }
+        acc = group_sum<LC>(acc);
+        if (lane == 0 && r < r1) {
+            double yn = y[r];
+            if (scaled) yn = (yn * s1) * s2;
+            yn = yn + acc;
+            y[r] = yn;
+            ssq_k += yn * yn;
+        }
+    }
     for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
          r += (long long)gridDim.x * BS) {
         double acc = 0.0;
@@ -256,7 +455,26 @@ __device__ __forceinline__ void cols_tiled_item(
         // slot range per round, so every round is a plain unrolled loop
         for (int r0 = 0; r0 < ns; r0 += CAP) {
             const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
-            for (int i = threadIdx.x; i < cnt; i += BS) g[i] = u[seg_ray[s0 + r0 + i]] * ib;
+            {
+                constexpr int PER = CAP / BS;
+                int rr[PER];
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const int i = threadIdx.x + q * BS;
+                    rr[q] = i < cnt ? ld_stream<NT>(seg_ray + s0 + r0 + i) : 0;
+                }
+                double gv[PER];
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const int i = threadIdx.x + q * BS;
+                    gv[q] = i < cnt ? u[rr[q]] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const int i = threadIdx.x + q * BS;
+                    if (i < cnt) g[i] = gv[q] * ib;
+                }
+            }
             __syncthreads();
             const int hi = r0 + cnt;
             int jend = len;
@@ -265,8 +483,10 @@ __device__ __forceinline__ void cols_tiled_item(
             const uint16_t *ss = sseg + base + v;
             for (; j + 3 * NW < jend; j += 4 * NW) {
                 const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
-                const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
-                const double a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                const int g0 = ld_stream<NT>(ss + o0) - r0, g1 = ld_stream<NT>(ss + o1) - r0,
+                          g2 = ld_stream<NT>(ss + o2) - r0, g3 = ld_stream<NT>(ss + o3) - r0;
+                const double a0 = ld_stream<NT>(sv + o0), a1 = ld_stream<NT>(sv + o1), a2 = ld_stream<NT>(sv + o2),
+                             a3 = ld_stream<NT>(sv + o3);
                 acc += a0 * g[g0];
                 acc += a1 * g[g1];
                 acc += a2 * g[g2];
@@ -274,7 +494,7 @@ __device__ __forceinline__ void cols_tiled_item(
             }
             for (; j < jend; j += NW) {
                 const long long o = (long long)j * TW;
-                acc += sv[o] * g[ss[o] - r0];
+                acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o) - r0];
             }
             __syncthreads();
         }
