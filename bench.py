@@ -104,14 +104,16 @@ def aprod_bytes(m_k, m_d, n, nnz_k, nnz_d, s_v=8, s_x=8, s_y=8):
     return b1, b2
 
 
-def time_aprod(hip, op, mode, x_dev, y_dev, reps):
+def time_aprod(hip, op, mode, x_dev, y_dev, reps, f32=False):
+    """Average ms of one aprod launch (FP64, or the mixed solver's FP32 operator)."""
+    call = op.aprod_device_f32 if f32 else op.aprod_device
     stream = op.lib.plsqr_op_get_stream(op.h)
     for _ in range(3):
-        op.aprod_device(mode, x_dev, y_dev)
+        call(mode, x_dev, y_dev)
     e0, e1 = hip.event(), hip.event()
     hip.record(e0, stream)
     for _ in range(reps):
-        op.aprod_device(mode, x_dev, y_dev)
+        call(mode, x_dev, y_dev)
     hip.record(e1, stream)
     ms = hip.elapsed_ms(e0, e1) / reps
     hip.lib.hipEventDestroy(e0)
@@ -322,6 +324,10 @@ def main():
     hip.h2d(b_dev, np.ascontiguousarray(s.b))
     hip.h2d(x_dev, np.random.default_rng(0).standard_normal(s.n))
     hip.h2d(y_dev, np.random.default_rng(1).standard_normal(s.m_k + s.m_d) * 1e-3)
+    x32_dev = hip.malloc(4 * s.n)
+    y32_dev = hip.malloc(4 * (s.m_k + s.m_d))
+    hip.h2d(x32_dev, np.random.default_rng(0).standard_normal(s.n).astype(np.float32))
+    hip.h2d(y32_dev, (np.random.default_rng(1).standard_normal(s.m_k + s.m_d) * 1e-3).astype(np.float32))
 
     if a.aprod_only:
         # a plain device-to-device copy of 1 GiB and a streaming sum over
@@ -342,10 +348,18 @@ def main():
         ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
         ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
         b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
-        print(json.dumps({"aprod_only": True, "mode1_ms": ms1, "mode2_ms": ms2,
-                          "mode1_bytes": b1, "mode2_bytes": b2}), flush=True)
+        line = {"aprod_only": True, "mode1_ms": ms1, "mode2_ms": ms2, "mode1_bytes": b1, "mode2_bytes": b2}
+        if not a.exact_order:
+            f1 = time_aprod(hip, op, 1, x32_dev, y32_dev, a.aprod_reps, f32=True)
+            f2 = time_aprod(hip, op, 2, x32_dev, y32_dev, a.aprod_reps, f32=True)
+            fb1, fb2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d, 4, 4, 4)
+            line.update({"mode1_ms_f32": f1, "mode2_ms_f32": f2, "mode1_bytes_f32": fb1, "mode2_bytes_f32": fb2})
+        print(json.dumps(line), flush=True)
         hip.free(cp_src)
         hip.free(cp_dst)
+        for p in (b_dev, x_dev, y_dev, x32_dev, y32_dev):
+            hip.free(p)
+        op.close()
         return
     for _ in range(a.warmup):
         op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec, refine_every=a.refine_every)
@@ -365,24 +379,38 @@ def main():
           file=sys.stderr, flush=True)
 
     # aprod kernels, timed with events on the handle's stream
-    ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
-    ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
-    b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
+    if prec == P.PREC_MIXED:
+        # the mixed solve's inner iterations run the FP32 operator
+        ms1 = time_aprod(hip, op, 1, x32_dev, y32_dev, a.aprod_reps, f32=True)
+        ms2 = time_aprod(hip, op, 2, x32_dev, y32_dev, a.aprod_reps, f32=True)
+        b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d, 4, 4, 4)
+    else:
+        ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
+        ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
+        b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
     gbs1, gbs2 = b1 / ms1 / 1e6, b2 / ms2 / 1e6
     gbs = (b1 + b2) / (ms1 + ms2) / 1e6
     print(f"[bench] aprod mode1 {ms1:.3f} ms {gbs1:.0f} GB/s; mode2 {ms2:.3f} ms {gbs2:.0f} GB/s",
           file=sys.stderr, flush=True)
     dom_mode, dom_ms, dom_b = (2, ms2, b2) if ms2 >= ms1 else (1, ms1, b1)
+    mixed = prec == P.PREC_MIXED
     traffic = read_traffic(a.workload)
     tr_val = None
-    if traffic and str(dom_mode) in traffic.get("bytes_per_launch", {}):
-        tr_val = traffic["bytes_per_launch"][str(dom_mode)]
+    tr_key = "bytes_per_launch_f32" if mixed else "bytes_per_launch"
+    if traffic and not a.exact_order and str(dom_mode) in traffic.get(tr_key, {}):
+        tr_val = traffic[tr_key][str(dom_mode)]
+    sfx = "32" if mixed else ""
+    if a.exact_order:
+        kname = "k_rows_exact: reference row order" if dom_mode == 1 else "k_cols_exact: reference column order"
+    elif dom_mode == 1:
+        kname = (f"k_rows_tiled_c{sfx} + k_rows_tiled_combine{sfx}: y += A x as per-brick ray segments "
+                 "then per-ray sums")
+    else:
+        kname = f"k_cols_tiled{sfx} + k_cols_combine{sfx}: x += A^T y on voxel bricks"
     roofline = {"bound": "hbm", "achieved": round(dom_b / dom_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(dom_b / dom_ms / 1e6 / HBM_PEAK_GBS, 4),
                 "traffic": tr_val,
-                "kernel": f"aprod mode {dom_mode} ("
-                          + ("k_rows: y += A x on the stacked CSR" if dom_mode == 1 else
-                             "k_cols_tiled + k_cols_combine: x += A^T y on voxel bricks") + ")",
+                "kernel": f"aprod mode {dom_mode} ({kname})",
                 "algorithmic_bytes": dom_b, "avg_ms": round(dom_ms, 4)}
     cpu = None
     if not a.no_cpu:
@@ -406,7 +434,7 @@ def main():
     }
     print(json.dumps(out), flush=True)
     op.close()
-    for p in (b_dev, x_dev, y_dev):
+    for p in (b_dev, x_dev, y_dev, x32_dev, y32_dev):
         hip.free(p)
 
 

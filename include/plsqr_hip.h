@@ -223,6 +223,16 @@ int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_
  * handle's stream; returns without synchronising. */
 int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev);
 
+/* The same operator in the mixed-precision solver's arithmetic: FP32 matrix
+ * values and vectors, every row / column dot product accumulated with the
+ * compensated Dot2 scheme (aprod.c:24-84 in FP32; no counterpart in the
+ * reference, which is FP64 only).  mode 1: y += A x; mode 2: x += A^T y; x: n
+ * floats, y: m_k+m_d floats, caller order.  The _device form takes device
+ * pointers and returns without synchronising; the host form copies in and
+ * out.  PLSQR_ERR_STATE on an exact-order handle. */
+int plsqr_aprod_f32(plsqr_op *op, int mode, float *x, float *y);
+int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident LSQR                                                    */
 /* ---------------------------------------------------------------------- */

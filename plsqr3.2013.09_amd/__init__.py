@@ -101,6 +101,17 @@ class Operator:
         _check(self.lib.plsqr_aprod_device(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
                self.h)
 
+    def aprod_f32(self, mode, x, y):
+        """The mixed-precision solver's operator: FP32 values and vectors with
+        compensated (Dot2) sums; in place on host float32 arrays."""
+        assert x.dtype == np.float32 and y.dtype == np.float32 and x.size == self.n and y.size == self.m
+        _check(self.lib.plsqr_aprod_f32(self.h, int(mode), fptr(x), fptr(y)), self.h)
+
+    def aprod_device_f32(self, mode, x_ptr, y_ptr):
+        """FP32 device pointers (ints), enqueued on the handle's stream."""
+        _check(self.lib.plsqr_aprod_device_f32(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
+               self.h)
+
     def set_vectors(self, u=None, v=None):
         _check(self.lib.plsqr_op_set_vectors(self.h, dptr(u) if u is not None else None,
                                              dptr(v) if v is not None else None), self.h)

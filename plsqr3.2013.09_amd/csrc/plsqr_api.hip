@@ -69,14 +69,17 @@ struct plsqr_op {
     bool nt_loads = false;
     bool rows_nt = false;  // streaming loads for the mode-1 matrix data
     // mode 1 on the bricked layout (segment partials + per-ray combine)
-    bool rtiled = false, rt_contig = false;
+    bool rtiled = false;
+    long long rt_nent = 0;
     int rt_lc = 4;
     int *rt_gbase = nullptr, *rt_rptr = nullptr, *rt_rseg = nullptr;
     long long *rt_ebase = nullptr;
     uint8_t *rt_len = nullptr, *rt_vox = nullptr;
     double *rt_val = nullptr, *rt_segv = nullptr;
     // FP32 copies for mixed-precision solves (allocated on first use)
-    float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr;
+    float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr, *rt_val32 = nullptr;
+    float *fx = nullptr, *fy = nullptr;  // caller-order FP32 staging of the host FP32 aprod
+    plsqr::Scal *sc_plain = nullptr;     // ib = 1, alpha = -1, vscale = 1: plain y += A x, x += A^T y
     double *dx = nullptr;
     // warm starts: x0 in the device column order and r0 = [b;0] - A x0
     double *x0i = nullptr, *r0 = nullptr;
@@ -159,14 +162,9 @@ int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double
     using namespace plsqr;
     dim3 g(op->G1), b(BS);
     if (op->rtiled) {
-        if (op->rt_contig)
-            hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
-                               op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
-                               op->rt_segv);
-        else
-            hipLaunchKernelGGL((k_rows_tiled<BS>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
-                               op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
-                               op->rt_segv);
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
+                           op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
+                           op->rt_segv);
         LAUNCH_CHECK(op);
         switch (op->rt_lc) {
             case 1: hipLaunchKernelGGL((k_rows_tiled_combine<1, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
@@ -202,6 +200,25 @@ int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double
 
 constexpr int TILE_W = 128;
 constexpr int TILE_CAP = 2048;
+
+// r = [b;0] - A x in the device order (x: n, b: m_k kernel rows, r: m)
+int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
+    using namespace plsqr;
+    if (op->rtiled) {
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->tS,
+                           op->t_start, op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x,
+                           (const Scal *)nullptr, op->rt_segv);
+        hipLaunchKernelGGL((k_rows_tiled_residual<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
+                           op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, b, r);
+    } else {
+        hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+                           op->m, op->rp, op->ci, op->rv, x, b, r);
+    }
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+
 
 int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
     using namespace plsqr;
@@ -562,7 +579,7 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         }
         if (R) {
             op->rtiled = true;
-            op->rt_contig = R->contiguous;
+            op->rt_nent = R->n_ent;
             if ((rc = dalloc(op, &op->bounds, (size_t)L->S + 1))) return rc;
             HIP_TRY(op, hipMemcpy(op->bounds, L->slice_bounds.data(), sizeof(int) * (L->S + 1), hipMemcpyHostToDevice));
             const long long items2 = (long long)L->T * L->S;
@@ -775,10 +792,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         }
         if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
             R.reset(new plsqr::RowTiles());
-            const int win = std::max(0, env_int("PLSQR_SEGWIN", 0)) / plsqr::RowTiles::G * plsqr::RowTiles::G;
-            const bool contig = env_int("PLSQR_ROWLAYOUT", 1) != 0;
-            if (!(contig ? plsqr::build_row_tiles_contiguous(*L, sys->m_k, R.get(), &why)
-                         : plsqr::build_row_tiles(*L, sys->m_k, win, R.get(), &why))) {
+            if (!plsqr::build_row_tiles(*L, sys->m_k, R.get(), &why)) {
                 R.reset();  // mode 1 stays on the CSR kernel
             } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::row_tiles_self_check(sys, *L, *R, &why)) {
                 return fail(set_err(op, PLSQR_ERR_STATE, why));
@@ -922,6 +936,111 @@ int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
     if (!x_dev || !y_dev) return set_err(op, PLSQR_ERR_ARG, "device pointers required");
     if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
     return aprod_ext_dev(op, mode, x_dev, y_dev);
+}
+
+static int ensure_fp32(plsqr_op *op);
+
+// FP32 aprod on internal-order u32 / v32 with the plain Scal
+static int aprod32_internal(plsqr_op *op, int mode) {
+    using namespace plsqr;
+    const Scal *sp = op->sc_plain;
+    if (mode == 1) {
+        if (op->rtiled) {
+            hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->tS,
+                               op->t_start, op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox,
+                               op->rt_val32, op->v32, sp, op->rt_segv);
+            hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
+                               op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv32,
+                               op->v32, op->u32, sp, op->part);
+        } else {
+            hipLaunchKernelGGL((k_rows32<16, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m, op->rp,
+                               op->ci, op->rv32, op->v32, op->u32, sp, op->part);
+        }
+    } else {
+        hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
+                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32,
+                           op->t_sseg, op->t_cut, op->t_vcut, op->u32, sp, op->P);
+        hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
+                           op->P, op->dcp, op->dri, op->dcv, op->u32, sp, op->v32, op->part);
+    }
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+static int ensure_aprod32(plsqr_op *op) {
+    int rc;
+    if ((rc = ensure_fp32(op))) return rc;
+    if (op->sc_plain) return PLSQR_OK;
+    if ((rc = dalloc(op, &op->fx, op->n)) || (rc = dalloc(op, &op->fy, op->m)) || (rc = dalloc(op, &op->sc_plain, 1)))
+        return rc;
+    plsqr::Scal z;
+    memset(&z, 0, sizeof(z));
+    z.ib = 1.0;
+    z.alpha = -1.0;
+    z.vscale = 1.0;
+    z.beta_pos = 1;
+    HIP_TRY(op, hipMemcpy(op->sc_plain, &z, sizeof(z), hipMemcpyHostToDevice));
+    return PLSQR_OK;
+}
+
+int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
+    using namespace plsqr;
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!x_dev || !y_dev) return set_err(op, PLSQR_ERR_ARG, "device pointers required");
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    int rc;
+    if ((rc = ensure_aprod32(op))) return rc;
+    const int gn = grid_for(op->n, 256, op->cu * 4), gm = grid_for(op->m_k, 256, op->cu * 4);
+    // caller order -> device order
+    if (op->permuted)
+        hipLaunchKernelGGL(k_gather32, dim3(gn), dim3(256), 0, op->stream, op->n, op->inv, x_dev, op->v32);
+    else
+        HIP_TRY(op, hipMemcpyAsync(op->v32, x_dev, sizeof(float) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    if (op->rowperm) {
+        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m_k, op->rperm, y_dev, op->u32);
+        if (op->m_d > 0)
+            HIP_TRY(op, hipMemcpyAsync(op->u32 + op->m_k, y_dev + op->m_k, sizeof(float) * op->m_d,
+                                       hipMemcpyDeviceToDevice, op->stream));
+    } else {
+        HIP_TRY(op, hipMemcpyAsync(op->u32, y_dev, sizeof(float) * op->m, hipMemcpyDeviceToDevice, op->stream));
+    }
+    LAUNCH_CHECK(op);
+    if ((rc = aprod32_internal(op, mode))) return rc;
+    // back to the caller's order: only the output moves
+    if (mode == 2) {
+        if (op->permuted)
+            hipLaunchKernelGGL(k_gather32, dim3(gn), dim3(256), 0, op->stream, op->n, op->perm, op->v32, x_dev);
+        else
+            HIP_TRY(op, hipMemcpyAsync(x_dev, op->v32, sizeof(float) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    } else if (op->rowperm) {
+        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m_k, op->rinv, op->u32, y_dev);
+        if (op->m_d > 0)
+            HIP_TRY(op, hipMemcpyAsync(y_dev + op->m_k, op->u32 + op->m_k, sizeof(float) * op->m_d,
+                                       hipMemcpyDeviceToDevice, op->stream));
+    } else {
+        HIP_TRY(op, hipMemcpyAsync(y_dev, op->u32, sizeof(float) * op->m, hipMemcpyDeviceToDevice, op->stream));
+    }
+    LAUNCH_CHECK(op);
+    return PLSQR_OK;
+}
+
+int plsqr_aprod_f32(plsqr_op *op, int mode, float *x, float *y) {
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!x || !y) return set_err(op, PLSQR_ERR_ARG, "host vectors required");
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    int rc;
+    if ((rc = ensure_aprod32(op))) return rc;
+    HIP_TRY(op, hipMemcpyAsync(op->fx, x, sizeof(float) * op->n, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(op->fy, y, sizeof(float) * op->m, hipMemcpyHostToDevice, op->stream));
+    if ((rc = plsqr_aprod_device_f32(op, mode, op->fx, op->fy))) return rc;
+    if (mode == 1)
+        HIP_TRY(op, hipMemcpyAsync(y, op->fy, sizeof(float) * op->m, hipMemcpyDeviceToHost, op->stream));
+    else
+        HIP_TRY(op, hipMemcpyAsync(x, op->fx, sizeof(float) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    return PLSQR_OK;
 }
 
 int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_d) {
@@ -1201,6 +1320,12 @@ static int ensure_fp32(plsqr_op *op) {
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(op->t_nslot, 256, op->cu * 16)), dim3(256), 0, op->stream,
                        op->t_nslot, op->t_sval, op->t_sval32);
     LAUNCH_CHECK(op);
+    if (op->rtiled) {
+        if ((rc = dalloc(op, &op->rt_val32, op->rt_nent))) return rc;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(op->rt_nent, 256, op->cu * 16)), dim3(256), 0, op->stream,
+                           op->rt_nent, op->rt_val, op->rt_val32);
+        LAUNCH_CHECK(op);
+    }
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     return PLSQR_OK;
 }
@@ -1208,7 +1333,14 @@ static int ensure_fp32(plsqr_op *op) {
 static int enqueue_iteration32(plsqr_op *op) {
     using namespace plsqr;
     dim3 g1(op->G1), b(BS);
-    switch (op->L1) {
+    if (op->rtiled) {
+        hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
+                           op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val32, op->v32,
+                           (const Scal *)op->sc, op->rt_segv);
+        hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds,
+                           op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv32, op->v32, op->u32,
+                           (const Scal *)op->sc, op->part);
+    } else switch (op->L1) {
         case 8: hipLaunchKernelGGL((k_rows32<8, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
         case 32: hipLaunchKernelGGL((k_rows32<32, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
         default: hipLaunchKernelGGL((k_rows32<16, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
@@ -1248,8 +1380,7 @@ static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, doubl
     const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
     const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
     const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k, op->m,
-                       op->rp, op->ci, op->rv, xt, b_dev, op->hy);
+    if ((rc = launch_residual(op, xt, b_dev, op->hy))) return rc;
     hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, (const double *)op->hy,
                        op->part);
     LAUNCH_CHECK(op);
@@ -1493,8 +1624,7 @@ int plsqr_lsqr_solve_x0_device(plsqr_op *op, const double *b_dev, const double *
         } else {
             HIP_TRY(op, hipMemcpyAsync(op->x0i, x0_dev, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
         }
-        hipLaunchKernelGGL(k_residual, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
-                           op->m, op->rp, op->ci, op->rv, (const double *)op->x0i, b_dev, op->r0);
+        if ((rc = launch_residual(op, op->x0i, b_dev, op->r0))) return rc;
         LAUNCH_CHECK(op);
         const int g = std::min(op->part_len, grid_for(op->m, BS, op->cu * 4));
         double rn2 = 0.0, bn2 = 0.0;
@@ -1569,9 +1699,10 @@ int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_hos
         int rcb = b_internal(op, op->bst, &bi);
         if (rcb) return rcb;
     }
-    const int g = grid_for(op->m, 256, op->cu * 4);
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv, xin,
-                       bi, op->hy);
+    {
+        int rcr = launch_residual(op, xin, bi, op->hy);
+        if (rcr) return rcr;
+    }
     LAUNCH_CHECK(op);
     std::vector<double> hp((size_t)op->part_len);
     auto host_sum = [&](int count) {

@@ -69,6 +69,131 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
     }
 }
 
+// Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
+// k_rows_tiled_c): FP32 entries and x tile, Dot2 sum per segment, the
+// partial stored as the FP64 value s + c of its two words.
+template <int BS, int CH>
+__global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
+                                                       const long long *__restrict__ seg_start,
+                                                       const int *__restrict__ grp_base,
+                                                       const long long *__restrict__ ent_base,
+                                                       const uint8_t *__restrict__ seg_len,
+                                                       const uint8_t *__restrict__ ent_vox,
+                                                       const float *__restrict__ ent_val, const float *__restrict__ x,
+                                                       const Scal *__restrict__ sc, double *__restrict__ segv) {
+    constexpr int G = 64;
+    constexpr int NWV = BS / G;
+    static_assert(G == kWave, "one segment group per wavefront");
+    static_assert(CH % G == 0, "whole wavefront copy steps");
+    __shared__ float xl[128];
+    __shared__ float lv[NWV][CH];
+    __shared__ uint8_t lx[NWV][CH];
+    if (sc->done) return;
+    const int it = blockIdx.x;
+    const int g0 = grp_base[it], g1 = grp_base[it + 1];
+    if (g0 == g1) return;  // the same for the whole workgroup
+    const int t = it / S;
+    const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
+    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x % G;
+    const int wv = threadIdx.x / G;
+    float *bv = lv[wv];
+    uint8_t *bx = lx[wv];
+    const long long sb = seg_start[it];
+    for (int g = g0 + wv; g < g1; g += NWV) {
+        const int len = seg_len[(size_t)g * G + lane];
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int up = __shfl_up(incl, o);
+            if (lane >= o) incl += up;
+        }
+        const int tot = __shfl(incl, G - 1);
+        const int beg = incl - len, end = incl;
+        const long long gb = ent_base[g];
+        F2 acc{0.f, 0.f};
+        for (int q0 = 0; q0 < tot; q0 += CH) {
+            const int cnt = tot - q0 < CH ? tot - q0 : CH;
+#pragma unroll 4
+            for (int i = lane; i < cnt; i += G) {
+                bv[i] = ent_val[gb + q0 + i];
+                bx[i] = ent_vox[gb + q0 + i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int e0 = beg > q0 ? beg : q0;
+            const int e1 = end < q0 + cnt ? end : q0 + cnt;
+            for (int e = e0; e < e1; ++e) dot2_add(acc, bv[e - q0], xl[bx[e - q0]]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (double)acc.s + (double)acc.c;
+    }
+}
+
+// Pass 2 in FP32: y = fl32(fl32(y*s1)*s2) + fl32(the ray's FP64 partials
+// added in tile order); damping rows and the FP64 ||y||^2 partials as in
+// k_rows32.  Workgroup b takes the rays of row slice b % S, as in
+// k_rows_tiled_combine.
+template <int LC, int BS>
+__global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int S, const int *__restrict__ bounds,
+                                                             const int *__restrict__ ray_ptr,
+                                                             const int *__restrict__ ray_seg,
+                                                             const double *__restrict__ segv,
+                                                             const int *__restrict__ rp, const int *__restrict__ ci,
+                                                             const float *__restrict__ av,
+                                                             const float *__restrict__ x, float *__restrict__ y,
+                                                             const Scal *__restrict__ sc, double *__restrict__ part) {
+    __shared__ double sh[BS / kWave];
+    if (sc->done) return;
+    const float s1 = (float)sc->ib, s2 = (float)(-sc->alpha);
+    constexpr int R = BS / LC;
+    const int lane = threadIdx.x % LC;
+    const int slot = threadIdx.x / LC;
+    double ssq_k = 0.0, ssq_d = 0.0;
+    const int q = blockIdx.x % S;
+    const int r0 = bounds[q], r1 = bounds[q + 1];
+    const int groups = (r1 - r0 + R - 1) / R;
+    for (int g = blockIdx.x / S; g < groups; g += gridDim.x / S) {
+        const int r = r0 + g * R + slot;
+        double acc = 0.0;
+        if (r < r1) {
+            const int b = ray_ptr[r], e = ray_ptr[r + 1];
+            int k = b + lane;
+            for (; k + LC < e; k += 2 * LC) {
+                const int i0 = ray_seg[k], i1 = ray_seg[k + LC];
+                const double p0 = segv[i0], p1 = segv[i1];
+                acc += p0;
+                acc += p1;
+            }
+            if (k < e) acc += segv[ray_seg[k]];
+        }
+        acc = group_sum<LC>(acc);
+        if (lane == 0 && r < r1) {
+            const float yn = (y[r] * s1) * s2 + (float)acc;
+            y[r] = yn;
+            ssq_k += (double)yn * (double)yn;
+        }
+    }
+    for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
+         r += (long long)gridDim.x * BS) {
+        F2 acc{0.f, 0.f};
+        for (int k = rp[r]; k < rp[r + 1]; ++k) dot2_add(acc, av[k], x[ci[k]]);
+        const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+        y[r] = yn;
+        ssq_d += (double)yn * (double)yn;
+    }
+    const double a = block_sum<BS>(ssq_k, sh);
+    const double d = block_sum<BS>(ssq_d, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = d;
+    }
+}
+
 // Mode 2 in FP32 on the tiled layout (same item structure as k_cols_tiled).
 template <int TW, int BS, int CAP>
 __global__ __launch_bounds__(BS) void k_cols_tiled32(

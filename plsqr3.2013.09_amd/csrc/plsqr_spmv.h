@@ -109,66 +109,14 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 }
 
 // Mode 1 on the bricked layout, pass 1: one workgroup per (tile, row slice)
-// item stages the tile of x (at most 128 values) in LDS; each wavefront
-// takes groups of 64 segments, one per lane, and sums its segment's entries
-// (ascending voxel order) against the staged values.  The lanes of a group
-// hold the item's segments in descending length, so lane 0 has the longest
-// and the step loop runs to its length.  The partial goes to
-// segv[seg_start[item] + position of the segment in that order].
-template <int BS>
-__global__ __launch_bounds__(BS) void k_rows_tiled(int S, const int *__restrict__ tile_start,
-                                                   const long long *__restrict__ seg_start,
-                                                   const int *__restrict__ grp_base,
-                                                   const long long *__restrict__ ent_base,
-                                                   const uint8_t *__restrict__ seg_len,
-                                                   const uint8_t *__restrict__ ent_vox,
-                                                   const double *__restrict__ ent_val, const double *__restrict__ x,
-                                                   const Scal *__restrict__ sc, double *__restrict__ segv) {
-    constexpr int G = 64;
-    __shared__ double xl[128];
-    if (sc && sc->done) return;
-    const int it = blockIdx.x;
-    const int g0 = grp_base[it], g1 = grp_base[it + 1];
-    if (g0 == g1) return;  // the same for the whole workgroup
-    const int t = it / S;
-    const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
-    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
-    __syncthreads();
-    const int lane = threadIdx.x % G;
-    const long long sb = seg_start[it];
-    for (int g = g0 + (int)threadIdx.x / G; g < g1; g += BS / G) {
-        const int len = seg_len[(size_t)g * G + lane];
-        const int kmax = seg_len[(size_t)g * G];  // lane 0: the group's longest segment
-        const long long o = ent_base[g] + lane;
-        double acc = 0.0;
-        int k = 0;
-        for (; k + 3 < kmax; k += 4) {
-            const long long o0 = o + (long long)k * G;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-            if (k < len) { a0 = ent_val[o0]; v0 = ent_vox[o0]; }
-            if (k + 1 < len) { a1 = ent_val[o0 + G]; v1 = ent_vox[o0 + G]; }
-            if (k + 2 < len) { a2 = ent_val[o0 + 2 * G]; v2 = ent_vox[o0 + 2 * G]; }
-            if (k + 3 < len) { a3 = ent_val[o0 + 3 * G]; v3 = ent_vox[o0 + 3 * G]; }
-            if (k < len) acc += a0 * xl[v0];
-            if (k + 1 < len) acc += a1 * xl[v1];
-            if (k + 2 < len) acc += a2 * xl[v2];
-            if (k + 3 < len) acc += a3 * xl[v3];
-        }
-        for (; k < len; ++k) {
-            const long long ok = o + (long long)k * G;
-            acc += ent_val[ok] * xl[ent_vox[ok]];
-        }
-        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = acc;
-    }
-}
-
-// Pass 1 on the contiguous layout: the item's segments in ray order, each
-// segment's entries stored together.  A wavefront takes a group of 64
+// item stages the tile of x (at most 128 values) in LDS.  The item's
+// segments (one ray's entries in the tile) are in ray order, each segment's
+// entries stored together in ascending voxel order.  A wavefront takes a group of 64
 // segments (one per lane), copies the group's entries into its own LDS
 // buffer with coalesced loads, CH at a time, and every lane adds the entries
 // of its segment found in the buffer (its first entry from a wavefront
 // prefix sum of the lane lengths), in entry order.  No padding is stored.
+// The partial goes to segv[seg_start[item] + segment index].
 template <int BS, int CH>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
                                                      const long long *__restrict__ seg_start,
@@ -559,6 +507,11 @@ __global__ void k_gather(int n, const int *__restrict__ idx, const double *__res
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) dst[i] = src[idx[i]];
 }
 
+__global__ void k_gather32(int n, const int *__restrict__ idx, const float *__restrict__ src,
+                           float *__restrict__ dst) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) dst[i] = src[idx[i]];
+}
+
 // a[i] = map[a[i]]
 __global__ void k_remap(long long len, const int *__restrict__ map, int *__restrict__ a) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
@@ -897,6 +850,41 @@ __global__ void k_copy(long long len, const double *__restrict__ src, double *__
 }
 
 // r = [b;0] - A x on the stacked CSR (rows), FP64
+// r = [b;0] - A x with the kernel rows' A x from the per-brick segment
+// partials of k_rows_tiled_c (added in tile order, LC lanes per ray) and the
+// damping rows from the CSR; rows of slice b % S per workgroup as in
+// k_rows_tiled_combine.
+template <int LC, int BS>
+__global__ __launch_bounds__(BS) void k_rows_tiled_residual(int m_k, int m, int S, const int *__restrict__ bounds,
+                                                            const int *__restrict__ ray_ptr,
+                                                            const int *__restrict__ ray_seg,
+                                                            const double *__restrict__ segv,
+                                                            const int *__restrict__ rp, const int *__restrict__ ci,
+                                                            const double *__restrict__ av,
+                                                            const double *__restrict__ x,
+                                                            const double *__restrict__ b, double *__restrict__ r) {
+    constexpr int R = BS / LC;
+    const int lane = threadIdx.x % LC;
+    const int slot = threadIdx.x / LC;
+    const int q = blockIdx.x % S;
+    const int r0 = bounds[q], r1 = bounds[q + 1];
+    const int groups = (r1 - r0 + R - 1) / R;
+    for (int g = blockIdx.x / S; g < groups; g += gridDim.x / S) {
+        const int i = r0 + g * R + slot;
+        double acc = 0.0;
+        if (i < r1)
+            for (int k = ray_ptr[i] + lane; k < ray_ptr[i + 1]; k += LC) acc += segv[ray_seg[k]];
+        acc = group_sum<LC>(acc);
+        if (lane == 0 && i < r1) r[i] = b[i] - acc;
+    }
+    for (long long i = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; i < m;
+         i += (long long)gridDim.x * BS) {
+        double acc = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) acc += av[k] * x[ci[k]];
+        r[i] = 0.0 - acc;
+    }
+}
+
 __global__ void k_residual(int m_k, int m, const int *__restrict__ rp, const int *__restrict__ ci,
                            const double *__restrict__ av, const double *__restrict__ x,
                            const double *__restrict__ b, double *__restrict__ r) {

@@ -255,113 +255,9 @@ void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_star
     }
 }
 
-bool build_row_tiles(const TileLayout &L, int m_k, int window, RowTiles *R, std::string *err) {
+bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err) {
     const int TW = L.TW, G = RowTiles::G;
     const long long items = (long long)L.T * L.S;
-    // segment lengths per item, then the sorted order and the group counts
-    std::vector<std::vector<int>> order((size_t)items);
-    std::vector<int> ngrp((size_t)items, 0);
-    std::vector<long long> nent((size_t)items, 0);
-    std::atomic<bool> ok{true};
-    par_for(items, [&](long long ib, long long ie) {
-        std::vector<int> len;
-        for (long long it = ib; it < ie; ++it) {
-            const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
-            len.assign((size_t)ns, 0);
-            const long long base = L.slot_base[(size_t)it];
-            for (int v = 0; v < TW; ++v)
-                for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) ++len[L.sseg[(size_t)(base + (long long)j * TW + v)]];
-            std::vector<int> &ord = order[(size_t)it];
-            ord.resize((size_t)ns);
-            for (int i = 0; i < ns; ++i) ord[(size_t)i] = i;
-            // descending length inside windows of `window` segments (a multiple
-            // of G, so every group is sorted and its lane 0 is its longest)
-            const int W = window > 0 ? window : std::max(ns, 1);
-            for (int w0 = 0; w0 < ns; w0 += W)
-                std::stable_sort(ord.begin() + w0, ord.begin() + std::min(ns, w0 + W),
-                                 [&](int a, int b) { return len[(size_t)a] > len[(size_t)b]; });
-            long long e = 0;
-            for (int g = 0; g * G < ns; ++g) {
-                const int l0 = len[(size_t)ord[(size_t)g * G]];
-                if (l0 > 255) ok.store(false);
-                e += (long long)l0 * G;
-            }
-            ngrp[(size_t)it] = (ns + G - 1) / G;
-            nent[(size_t)it] = e;
-        }
-    });
-    if (!ok.load()) {
-        *err = "a segment holds more than 255 entries";
-        return false;
-    }
-    R->grp_base.assign((size_t)items + 1, 0);
-    long long gtot = 0;
-    for (long long it = 0; it < items; ++it) {
-        gtot += ngrp[(size_t)it];
-        if (gtot > INT_MAX / G) {
-            *err = "too many segment groups";
-            return false;
-        }
-        R->grp_base[(size_t)it + 1] = (int)gtot;
-    }
-    R->n_grp = (int)gtot;
-    R->ent_base.assign((size_t)gtot + 1, 0);
-    R->seg_len.assign((size_t)gtot * G, 0);
-    // entry offsets: per item, then per group inside it
-    std::vector<long long> item_ent((size_t)items + 1, 0);
-    for (long long it = 0; it < items; ++it) item_ent[(size_t)it + 1] = item_ent[(size_t)it] + nent[(size_t)it];
-    R->n_ent = item_ent[(size_t)items];
-    R->ent_vox.assign((size_t)R->n_ent, 0);
-    R->ent_val.assign((size_t)R->n_ent, 0.0);
-    std::vector<int> seg_pos((size_t)L.n_seg, 0);  // original segment -> position in the sorted order
-    par_for(items, [&](long long ib, long long ie) {
-        std::vector<int> len, rank, fill;
-        for (long long it = ib; it < ie; ++it) {
-            const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
-            const std::vector<int> &ord = order[(size_t)it];
-            rank.assign((size_t)ns, 0);
-            for (int p = 0; p < ns; ++p) rank[(size_t)ord[(size_t)p]] = p;
-            len.assign((size_t)ns, 0);
-            const long long base = L.slot_base[(size_t)it];
-            for (int v = 0; v < TW; ++v)
-                for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) ++len[L.sseg[(size_t)(base + (long long)j * TW + v)]];
-            const int g0 = R->grp_base[(size_t)it];
-            long long e = item_ent[(size_t)it];
-            for (int g = 0; g * G < ns; ++g) {
-                R->ent_base[(size_t)g0 + g] = e;
-                e += (long long)len[(size_t)ord[(size_t)g * G]] * G;
-                for (int l = 0; l < G && g * G + l < ns; ++l)
-                    R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)len[(size_t)ord[(size_t)(g * G + l)]];
-            }
-            // entries: voxels ascending, so each segment's list is in voxel order
-            fill.assign((size_t)ns, 0);
-            for (int v = 0; v < TW; ++v)
-                for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) {
-                    const long long slot = base + (long long)j * TW + v;
-                    const int sg = L.sseg[(size_t)slot];
-                    const int p = rank[(size_t)sg];
-                    const long long o = R->ent_base[(size_t)g0 + p / G] + (long long)fill[(size_t)sg]++ * G + p % G;
-                    R->ent_vox[(size_t)o] = (uint8_t)v;
-                    R->ent_val[(size_t)o] = L.sval[(size_t)slot];
-                }
-            for (int i = 0; i < ns; ++i) seg_pos[(size_t)(L.seg_start[(size_t)it] + i)] = (int)(L.seg_start[(size_t)it] + rank[(size_t)i]);
-        }
-    });
-    R->ent_base[(size_t)gtot] = R->n_ent;
-    // rays: their segments in item (= tile) order
-    R->ray_ptr.assign((size_t)m_k + 1, 0);
-    for (long long k = 0; k < L.n_seg; ++k) ++R->ray_ptr[(size_t)L.seg_ray[(size_t)k] + 1];
-    for (int r = 0; r < m_k; ++r) R->ray_ptr[(size_t)r + 1] += R->ray_ptr[(size_t)r];
-    R->ray_seg.assign((size_t)L.n_seg, 0);
-    std::vector<int> next(R->ray_ptr.begin(), R->ray_ptr.end() - 1);
-    for (long long k = 0; k < L.n_seg; ++k) R->ray_seg[(size_t)next[(size_t)L.seg_ray[(size_t)k]]++] = seg_pos[(size_t)k];
-    return true;
-}
-
-bool build_row_tiles_contiguous(const TileLayout &L, int m_k, RowTiles *R, std::string *err) {
-    const int TW = L.TW, G = RowTiles::G;
-    const long long items = (long long)L.T * L.S;
-    R->contiguous = true;
     R->grp_base.assign((size_t)items + 1, 0);
     long long gtot = 0;
     for (long long it = 0; it < items; ++it) {
@@ -444,13 +340,11 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
                 for (int l = 0; l < G; ++l) {
                     const int len = R.seg_len[(size_t)g * G + l];
                     if (len == 0) continue;
-                    long long lo = 0;  // contiguous: the lengths of the lanes before l
-                    if (R.contiguous)
-                        for (int q = 0; q < l; ++q) lo += R.seg_len[(size_t)g * G + q];
+                    long long lo = 0;  // the lengths of the lanes before l
+                    for (int q = 0; q < l; ++q) lo += R.seg_len[(size_t)g * G + q];
                     double acc = 0.0;
                     for (int k = 0; k < len; ++k) {
-                        const long long o = R.contiguous ? R.ent_base[(size_t)g] + lo + k
-                                                         : R.ent_base[(size_t)g] + (long long)k * G + l;
+                        const long long o = R.ent_base[(size_t)g] + lo + k;
                         acc += R.ent_val[(size_t)o] * x[c0 + R.ent_vox[(size_t)o]];
                     }
                     segv[(size_t)(L.seg_start[(size_t)it] + (long long)(g - R.grp_base[(size_t)it]) * G + l)] = acc;

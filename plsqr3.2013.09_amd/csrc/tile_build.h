@@ -69,16 +69,15 @@ struct TileLayout {
 
 // Mode-1 layout on the same (tile, row slice) items.  Inside an item every
 // segment (one ray's entries in the tile) is a list of (voxel, value) pairs
-// in ascending voxel order.  The item's segments (in ray order) are sorted by
-// descending length (stable) inside windows of `window` segments (a multiple
-// of 64; 0: the whole item) and dealt to groups of 64, one per lane: short
-// windows keep neighbouring rays' partials close together for the combine
-// pass, long ones pad less.  A group
-// stores its entries step-major, entry k of lane l at
-// ent_base[group] + k * 64 + l, padded to the group's longest segment.  The
-// kernel sums each segment against the tile of x staged in LDS and writes
-// the partial to segv[seg_start[item] + position in the sorted order]; the
-// combine pass adds a ray's partials in tile order through ray_ptr/ray_seg.
+// in ascending voxel order; the segments keep the item's ray order and are
+// dealt to groups of 64, one per lane (a wavefront), with each segment's
+// entries stored together: entry k of lane l sits at ent_base[group] + (the
+// lengths of lanes < l) + k.  No padding is stored.  The kernel sums each
+// segment against the tile of x staged in LDS and writes the partial to
+// segv[seg_start[item] + segment index]; the combine pass adds a ray's
+// partials in tile order through ray_ptr/ray_seg.  Keeping ray order makes
+// neighbouring rays' partials neighbours in segv, so the combine's reads
+// share cache lines.
 struct RowTiles {
     static constexpr int G = 64;       // segments per group (one wavefront)
     std::vector<int> grp_base;         // items+1: first group of each item
@@ -87,24 +86,15 @@ struct RowTiles {
     std::vector<uint8_t> ent_vox;      // entries: voxel inside the tile
     std::vector<double> ent_val;       // entries
     std::vector<int> ray_ptr;          // m_k+1
-    std::vector<int> ray_seg;          // n_seg: positions in segv, tile order per ray
+    std::vector<int> ray_seg;          // n_seg: segment indices, tile order per ray
     long long n_ent = 0;
     int n_grp = 0;
-    // contiguous: segments in ray order, each segment's entries stored
-    // together (entry k of lane l at ent_base[group] + (sum of the lengths of
-    // lanes < l) + k), no padding; the kernel forms the offsets with a
-    // wavefront prefix sum
-    bool contiguous = false;
 };
 
 // Builds the mode-1 layout from a mode-2 layout (same items and segments).
 // Returns false with *err set when a segment holds more than 255 entries or
 // the group count overflows.
-bool build_row_tiles(const TileLayout &L, int m_k, int window, RowTiles *out, std::string *err);
-
-// The contiguous variant (RowTiles::contiguous): segments keep the item's ray
-// order, so neighbouring rays' partials are neighbours in segv.
-bool build_row_tiles_contiguous(const TileLayout &L, int m_k, RowTiles *out, std::string *err);
+bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err);
 
 // Host evaluation of y = A_k x (kernel rows, new column numbering) through
 // the mode-1 layout, adding every ray's partials in tile order; the check

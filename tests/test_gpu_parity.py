@@ -51,6 +51,38 @@ def test_aprod_matches_oracle(P, O, MG, gpu_available, name, exact):
             assert np.array_equal(xg2, xg) and np.array_equal(yg2, yg)
 
 
+@pytest.mark.parametrize("name", SMALL)
+def test_aprod_f32_matches_oracle(P, O, MG, gpu_available, name):
+    """The mixed-precision operator (FP32 values and vectors, Dot2 sums)
+    against the FP64 oracle: FP32 rounding of the values and of the result
+    bounds the difference by a few 1e-8 relative; the bar is 1e-6."""
+    announce(f"aprod f32 {name}")
+    s = MG.make_system(P, MG.CASES[name])
+    orc = O.Oracle(s)
+    rng = np.random.default_rng(19)
+    x = rng.standard_normal(s.n).astype(np.float32)
+    y = rng.standard_normal(s.m_k + s.m_d).astype(np.float32)
+    with P.Operator(s) as op:
+        for mode in (1, 2):
+            xg, yg = x.copy(), y.copy()
+            op.aprod_f32(mode, xg, yg)
+            xo, yo = x.astype(np.float64), y.astype(np.float64)
+            orc.aprod(mode, xo, yo)
+            got, want = (yg, yo) if mode == 1 else (xg, xo)
+            assert rel(got.astype(np.float64), want) < 1e-6, (mode, rel(got.astype(np.float64), want))
+            if mode == 1:
+                assert np.array_equal(xg, x)
+            else:
+                assert np.array_equal(yg, y)
+            xg2, yg2 = x.copy(), y.copy()
+            op.aprod_f32(mode, xg2, yg2)
+            assert np.array_equal(xg2, xg) and np.array_equal(yg2, yg)
+    with P.Operator(s, P.OP_EXACT_ORDER) as op:
+        with pytest.raises(P.PlsqrError) as e:
+            op.aprod_f32(1, x.copy(), y.copy())
+        assert e.value.code == 6
+
+
 def test_aprod_entry_points_agree(P, MG, gpu_available):
     announce("aprod entry points")
     s = MG.make_system(P, MG.CASES["grid16_rays20k_itn1"])
