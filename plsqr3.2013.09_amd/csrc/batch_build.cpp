@@ -233,7 +233,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     }
     // offsets and pieces
     for (auto *v : {&plan->row0, &plan->col0, &plan->nnz0, &plan->dnnz0, &plan->slot0, &plan->seg0, &plan->cut0,
-                    &plan->pofs})
+                    &plan->pofs, &plan->grp0, &plan->ent0})
         v->assign((size_t)P + 1, 0);
     plan->item0.assign((size_t)P + 1, 0);
     plan->nslice.assign((size_t)P, 1);
@@ -255,6 +255,17 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         plan->slot0[(size_t)p + 1] = plan->slot0[(size_t)p] + L.tiles.n_slot;
         plan->seg0[(size_t)p + 1] = plan->seg0[(size_t)p] + L.tiles.n_seg;
         plan->cut0[(size_t)p + 1] = plan->cut0[(size_t)p] + L.tiles.cut_base.back();
+        {
+            // the mode-1 layout built from these tiles at upload: one group per
+            // 64 segments of an item, one entry per kernel nonzero
+            long long g = 0, e = 0;
+            const long long nit = (long long)L.tiles.T * L.tiles.S;
+            for (long long it = 0; it < nit; ++it)
+                g += (L.tiles.seg_start[(size_t)it + 1] - L.tiles.seg_start[(size_t)it] + RowTiles::G - 1) / RowTiles::G;
+            for (uint16_t l : L.tiles.vox_len) e += l;
+            plan->grp0[(size_t)p + 1] = plan->grp0[(size_t)p] + g;
+            plan->ent0[(size_t)p + 1] = plan->ent0[(size_t)p] + e;
+        }
         plan->item0[(size_t)p + 1] = plan->item0[(size_t)p] + L.tiles.T * L.S;
         plan->pofs[(size_t)p + 1] = plan->pofs[(size_t)p] + (long long)L.S * L.n;
         plan->nslice[(size_t)p] = L.S;
@@ -318,6 +329,8 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     plan->dnnz = plan->dnnz0[(size_t)P];
     plan->slots = plan->slot0[(size_t)P];
     plan->segs = plan->seg0[(size_t)P];
+    plan->groups = plan->grp0[(size_t)P];
+    plan->ents = plan->ent0[(size_t)P];
     plan->cuts = plan->cut0[(size_t)P];
     plan->items = plan->item0[(size_t)P];
     plan->partials = plan->pofs[(size_t)P];

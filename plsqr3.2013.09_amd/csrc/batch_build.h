@@ -44,6 +44,8 @@ struct BatchPlan {
     std::vector<ProblemLayout> prob;
     // offsets (P + 1 entries each)
     std::vector<long long> row0, col0, nnz0, dnnz0, slot0, seg0, cut0;
+    // mode-1 segment groups and entries (RowTiles of each problem's tiles)
+    std::vector<long long> grp0, ent0;
     std::vector<long long> pofs;         // start of each problem's slice partials (in columns)
     std::vector<int> item0, rch0, cch0;  // items = (tile, slice) pairs
     std::vector<int> nslice;             // row slices per problem
@@ -57,6 +59,7 @@ struct BatchPlan {
     std::vector<uint8_t> rch_kind;       // 0 kernel rows, 1 damping rows
     std::vector<int> cch_col, cch_prob;  // column chunks: cch_col has count + 1 entries
     long long rows = 0, cols = 0, nnz = 0, dnnz = 0, slots = 0, segs = 0, cuts = 0, partials = 0;
+    long long groups = 0, ents = 0;
     int items = 0;
     int max_seg_per_item = 0;
 };

@@ -75,7 +75,8 @@ struct plsqr_op {
     int *rt_gbase = nullptr, *rt_rptr = nullptr, *rt_rseg = nullptr;
     long long *rt_ebase = nullptr;
     uint8_t *rt_len = nullptr, *rt_vox = nullptr;
-    double *rt_val = nullptr, *rt_segv = nullptr;
+    double *rt_val = nullptr;
+    double *rt_segv = nullptr;  // segment partials: n_seg doubles, or n_seg floats on the FP32 path
     // FP32 copies for mixed-precision solves (allocated on first use)
     float *rv32 = nullptr, *t_sval32 = nullptr, *u32 = nullptr, *v32 = nullptr, *rt_val32 = nullptr;
     float *fx = nullptr, *fy = nullptr;  // caller-order FP32 staging of the host FP32 aprod
@@ -948,10 +949,10 @@ static int aprod32_internal(plsqr_op *op, int mode) {
         if (op->rtiled) {
             hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->tS,
                                op->t_start, op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox,
-                               op->rt_val32, op->v32, sp, op->rt_segv);
+                               op->rt_val32, op->v32, sp, (float *)op->rt_segv);
             hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
-                               op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv32,
-                               op->v32, op->u32, sp, op->part);
+                               op->tS, op->bounds, op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp,
+                               op->ci, op->rv32, op->v32, op->u32, sp, op->part);
         } else {
             hipLaunchKernelGGL((k_rows32<16, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m, op->rp,
                                op->ci, op->rv32, op->v32, op->u32, sp, op->part);
@@ -1336,10 +1337,10 @@ static int enqueue_iteration32(plsqr_op *op) {
     if (op->rtiled) {
         hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
                            op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val32, op->v32,
-                           (const Scal *)op->sc, op->rt_segv);
+                           (const Scal *)op->sc, (float *)op->rt_segv);
         hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds,
-                           op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv32, op->v32, op->u32,
-                           (const Scal *)op->sc, op->part);
+                           op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp, op->ci, op->rv32, op->v32,
+                           op->u32, (const Scal *)op->sc, op->part);
     } else switch (op->L1) {
         case 8: hipLaunchKernelGGL((k_rows32<8, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
         case 32: hipLaunchKernelGGL((k_rows32<32, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;

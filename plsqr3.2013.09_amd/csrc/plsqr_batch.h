@@ -239,6 +239,185 @@ __global__ __launch_bounds__(BS) void kb_rows(int nch, const int *__restrict__ c
 }
 
 // ---------------------------------------------------------------------------
+// Mode 1 on (tile, row slice) items, pass 1: workgroup b runs item
+// blk_item[b] (-1: none), the same queue as mode 2.  The brick's x values
+// (all K) are staged in LDS; a wavefront takes a group of 64 of the item's
+// ray segments (one per lane, entries stored together in voxel order),
+// copies the group's entries into its LDS buffer CH at a time, and each lane
+// sums its segment for the K right-hand sides.  segv[seg*K + k] = the
+// partial, stored in T (the FP32 path rounds its Dot2 sum s + c once).
+// ---------------------------------------------------------------------------
+template <class T, int K, int BS, int CH>
+__global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, const int *__restrict__ i_prob,
+                                             const int *__restrict__ i_c0, const int *__restrict__ i_c1,
+                                             const long long *__restrict__ seg_start,
+                                             const int *__restrict__ grp_base,
+                                             const long long *__restrict__ ent_base,
+                                             const uint8_t *__restrict__ seg_len,
+                                             const uint8_t *__restrict__ ent_vox, const T *__restrict__ ent_val,
+                                             const T *__restrict__ x, const Scal *__restrict__ sc,
+                                             T *__restrict__ segv) {
+    constexpr int G = 64;
+    constexpr int NWV = BS / G;
+    static_assert(G == kWave, "one segment group per wavefront");
+    static_assert(CH % G == 0, "whole wavefront copy steps");
+    __shared__ T xl[128 * K];
+    __shared__ T lv[NWV][CH];
+    __shared__ uint8_t lx[NWV][CH];
+    const int it = blk_item[blockIdx.x];
+    if (it < 0) return;
+    const int p = i_prob[it];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) any = any || !sc[p * K + k].done;
+    if (!any) return;  // the same for the whole workgroup
+    const int g0 = grp_base[it], g1 = grp_base[it + 1];
+    if (g0 == g1) return;
+    const int c0 = i_c0[it], tw = i_c1[it] - c0;
+    for (int i = threadIdx.x; i < tw * K; i += BS) xl[i] = x[(size_t)c0 * K + i];
+    __syncthreads();
+    const int lane = threadIdx.x % G;
+    const int wv = threadIdx.x / G;
+    T *bv = lv[wv];
+    uint8_t *bx = lx[wv];
+    const long long sb = seg_start[it];
+    for (int g = g0 + wv; g < g1; g += NWV) {
+        const int len = seg_len[(size_t)g * G + lane];
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int up = __shfl_up(incl, o);
+            if (lane >= o) incl += up;
+        }
+        const int tot = __shfl(incl, G - 1);
+        const int beg = incl - len, end = incl;
+        const long long gb = ent_base[g];
+        Acc<T> acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k].zero();
+        for (int q0 = 0; q0 < tot; q0 += CH) {
+            const int cnt = tot - q0 < CH ? tot - q0 : CH;
+            for (int i = lane; i < cnt; i += G) {
+                bv[i] = ent_val[gb + q0 + i];
+                bx[i] = ent_vox[gb + q0 + i];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int e0 = beg > q0 ? beg : q0;
+            const int e1 = end < q0 + cnt ? end : q0 + cnt;
+            for (int e = e0; e < e1; ++e) {
+                const T a = bv[e - q0];
+                const T *xv = xl + (int)bx[e - q0] * K;
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k].add(a, xv[k]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (len > 0) {
+            T *o = segv + (size_t)(sb + (long long)(g - g0) * G + lane) * K;
+#pragma unroll
+            for (int k = 0; k < K; ++k) o[k] = (T)acc[k].sum64();
+        }
+    }
+}
+
+// Pass 2 on row chunks: kernel rows y <- fl(fl(y*ib)*(-alpha)) + (the ray's
+// segment partials added in tile order in FP64, LC lanes per ray); damping
+// rows as kb_rows.  part[ch*K + k] as kb_rows.
+template <class T, int K, int LC, int BS>
+__global__ __launch_bounds__(BS) void kb_rows_comb(int nch, const int *__restrict__ ch_row,
+                                                   const int *__restrict__ ch_prob, const uint8_t *__restrict__ ch_kind,
+                                                   const int *__restrict__ ray_ptr, const int *__restrict__ ray_seg,
+                                                   const T *__restrict__ segv, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci, const T *__restrict__ av,
+                                                   const T *__restrict__ x, T *__restrict__ y,
+                                                   const Scal *__restrict__ sc, double *__restrict__ part) {
+    __shared__ double sh[K][BS / kWave];
+    constexpr int R = BS / LC;
+    const int lane = threadIdx.x % LC;
+    const int slot = threadIdx.x / LC;
+    for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const int p = ch_prob[ch];
+        T s1[K], s2[K];
+        bool live[K];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const Scal &s = sc[p * K + k];
+            live[k] = !s.done;
+            s1[k] = (T)s.ib;
+            s2[k] = (T)(-s.alpha);
+            any = any || live[k];
+        }
+        if (!any) continue;  // uniform across the block
+        const int r0 = ch_row[ch], r1 = ch_row[ch + 1];
+        double ssq[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) ssq[k] = 0.0;
+        if (ch_kind[ch] == 0) {
+            for (int rb = r0; rb < r1; rb += R) {
+                const int r = rb + slot;
+                double acc[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = 0.0;
+                if (r < r1) {
+                    const int e = ray_ptr[r + 1];
+                    for (int j = ray_ptr[r] + lane; j < e; j += LC) {
+                        const T *sv = segv + (size_t)ray_seg[j] * K;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) acc[k] += (double)sv[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = group_sum<LC>(acc[k]);
+                if (lane == 0 && r < r1) {
+                    VK<T, K> yv = ldk<T, K>(y, (size_t)r);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const T yn = (yv.a[k] * s1[k]) * s2[k] + (T)acc[k];
+                        if (live[k]) {
+                            yv.a[k] = yn;
+                            ssq[k] += (double)yn * (double)yn;
+                        }
+                    }
+                    stk<T, K>(y, (size_t)r, yv);
+                }
+            }
+        } else {
+            for (int r = r0 + threadIdx.x; r < r1; r += BS) {
+                Acc<T> acc[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k].zero();
+                for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                    const VK<T, K> xv = ldk<T, K>(x, (size_t)ci[j]);
+                    const T a = av[j];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) acc[k].add(a, xv.a[k]);
+                }
+                VK<T, K> yv = ldk<T, K>(y, (size_t)r);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const T yn = (yv.a[k] * s1[k]) * s2[k] + acc[k].value();
+                    if (live[k]) {
+                        yv.a[k] = yn;
+                        ssq[k] += (double)yn * (double)yn;
+                    }
+                }
+                stk<T, K>(y, (size_t)r, yv);
+            }
+        }
+        block_sum_k<BS, K>(ssq, sh);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) part[(size_t)ch * K + k] = ssq[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Mode 2 on (tile, row slice) items: P[(pbase + c)*K + k] = sum over the
 // slice's rows r of a(r, c) * u'[r][k] for the columns c of one brick, u' =
 // u*ib (the pending 1/beta).  Workgroup b runs item blk_item[b] (-1: none):

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "batch_build.h"
@@ -25,6 +26,7 @@ constexpr int ROW_CHUNK = 512;  // kernel rows per mode-1 piece
 constexpr int DAMP_CHUNK = 2048;
 constexpr int COL_CHUNK = 1024;
 constexpr int LDS_GATHER = 32768;  // bytes of LDS for the gathered ray values of a tile
+constexpr int SEG_CH = 512;        // mode-1 segment entries staged per wavefront copy
 
 struct Shape {
     int L = 16, bs1 = 256, U = 4, bs2 = 256;
@@ -62,6 +64,15 @@ struct plsqr_batch {
     uint16_t *vox_len = nullptr, *sseg = nullptr, *vox_cut = nullptr;
     double *sval = nullptr;
     float *sval32 = nullptr;
+    // mode 1 as per-brick ray segments (RowTiles of every problem) + per-ray sums
+    bool rtiled = false;
+    long long groups = 0, ents = 0;
+    int *rt_gbase = nullptr, *rt_rptr = nullptr, *rt_rseg = nullptr;
+    long long *rt_ebase = nullptr;
+    uint8_t *rt_len = nullptr, *rt_vox = nullptr;
+    double *rt_val = nullptr;
+    double *segv = nullptr;  // segment partials: segs*K doubles, or segs*K floats on the FP32 path
+    float *rt_val32 = nullptr;
     // pieces and ranges
     int *rch_row = nullptr, *rch_prob = nullptr, *cch_col = nullptr, *cch_prob = nullptr;
     uint8_t *rch_kind = nullptr;
@@ -164,6 +175,23 @@ ExtMap ext_map(const plsqr_batch *b) {
 template <class T, int K>
 int rows_k(plsqr_batch *b, const Shape &s, const T *av, const T *xv, T *yv, const Scal *sc) {
     const dim3 g(b->nrch), blk1(s.bs1);
+    if (b->rtiled) {
+        const T *ev = std::is_same<T, float>::value ? (const T *)b->rt_val32 : (const T *)b->rt_val;
+        hipLaunchKernelGGL((kb_seg<T, K, 256, SEG_CH>), dim3(b->nblk), dim3(256), 0, b->stream, b->blk_item,
+                           b->i_prob, b->i_c0, b->i_c1, b->seg_start, b->rt_gbase, b->rt_ebase, b->rt_len, b->rt_vox,
+                           ev, xv, sc, (T *)b->segv);
+        BT_LAUNCH(b);
+        if (s.bs1 == 128)
+            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 128>), g, dim3(128), 0, b->stream, b->nrch, b->rch_row,
+                               b->rch_prob, b->rch_kind, b->rt_rptr, b->rt_rseg, (const T *)b->segv, b->rp,
+                               b->ci, av, xv, yv, sc, b->part);
+        else
+            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 256>), g, dim3(256), 0, b->stream, b->nrch, b->rch_row,
+                               b->rch_prob, b->rch_kind, b->rt_rptr, b->rt_rseg, (const T *)b->segv, b->rp,
+                               b->ci, av, xv, yv, sc, b->part);
+        BT_LAUNCH(b);
+        return PLSQR_OK;
+    }
 #define ROWS_CASE(L_, BS_, U_)                                                                                  \
     if (s.L == L_ && s.bs1 == BS_ && s.U == U_) {                                                              \
         hipLaunchKernelGGL((kb_rows<T, K, L_, BS_, U_>), g, blk1, 0, b->stream, b->nrch, b->rch_row, b->rch_prob, \
@@ -342,6 +370,12 @@ int ensure_fp32(plsqr_batch *b) {
     hipLaunchKernelGGL((kb_convert<float>), dim3(grid_for(b->slots, 256, b->cu * 8)), dim3(256), 0, b->stream,
                        b->slots, b->sval, b->sval32);
     BT_LAUNCH(b);
+    if (b->rtiled) {
+        if ((rc = balloc(b, &b->rt_val32, (size_t)b->ents))) return rc;
+        hipLaunchKernelGGL((kb_convert<float>), dim3(grid_for(b->ents, 256, b->cu * 8)), dim3(256), 0, b->stream,
+                           b->ents, b->rt_val, b->rt_val32);
+        BT_LAUNCH(b);
+    }
     BT_TRY(b, hipStreamSynchronize(b->stream));
     return PLSQR_OK;
 }
@@ -824,6 +858,16 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->d_mk, (size_t)P)) || (rc = balloc(b, &b->d_mrow, (size_t)P)) ||
         (rc = balloc(b, &b->d_ncol, (size_t)P)))
         return rc;
+    b->rtiled = env_int("PLSQR_BATCH_ROWTILES", 1) != 0;
+    b->groups = plan.groups;
+    b->ents = plan.ents;
+    if (b->rtiled &&
+        ((rc = balloc(b, &b->rt_gbase, (size_t)b->items + 1)) || (rc = balloc(b, &b->rt_ebase, (size_t)b->groups + 1)) ||
+         (rc = balloc(b, &b->rt_len, (size_t)b->groups * plsqr::RowTiles::G)) ||
+         (rc = balloc(b, &b->rt_vox, (size_t)b->ents)) || (rc = balloc(b, &b->rt_val, (size_t)b->ents)) ||
+         (rc = balloc(b, &b->rt_rptr, (size_t)b->rows + 1)) || (rc = balloc(b, &b->rt_rseg, (size_t)b->segs)) ||
+         (rc = balloc(b, &b->segv, (size_t)b->segs * K))))
+        return rc;
     // per-problem pieces, shifted to global numbering on the host and
     // uploaded one problem at a time (the host copy is released after)
     for (int p = 0; p < P; ++p) {
@@ -873,6 +917,32 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
             (rc = upload(b, b->sval + plan.slot0[(size_t)p], T.sval.data(), T.sval.size())) ||
             (rc = upload(b, b->sseg + plan.slot0[(size_t)p], T.sseg.data(), T.sseg.size())))
             return rc;
+        if (b->rtiled) {
+            // mode-1 segment layout of this problem's tiles, shifted to global numbering
+            plsqr::RowTiles R;
+            std::string why;
+            if (!plsqr::build_row_tiles(T, L.m_k, &R, &why)) return set_err(b, PLSQR_ERR_FORMAT, why);
+            if ((long long)R.n_grp != plan.grp0[(size_t)p + 1] - plan.grp0[(size_t)p] ||
+                R.n_ent != plan.ent0[(size_t)p + 1] - plan.ent0[(size_t)p])
+                return set_err(b, PLSQR_ERR_STATE, "mode-1 segment layout disagrees with the plan");
+            std::vector<int> gb((size_t)nit), rpt((size_t)m), rsg(R.ray_seg.size());
+            std::vector<long long> eb((size_t)R.n_grp);
+            for (int i = 0; i < nit; ++i) gb[(size_t)i] = (int)(plan.grp0[(size_t)p] + R.grp_base[(size_t)i]);
+            for (int g = 0; g < R.n_grp; ++g) eb[(size_t)g] = plan.ent0[(size_t)p] + R.ent_base[(size_t)g];
+            for (int i = 0; i < m; ++i)
+                rpt[(size_t)i] = (int)(plan.seg0[(size_t)p] + R.ray_ptr[(size_t)std::min(i, L.m_k)]);
+            for (size_t j = 0; j < rsg.size(); ++j) rsg[j] = (int)(plan.seg0[(size_t)p] + R.ray_seg[j]);
+            if ((rc = upload(b, b->rt_gbase + t0, gb.data(), gb.size())) ||
+                (rc = upload(b, b->rt_ebase + plan.grp0[(size_t)p], eb.data(), eb.size())) ||
+                (rc = upload(b, b->rt_len + (size_t)plan.grp0[(size_t)p] * plsqr::RowTiles::G, R.seg_len.data(),
+                             R.seg_len.size())) ||
+                (rc = upload(b, b->rt_vox + plan.ent0[(size_t)p], R.ent_vox.data(), R.ent_vox.size())) ||
+                (rc = upload(b, b->rt_val + plan.ent0[(size_t)p], R.ent_val.data(), R.ent_val.size())) ||
+                (rc = upload(b, b->rt_rptr + r0, rpt.data(), rpt.size())) ||
+                (rc = upload(b, b->rt_rseg + plan.seg0[(size_t)p], rsg.data(), rsg.size())))
+                return rc;
+            BT_TRY(b, hipStreamSynchronize(b->stream));
+        }
         std::vector<int> rpb((size_t)m, p), cpb((size_t)L.n, p), rold((size_t)m);
         for (int i = 0; i < m; ++i) rold[(size_t)i] = i < L.m_k ? L.rperm[(size_t)i] : i;
         if ((rc = upload(b, b->row_prob + r0, rpb.data(), rpb.size())) ||
@@ -913,6 +983,13 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = upload(b, b->d_mrow, mrow.data(), mrow.size())) || (rc = upload(b, b->d_ncol, b->np.data(), b->np.size())) ||
         (rc = upload(b, b->d_nslice, nsl.data(), nsl.size())))
         return rc;
+    if (b->rtiled) {
+        const int groups_i = (int)b->groups, segs_i = (int)b->segs;
+        const long long ents_l = b->ents;
+        if ((rc = upload(b, b->rt_gbase + b->items, &groups_i, 1)) || (rc = upload(b, b->rt_ebase + b->groups, &ents_l, 1)) ||
+            (rc = upload(b, b->rt_rptr + rows_i, &segs_i, 1)))
+            return rc;
+    }
     BT_TRY(b, hipStreamSynchronize(b->stream));
     // vectors and scratch
     b->part_len = std::max(b->nrch, b->ncch) * K;

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
 
 // Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
 // k_rows_tiled_c): FP32 entries and x tile, Dot2 sum per segment, the
-// partial stored as the FP64 value s + c of its two words.
+// partial s + c rounded once to FP32.
 template <int BS, int CH>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
                                                        const long long *__restrict__ seg_start,
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
                                                        const uint8_t *__restrict__ seg_len,
                                                        const uint8_t *__restrict__ ent_vox,
                                                        const float *__restrict__ ent_val, const float *__restrict__ x,
-                                                       const Scal *__restrict__ sc, double *__restrict__ segv) {
+                                                       const Scal *__restrict__ sc, float *__restrict__ segv) {
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
@@ -130,19 +130,19 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (double)acc.s + (double)acc.c;
+        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (float)((double)acc.s + (double)acc.c);
     }
 }
 
-// Pass 2 in FP32: y = fl32(fl32(y*s1)*s2) + fl32(the ray's FP64 partials
-// added in tile order); damping rows and the FP64 ||y||^2 partials as in
+// Pass 2 in FP32: y = fl32(fl32(y*s1)*s2) + fl32(the ray's FP32 partials
+// added in tile order in FP64); damping rows and the FP64 ||y||^2 partials as in
 // k_rows32.  Workgroup b takes the rays of row slice b % S, as in
 // k_rows_tiled_combine.
 template <int LC, int BS>
 __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int S, const int *__restrict__ bounds,
                                                              const int *__restrict__ ray_ptr,
                                                              const int *__restrict__ ray_seg,
-                                                             const double *__restrict__ segv,
+                                                             const float *__restrict__ segv,
                                                              const int *__restrict__ rp, const int *__restrict__ ci,
                                                              const float *__restrict__ av,
                                                              const float *__restrict__ x, float *__restrict__ y,
