@@ -90,7 +90,6 @@ struct plsqr_op {
     int *rperm = nullptr, *rinv = nullptr;  // new -> old, old -> new (kernel rows)
     double *hy2 = nullptr;                  // m, new-order staging
     double *bint = nullptr;                 // m_k, b in the new order
-    int tiled_persistent = 0;  // > 0: workgroups of the persistent tiled kernel
     std::vector<void *> allocs;
     size_t bytes = 0;
     // launch shapes
@@ -224,12 +223,7 @@ int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
 int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
     using namespace plsqr;
     if (op->tiled) {
-        if (op->tiled_persistent) {
-            const int q = std::max(1, std::min(op->tT, op->tiled_persistent / op->tS));
-            hipLaunchKernelGGL((k_cols_tiled_persistent<TILE_W, BS, TILE_CAP, false>), dim3(q * op->tS), dim3(BS), 0,
-                               op->stream, op->n, op->tS, op->tT, op->t_start, op->t_slot, op->t_seg, op->t_vlen,
-                               op->t_ray, op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
-        } else if (op->nt_loads) {
+        if (op->nt_loads) {
             hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, true>), dim3(op->tT * op->tS), dim3(BS), 0,
                                op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
                                op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
@@ -827,7 +821,6 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     op->nt_loads = env_int("PLSQR_NT", 0) != 0;
     op->rows_nt = env_int("PLSQR_ROWS_NT", 0) != 0;
     op->rt_lc = env_int("PLSQR_LC", 4);
-    op->tiled_persistent = env_int("PLSQR_TILED_WG", 0);
     op->L1 = env_int("PLSQR_L1", 16);
     op->L2 = env_int("PLSQR_L2", 16);
     op->G1 = grid_for((long long)op->m_k * op->L1, BS, env_int("PLSQR_G1", op->cu * 8));

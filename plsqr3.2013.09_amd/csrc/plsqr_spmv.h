@@ -477,30 +477,6 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
                                      sval, sseg, cut_base, vox_cut, u, ib, P, g, part);
 }
 
-// Persistent form: gridDim.x = S * Q workgroups; workgroup b keeps slice
-// b % S (its XCD) and walks the tiles b / S, b / S + Q, ...
-template <int TW, int BS, int CAP, bool NT>
-__global__ __launch_bounds__(BS) void k_cols_tiled_persistent(
-    int n, int S, int T, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
-    const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
-    const int *__restrict__ seg_ray, const double *__restrict__ sval,
-    const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
-    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, const Scal *__restrict__ sc,
-    double *__restrict__ P) {
-    __shared__ double g[CAP];
-    __shared__ double part[BS / TW][TW];
-    double ib = 1.0;
-    if (sc) {
-        if (sc->done || !sc->beta_pos) return;
-        ib = sc->ib;
-    }
-    const int s = blockIdx.x % S;
-    const int Q = gridDim.x / S;
-    for (int t = blockIdx.x / S; t < T; t += Q)
-        cols_tiled_item<TW, BS, CAP, NT>(t * S + s, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
-                                         sval, sseg, cut_base, vox_cut, u, ib, P, g, part);
-}
-
 // dst[i] = src[idx[i]]  (column renumbering at the API boundary)
 __global__ void k_gather(int n, const int *__restrict__ idx, const double *__restrict__ src,
                          double *__restrict__ dst) {
