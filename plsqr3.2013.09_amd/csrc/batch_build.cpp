@@ -213,7 +213,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         std::vector<int> slices = global_sort ? std::vector<int>() : cut_slices(s);
         L.S = S;
         RaySorted rays;
-        sort_rays(s, perm, tiles, slices, &rays);
+        sort_rays(s, perm, tiles, slices, (std::getenv("PLSQR_RAYKEY") && std::atoi(std::getenv("PLSQR_RAYKEY")) == 0) ? nullptr : g, &rays);
         L.rperm = rays.rperm;
         const plsqr_system *ss = &rays.sys;
         if (global_sort) slices = cut_slices(ss);

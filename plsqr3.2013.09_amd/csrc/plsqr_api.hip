@@ -765,18 +765,12 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
             tiles.push_back(s->n);
         }
         if (env_int("PLSQR_VOXSORT", 1) != 0) plsqr::order_tile_columns(s, tiles, perm);
-        const int raysort = env_int("PLSQR_RAYSORT", 1);
-        if (raysort != 0) {
+        if (env_int("PLSQR_RAYSORT", 1) != 0) {
+            // within the row slices, so hb stays valid; PLSQR_RAYKEY=0 sorts by
+            // first and last tile even when the grid is known
             rays.reset(new plsqr::RaySorted());
-            if (raysort == 2) {
-                // one sort over all rays, slices cut afterwards on the sorted rows
-                plsqr::sort_rays(s, perm, tiles, std::vector<int>(), rays.get());
-                sys = &rays->sys;
-                hb = slice_bounds_host(sys, nsl);
-            } else {
-                plsqr::sort_rays(s, perm, tiles, hb, rays.get());  // within the row slices: hb stays valid
-                sys = &rays->sys;
-            }
+            plsqr::sort_rays(s, perm, tiles, hb, env_int("PLSQR_RAYKEY", 1) != 0 ? grid : nullptr, rays.get());
+            sys = &rays->sys;
         }
         L.reset(new plsqr::TileLayout());
         std::string why;

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <thread>
+#include <tuple>
 
 namespace plsqr {
 
@@ -358,29 +359,135 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
     }
 }
 
-void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
-               const std::vector<int> &slice_bounds, RaySorted *o) {
-    const int n = s->n, mk = s->m_k;
-    std::vector<int> tile_of((size_t)n, 0);
-    for (size_t t = 0; t + 1 < tile_start.size(); ++t)
-        for (int c = tile_start[t]; c < tile_start[t + 1]; ++c) tile_of[(size_t)c] = (int)t;
-    std::vector<int> lo((size_t)mk, INT_MAX), hi((size_t)mk, -1);
-    for (int c = 0; c < n; ++c) {
-        const int t = tile_of[(size_t)perm[(size_t)c]];
-        for (int e = s->k_col_ptr[c]; e < s->k_col_ptr[c + 1]; ++e) {
-            const int r = s->k_row_idx[e];
-            lo[(size_t)r] = std::min(lo[(size_t)r], t);
-            hi[(size_t)r] = std::max(hi[(size_t)r], t);
+namespace {
+
+// Morton code of six coordinates of `bits` bits each (bit i of coordinate j
+// at position 6 i + j)
+uint64_t morton6(const int *v, int bits) {
+    uint64_t r = 0;
+    for (int i = 0; i < bits; ++i)
+        for (int j = 0; j < 6; ++j) r |= (uint64_t)((v[j] >> i) & 1) << (6 * i + j);
+    return r;
+}
+
+// Per kernel row, a 6-D Morton key of its two end voxels on an nx*ny*nz grid
+// (columns in the loaded numbering, x fastest).  The ends are the voxels
+// with the smallest and the largest coordinate along the axis of the row's
+// longest extent (the mean of ties, rounded), ordered by (z, y, x) descending
+// so both directions of a ray give one key.  Rows sorted by it put rays with
+// nearby ends, which cross the same bricks, next to each other.
+std::vector<uint64_t> end_keys(const plsqr_system *s, int nx, int ny, int nz) {
+    const int mk = s->m_k, n = s->n;
+    // the columns of every kernel row, by a transpose with per-thread row
+    // counts over column chunks (no shared counters)
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (const char *e = std::getenv("PLSQR_HOST_THREADS")) nt = (unsigned)std::max(1, std::min(16, atoi(e)));
+    const long long chunk = ((long long)n + nt - 1) / nt;
+    std::vector<std::vector<long long>> off(nt, std::vector<long long>((size_t)mk, 0));
+    auto run = [&](auto &&body) {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) {
+            const long long cb = (long long)t * chunk, ce = std::min<long long>(n, cb + chunk);
+            th.emplace_back([&body, t, cb, ce] { body(t, cb, ce); });
         }
+        for (auto &x : th) x.join();
+    };
+    run([&](unsigned t, long long cb, long long ce) {
+        std::vector<long long> &o = off[t];
+        for (long long c = cb; c < ce; ++c)
+            for (int e = s->k_col_ptr[c]; e < s->k_col_ptr[c + 1]; ++e) ++o[(size_t)s->k_row_idx[e]];
+    });
+    std::vector<long long> rp((size_t)mk + 1, 0);
+    for (int r = 0; r < mk; ++r) {
+        long long at = rp[(size_t)r];
+        for (unsigned t = 0; t < nt; ++t) {
+            const long long k = off[t][(size_t)r];
+            off[t][(size_t)r] = at;
+            at += k;
+        }
+        rp[(size_t)r + 1] = at;
     }
+    std::vector<int> cols((size_t)rp[(size_t)mk]);
+    run([&](unsigned t, long long cb, long long ce) {
+        std::vector<long long> &o = off[t];
+        for (long long c = cb; c < ce; ++c)
+            for (int e = s->k_col_ptr[c]; e < s->k_col_ptr[c + 1]; ++e) cols[(size_t)o[(size_t)s->k_row_idx[e]]++] = (int)c;
+    });
+    std::vector<std::vector<long long>>().swap(off);
+    int bits = 1;
+    while ((1 << bits) < std::max({nx, ny, nz})) ++bits;
+    bits = std::min(bits, 10);
+    std::vector<uint64_t> key((size_t)mk, 0);
+    par_for(mk, [&](long long rb, long long re) {
+        for (long long r = rb; r < re; ++r) {
+            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {-1, -1, -1};
+            for (long long k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+                const int c = cols[(size_t)k];
+                const int p[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], p[a]);
+                    hi[a] = std::max(hi[a], p[a]);
+                }
+            }
+            if (rp[(size_t)r] == rp[(size_t)r + 1]) continue;  // empty row: key 0
+            int ax = 0;
+            for (int a = 1; a < 3; ++a)
+                if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+            long long sum[2][3] = {{0, 0, 0}, {0, 0, 0}};
+            long long cnt[2] = {0, 0};
+            for (long long k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+                const int c = cols[(size_t)k];
+                const int p[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
+                for (int end = 0; end < 2; ++end) {
+                    if (p[ax] != (end == 0 ? lo[ax] : hi[ax])) continue;
+                    for (int q = 0; q < 3; ++q) sum[end][q] += p[q];
+                    ++cnt[end];
+                }
+            }
+            int v[2][3];
+            for (int end = 0; end < 2; ++end)
+                for (int q = 0; q < 3; ++q) v[end][q] = (int)std::lround((double)sum[end][q] / (double)cnt[end]);
+            const bool swap = std::make_tuple(v[1][2], v[1][1], v[1][0]) > std::make_tuple(v[0][2], v[0][1], v[0][0]);
+            const int *f = swap ? v[1] : v[0], *g = swap ? v[0] : v[1];
+            const int k6[6] = {f[0], f[1], f[2], g[0], g[1], g[2]};
+            key[(size_t)r] = morton6(k6, bits);
+        }
+    });
+    return key;
+}
+
+}  // namespace
+
+void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
+               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *o) {
+    const int n = s->n, mk = s->m_k;
     o->rperm.resize((size_t)mk);
     for (int i = 0; i < mk; ++i) o->rperm[(size_t)i] = i;
     std::vector<int> sb = slice_bounds;
     if (sb.size() < 2) sb = {0, mk};
-    for (size_t q = 0; q + 1 < sb.size(); ++q)
-        std::stable_sort(o->rperm.begin() + sb[q], o->rperm.begin() + sb[q + 1], [&](int a, int b) {
-            return lo[(size_t)a] != lo[(size_t)b] ? lo[(size_t)a] < lo[(size_t)b] : hi[(size_t)a] < hi[(size_t)b];
-        });
+    if (grid && grid->nx > 0 && grid->ny > 0 && grid->nz > 0 && (long long)grid->nx * grid->ny * grid->nz == n) {
+        const std::vector<uint64_t> key = end_keys(s, grid->nx, grid->ny, grid->nz);
+        for (size_t q = 0; q + 1 < sb.size(); ++q)
+            std::stable_sort(o->rperm.begin() + sb[q], o->rperm.begin() + sb[q + 1],
+                             [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+    } else {
+        std::vector<int> tile_of((size_t)n, 0);
+        for (size_t t = 0; t + 1 < tile_start.size(); ++t)
+            for (int c = tile_start[t]; c < tile_start[t + 1]; ++c) tile_of[(size_t)c] = (int)t;
+        std::vector<int> lo((size_t)mk, INT_MAX), hi((size_t)mk, -1);
+        for (int c = 0; c < n; ++c) {
+            const int t = tile_of[(size_t)perm[(size_t)c]];
+            for (int e = s->k_col_ptr[c]; e < s->k_col_ptr[c + 1]; ++e) {
+                const int r = s->k_row_idx[e];
+                lo[(size_t)r] = std::min(lo[(size_t)r], t);
+                hi[(size_t)r] = std::max(hi[(size_t)r], t);
+            }
+        }
+        for (size_t q = 0; q + 1 < sb.size(); ++q)
+            std::stable_sort(o->rperm.begin() + sb[q], o->rperm.begin() + sb[q + 1], [&](int a, int b) {
+                return lo[(size_t)a] != lo[(size_t)b] ? lo[(size_t)a] < lo[(size_t)b] : hi[(size_t)a] < hi[(size_t)b];
+            });
+    }
     o->rinv.resize((size_t)mk);
     for (int i = 0; i < mk; ++i) o->rinv[(size_t)o->rperm[(size_t)i]] = i;
     o->ri.resize((size_t)s->nnz_k);

@@ -108,10 +108,12 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
 // it fetches are full instead of mostly padding.
 void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_start, std::vector<int> &perm);
 
-// Kernel rows renumbered by the first and last tile their columns fall in
-// (a stable sort, so equal keys keep the loaded order): rays with the same
-// entry and exit tiles become neighbours, and the gathers of one wavefront
-// share cache lines.  With slice bounds (row ranges of the loaded order) the
+// Kernel rows renumbered so that rays crossing the same bricks are
+// neighbours (a stable sort, equal keys keep the loaded order): with the
+// voxel grid known, by a 6-D Morton code of the ray's two end voxels (found
+// from its columns: the extremes along its longest axis); without one, by
+// the first and last tile its columns fall in.  Neighbouring rays then share
+// cache lines of u and of the segment partials inside a brick.  With slice bounds (row ranges of the loaded order) the
 // sort stays inside each slice, so every slice keeps its rows and its mix of
 // ray directions.  sys gets the renumbered kernel CSC (rows ascending in
 // every column); the damping arrays and b are the caller's.  rperm[new] =
@@ -123,7 +125,7 @@ struct RaySorted {
 };
 
 void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
-               const std::vector<int> &slice_bounds, RaySorted *out);
+               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *out);
 
 // Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
 // voxels, voxels inside a brick x fastest.  Bricks are numbered z fastest
