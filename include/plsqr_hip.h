@@ -224,8 +224,8 @@ int plsqr_aprod_split(plsqr_op *op, int mode, double *x, double *y_k, double *y_
 int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev);
 
 /* The same operator in the mixed-precision solver's arithmetic: FP32 matrix
- * values and vectors, every row / column dot product accumulated with the
- * compensated Dot2 scheme (aprod.c:24-84 in FP32; no counterpart in the
+ * values and vectors, every row / column dot product accumulated in FP64 from
+ * the exact products and rounded to FP32 once (aprod.c:24-84 in FP32; no counterpart in the
  * reference, which is FP64 only).  mode 1: y += A x; mode 2: x += A^T y; x: n
  * floats, y: m_k+m_d floats, caller order.  The _device form takes device
  * pointers and returns without synchronising; the host form copies in and
@@ -238,7 +238,7 @@ int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev);
 /* ---------------------------------------------------------------------- */
 enum {
     PLSQR_PREC_FP64 = 0,   /* plain FP64, the reference's arithmetic order          */
-    PLSQR_PREC_MIXED = 1   /* FP32 values and u, v with compensated accumulation,
+    PLSQR_PREC_MIXED = 1   /* FP32 values and u, v with FP64-accumulated dot products,
                               FP64 scalars and an FP64 residual refinement          */
 };
 

@@ -103,7 +103,7 @@ class Operator:
 
     def aprod_f32(self, mode, x, y):
         """The mixed-precision solver's operator: FP32 values and vectors with
-        compensated (Dot2) sums; in place on host float32 arrays."""
+        dot products accumulated in FP64; in place on host float32 arrays."""
         assert x.dtype == np.float32 and y.dtype == np.float32 and x.size == self.n and y.size == self.m
         _check(self.lib.plsqr_aprod_f32(self.h, int(mode), fptr(x), fptr(y)), self.h)
 

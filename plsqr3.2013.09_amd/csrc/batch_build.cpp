@@ -184,7 +184,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         L.n = s->n;
         std::vector<int> perm, tiles;
         if (g) {
-            brick_order(g->nx, g->ny, g->nz, 4, 4, 8, true, perm, tiles);
+            brick_order(g->nx, g->ny, g->nz, 4, 4, 8, std::getenv("PLSQR_BRICKS") ? std::atoi(std::getenv("PLSQR_BRICKS")) : 1, perm, tiles);
         } else {
             perm.resize((size_t)s->n);
             for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;

@@ -757,7 +757,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
         std::vector<int> perm, tiles;
         if (grid) {
-            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, env_int("PLSQR_ZCOL", 1) != 0, perm, tiles);
+            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, env_int("PLSQR_BRICKS", 1), perm, tiles);
         } else {
             perm.resize((size_t)s->n);
             for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
@@ -1385,7 +1385,7 @@ static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, doubl
     return PLSQR_OK;
 }
 
-// Mixed precision: an inner LSQR in FP32 (values, u, v; Dot2 sums; FP64
+// Mixed precision: an inner LSQR in FP32 (values, u, v; FP64-accumulated dot products; FP64
 // scalars, w and correction dx) solves min ||A dx - r|| for r = [b;0] - A x.
 // Every refine_every inner iterations the FP64 residual of xt = x + dx is
 // formed and its test2 = ||A^T r|| / (anorm ||r||) checked in FP64 (anorm:

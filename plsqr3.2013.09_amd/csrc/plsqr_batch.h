@@ -9,8 +9,8 @@
 // that has converged keeps its vectors untouched while the others go on.
 //
 // T = double: FP64 values and vectors, plain sums (the FP64 path).
-// T = float : FP32 values and vectors with the compensated Dot2 accumulation
-//             of plsqr_kernels.h; scalars, x and w stay FP64.
+// T = float : FP32 values and vectors, each dot product accumulated in FP64
+//             from exact products (dot32_add); scalars, x and w stay FP64.
 #pragma once
 
 #include <type_traits>
@@ -40,16 +40,16 @@ struct Acc<double> {
 
 template <>
 struct Acc<float> {
-    F2 f;
-    __device__ __forceinline__ void zero() { f = F2{0.f, 0.f}; }
-    __device__ __forceinline__ void add(float a, float x) { dot2_add(f, a, x); }
+    double s;  // exact FP64 products of the FP32 operands, summed in FP64
+    __device__ __forceinline__ void zero() { s = 0.0; }
+    __device__ __forceinline__ void add(float a, float x) { dot32_add(s, a, x); }
     template <int W>
     __device__ __forceinline__ void group() {
-        f = group_sum_f2<W>(f);
+        s = group_sum<W>(s);
     }
-    __device__ __forceinline__ void merge(const Acc &o) { f = f2_merge(f, o.f); }
-    __device__ __forceinline__ float value() const { return f.s + f.c; }
-    __device__ __forceinline__ double sum64() const { return (double)f.s + (double)f.c; }
+    __device__ __forceinline__ void merge(const Acc &o) { s += o.s; }
+    __device__ __forceinline__ float value() const { return (float)s; }
+    __device__ __forceinline__ double sum64() const { return s; }
 };
 
 // K consecutive values (one index of an interleaved vector)
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(BS) void kb_rows(int nch, const int *__restrict__ c
 // ray segments (one per lane, entries stored together in voxel order),
 // copies the group's entries into its LDS buffer CH at a time, and each lane
 // sums its segment for the K right-hand sides.  segv[seg*K + k] = the
-// partial, stored in T (the FP32 path rounds its Dot2 sum s + c once).
+// partial, stored in T (the FP32 path rounds its FP64 sum once).
 // ---------------------------------------------------------------------------
 template <class T, int K, int BS, int CH>
 __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, const int *__restrict__ i_prob,

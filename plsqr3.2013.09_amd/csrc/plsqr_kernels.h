@@ -77,47 +77,12 @@ __device__ double reduce_partials(const double *p, int count, int stride, double
     return block_sum<BS>(s, sh);
 }
 
-// Compensated FP32 dot products (Ogita, Rump and Oishi, "Accurate sum and dot
-// product", SIAM J. Sci. Comput. 26, 2005, Algorithm Dot2): TwoProduct via
-// fmaf and Knuth's TwoSum carry each rounding error in a second FP32 word.
-struct F2 {  // s + c, |c| << |s|
-    float s, c;
-};
-
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-__device__ __forceinline__ void dot2_add(F2 &acc, float a, float x) {
-    const float p = a * x;
-    const float ep = fmaf(a, x, -p);  // exact error of the product
-    float s, es;
-    two_sum(acc.s, p, s, es);
-    acc.s = s;
-    acc.c += ep + es;
-}
-
-__device__ __forceinline__ F2 f2_merge(F2 a, F2 b) {
-    F2 r;
-    float e;
-    two_sum(a.s, b.s, r.s, e);
-    r.c = (a.c + b.c) + e;
-    return r;
-}
-
-template <int W>
-__device__ __forceinline__ F2 group_sum_f2(F2 v) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        F2 t;
-        t.s = __shfl_xor(v.s, o, W);
-        t.c = __shfl_xor(v.c, o, W);
-        v = f2_merge(v, t);
-    }
-    return v;
-}
+// FP32 dot products with FP64 accumulation: the product of two FP32 values
+// is exact in FP64 (24 + 24 significant bits <= 53), so acc += (double)a *
+// (double)x rounds once per term, in FP64 (the build keeps the multiply and
+// the add separate).  Each FP32 dot product is thus as accurate as if it were
+// computed in twice the working precision, at two FP64 operations per term.
+__device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc += (double)a * (double)x; }
 
 // cblas.h:53-63
 __device__ __forceinline__ double d2norm(double a, double b) {

@@ -1,11 +1,9 @@
 // Mixed-precision kernels (included once, by plsqr_api.hip).
 //
-// Matrix values and the Golub-Kahan vectors u, v are stored and multiplied in
-// FP32; every row / column dot product is accumulated with the compensated
-// FP32 scheme of Ogita, Rump and Oishi ("Accurate sum and dot product", SIAM
-// J. Sci. Comput. 26, 2005, Algorithm Dot2): TwoProduct via fmaf and
-// Knuth's TwoSum carry the rounding errors in a second FP32 word, so each dot
-// product is as accurate as if computed in twice the working precision.
+// Matrix values and the Golub-Kahan vectors u, v are stored in FP32; every
+// row / column dot product accumulates the exact FP64 products of its FP32
+// operands in FP64 (dot32_add), so each dot product is as accurate as if
+// computed in twice the working precision and is rounded to FP32 once.
 // alpha, beta, the rotations and the estimates stay FP64 (the Scal kernels),
 // and so do x, w and the outer residual r = [b;0] - A x.
 #pragma once
@@ -14,7 +12,7 @@
 
 namespace plsqr {
 
-// Mode 1 in FP32: y <- fl32(fl32(y*s1)*s2) + dot2(row, x); FP64 partial sums
+// Mode 1 in FP32: y <- fl32(fl32(y*s1)*s2) + fl32(dot(row, x)); FP64 partial sums
 // of squares of the new y, pair layout (kernel rows, damping rows).
 template <int L, int BS>
 __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__restrict__ rp,
@@ -31,7 +29,7 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
     const int groups = (m_k + R - 1) / R;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
         const int r = g * R + slot;
-        F2 acc{0.f, 0.f};
+        double acc = 0.0;
         if (r < m_k) {
             const int b = rp[r], e = rp[r + 1];
             int k = b + lane;
@@ -39,25 +37,25 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
                 const int c0 = ci[k], c1 = ci[k + L], c2 = ci[k + 2 * L], c3 = ci[k + 3 * L];
                 const float a0 = av[k], a1 = av[k + L], a2 = av[k + 2 * L], a3 = av[k + 3 * L];
                 const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-                dot2_add(acc, a0, x0);
-                dot2_add(acc, a1, x1);
-                dot2_add(acc, a2, x2);
-                dot2_add(acc, a3, x3);
+                dot32_add(acc, a0, x0);
+                dot32_add(acc, a1, x1);
+                dot32_add(acc, a2, x2);
+                dot32_add(acc, a3, x3);
             }
-            for (; k < e; k += L) dot2_add(acc, av[k], x[ci[k]]);
+            for (; k < e; k += L) dot32_add(acc, av[k], x[ci[k]]);
         }
-        acc = group_sum_f2<L>(acc);
+        acc = group_sum<L>(acc);
         if (lane == 0 && r < m_k) {
-            const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+            const float yn = (y[r] * s1) * s2 + (float)acc;
             y[r] = yn;
             ssq_k += (double)yn * (double)yn;
         }
     }
     for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
          r += (long long)gridDim.x * BS) {
-        F2 acc{0.f, 0.f};
-        for (int k = rp[r]; k < rp[r + 1]; ++k) dot2_add(acc, av[k], x[ci[k]]);
-        const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) dot32_add(acc, av[k], x[ci[k]]);
+        const float yn = (y[r] * s1) * s2 + (float)acc;
         y[r] = yn;
         ssq_d += (double)yn * (double)yn;
     }
@@ -70,7 +68,7 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
 }
 
 // Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
-// k_rows_tiled_c): FP32 entries and x tile, Dot2 sum per segment, the
+// k_rows_tiled_c): FP32 entries and x tile, FP64 sum per segment, the
 // partial s + c rounded once to FP32.
 template <int BS, int CH>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
@@ -112,7 +110,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
         const int tot = __shfl(incl, G - 1);
         const int beg = incl - len, end = incl;
         const long long gb = ent_base[g];
-        F2 acc{0.f, 0.f};
+        double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
 #pragma unroll 4
@@ -125,12 +123,12 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int e0 = beg > q0 ? beg : q0;
             const int e1 = end < q0 + cnt ? end : q0 + cnt;
-            for (int e = e0; e < e1; ++e) dot2_add(acc, bv[e - q0], xl[bx[e - q0]]);
+            for (int e = e0; e < e1; ++e) dot32_add(acc, bv[e - q0], xl[bx[e - q0]]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (float)((double)acc.s + (double)acc.c);
+        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (float)acc;
     }
 }
 
@@ -180,9 +178,9 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int
     }
     for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
          r += (long long)gridDim.x * BS) {
-        F2 acc{0.f, 0.f};
-        for (int k = rp[r]; k < rp[r + 1]; ++k) dot2_add(acc, av[k], x[ci[k]]);
-        const float yn = (y[r] * s1) * s2 + (acc.s + acc.c);
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) dot32_add(acc, av[k], x[ci[k]]);
+        const float yn = (y[r] * s1) * s2 + (float)acc;
         y[r] = yn;
         ssq_d += (double)yn * (double)yn;
     }
@@ -204,7 +202,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     const Scal *__restrict__ sc, double *__restrict__ P) {
     constexpr int NW = BS / TW;
     __shared__ float g[CAP];
-    __shared__ F2 part[NW][TW];
+    __shared__ double part[NW][TW];
     if (sc->done || !sc->beta_pos) return;
     const float ib = (float)sc->ib;
     const int it = blockIdx.x;
@@ -218,7 +216,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     const long long s0 = seg_start[it];
     const int ns = (int)(seg_start[it + 1] - s0);
     const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
-    F2 acc{0.f, 0.f};
+    double acc = 0.0;
     int j = w;
     for (int r0 = 0; r0 < ns; r0 += CAP) {
         const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
@@ -249,14 +247,14 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
                 const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
                 const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
                 const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
-                dot2_add(acc, a0, g[g0]);
-                dot2_add(acc, a1, g[g1]);
-                dot2_add(acc, a2, g[g2]);
-                dot2_add(acc, a3, g[g3]);
+                dot32_add(acc, a0, g[g0]);
+                dot32_add(acc, a1, g[g1]);
+                dot32_add(acc, a2, g[g2]);
+                dot32_add(acc, a3, g[g3]);
             }
             for (; j < len; j += NW) {
                 const long long o = (long long)j * TW;
-                dot2_add(acc, sv[o], g[ss[o]]);
+                dot32_add(acc, sv[o], g[ss[o]]);
             }
         } else {
             int jend = len;
@@ -267,14 +265,14 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
                 const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
                 const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
                 const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
-                dot2_add(acc, a0, g[g0]);
-                dot2_add(acc, a1, g[g1]);
-                dot2_add(acc, a2, g[g2]);
-                dot2_add(acc, a3, g[g3]);
+                dot32_add(acc, a0, g[g0]);
+                dot32_add(acc, a1, g[g1]);
+                dot32_add(acc, a2, g[g2]);
+                dot32_add(acc, a3, g[g3]);
             }
             for (; j < jend; j += NW) {
                 const long long o = (long long)j * TW;
-                dot2_add(acc, sv[o], g[ss[o] - r0]);
+                dot32_add(acc, sv[o], g[ss[o] - r0]);
             }
         }
         __syncthreads();
@@ -282,10 +280,10 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     part[w][v] = acc;
     __syncthreads();
     if (w == 0 && v < tw) {
-        F2 tot = part[0][v];
+        double tot = part[0][v];
 #pragma unroll
-        for (int k = 1; k < NW; ++k) tot = f2_merge(tot, part[k][v]);
-        P[(size_t)s * n + c0 + v] = (double)tot.s + (double)tot.c;
+        for (int k = 1; k < NW; ++k) tot += part[k][v];
+        P[(size_t)s * n + c0 + v] = tot;
     }
 }
 

@@ -23,7 +23,7 @@ inline void slice_range(const plsqr_system *s, int o, int r0, int r1, int *b, in
 
 }  // namespace
 
-void brick_order(int nx, int ny, int nz, int bx, int by, int bz, bool z_columns,
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, int order,
                  std::vector<int> &perm, std::vector<int> &tile_start) {
     const long long n = (long long)nx * ny * nz;
     perm.assign((size_t)n, 0);
@@ -36,7 +36,25 @@ void brick_order(int nx, int ny, int nz, int bx, int by, int bz, bool z_columns,
                 for (int x = x0; x < std::min(nx, x0 + bx); ++x)
                     perm[(size_t)z * ny * nx + (size_t)y * nx + x] = next++;
     };
-    if (z_columns) {
+    if (order == 2) {
+        // Morton (Z-order) of the brick coordinates
+        const int nbx = (nx + bx - 1) / bx, nby = (ny + by - 1) / by, nbz = (nz + bz - 1) / bz;
+        std::vector<std::pair<uint64_t, int>> key;
+        for (int z = 0; z < nbz; ++z)
+            for (int y = 0; y < nby; ++y)
+                for (int x = 0; x < nbx; ++x) {
+                    uint64_t m = 0;
+                    for (int i = 0; i < 20; ++i)
+                        m |= (uint64_t)((x >> i) & 1) << (3 * i) | (uint64_t)((y >> i) & 1) << (3 * i + 1) |
+                             (uint64_t)((z >> i) & 1) << (3 * i + 2);
+                    key.emplace_back(m, (z * nby + y) * nbx + x);
+                }
+        std::sort(key.begin(), key.end());
+        for (const auto &k : key) {
+            const int id = k.second, x = id % nbx, y = (id / nbx) % nby, z = id / (nbx * nby);
+            brick(x * bx, y * by, z * bz);
+        }
+    } else if (order == 1) {
         for (int y0 = 0; y0 < ny; y0 += by)
             for (int x0 = 0; x0 < nx; x0 += bx)
                 for (int z0 = 0; z0 < nz; z0 += bz) brick(x0, y0, z0);

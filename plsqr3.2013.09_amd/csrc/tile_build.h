@@ -128,12 +128,12 @@ void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::v
                const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *out);
 
 // Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
-// voxels, voxels inside a brick x fastest.  Bricks are numbered z fastest
-// (vertical stacks of bricks, then x, then y) when z_columns is set, so that
-// the workgroups running at the same time on one XCD cover a few stacks and
-// the rays crossing them; otherwise in x-fastest brick order.  Fills perm
-// (old -> new) and the tile starts (one tile per brick).
-void brick_order(int nx, int ny, int nz, int bx, int by, int bz, bool z_columns,
+// voxels, voxels inside a brick x fastest.  Brick order: 1 = z fastest
+// (vertical stacks of bricks, then x, then y), 2 = Morton (Z-order) of the
+// brick coordinates, 0 = x fastest.  Workgroups run roughly in brick order,
+// so the order sets how far apart in time one ray's bricks are processed.
+// Fills perm (old -> new) and the tile starts (one tile per brick).
+void brick_order(int nx, int ny, int nz, int bx, int by, int bz, int order,
                  std::vector<int> &perm, std::vector<int> &tile_start);
 
 // Builds the layout from the kernel CSC of `s`.  perm/tile_start describe the

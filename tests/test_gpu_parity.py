@@ -53,7 +53,7 @@ def test_aprod_matches_oracle(P, O, MG, gpu_available, name, exact):
 
 @pytest.mark.parametrize("name", SMALL)
 def test_aprod_f32_matches_oracle(P, O, MG, gpu_available, name):
-    """The mixed-precision operator (FP32 values and vectors, Dot2 sums)
+    """The mixed-precision operator (FP32 values and vectors, FP64 sums)
     against the FP64 oracle: FP32 rounding of the values and of the result
     bounds the difference by a few 1e-8 relative; the bar is 1e-6."""
     announce(f"aprod f32 {name}")
