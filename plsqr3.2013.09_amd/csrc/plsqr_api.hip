@@ -581,7 +581,8 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
             if ((rc = dalloc(op, &op->rt_gbase, (size_t)items2 + 1)) ||
                 (rc = dalloc(op, &op->rt_ebase, (size_t)R->n_grp + 1)) ||
                 (rc = dalloc(op, &op->rt_len, (size_t)R->n_grp * plsqr::RowTiles::G)) ||
-                (rc = dalloc(op, &op->rt_vox, (size_t)R->n_ent)) || (rc = dalloc(op, &op->rt_val, (size_t)R->n_ent)) ||
+                // entries carry 4 of slack: the 16-byte staging loads may read past the last group
+                (rc = dalloc(op, &op->rt_vox, (size_t)R->n_ent + 4)) || (rc = dalloc(op, &op->rt_val, (size_t)R->n_ent + 4)) ||
                 (rc = dalloc(op, &op->rt_rptr, (size_t)op->m_k + 1)) || (rc = dalloc(op, &op->rt_rseg, (size_t)L->n_seg)) ||
                 (rc = dalloc(op, &op->rt_segv, (size_t)L->n_seg)))
                 return rc;
@@ -1309,7 +1310,7 @@ static int ensure_fp32(plsqr_op *op) {
                        op->t_nslot, op->t_sval, op->t_sval32);
     LAUNCH_CHECK(op);
     if (op->rtiled) {
-        if ((rc = dalloc(op, &op->rt_val32, op->rt_nent))) return rc;
+        if ((rc = dalloc(op, &op->rt_val32, op->rt_nent + 4))) return rc;
         hipLaunchKernelGGL(k_to_f32, dim3(grid_for(op->rt_nent, 256, op->cu * 16)), dim3(256), 0, op->stream,
                            op->rt_nent, op->rt_val, op->rt_val32);
         LAUNCH_CHECK(op);

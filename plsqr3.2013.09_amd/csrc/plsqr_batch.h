@@ -262,8 +262,8 @@ __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, c
     static_assert(G == kWave, "one segment group per wavefront");
     static_assert(CH % G == 0, "whole wavefront copy steps");
     __shared__ T xl[128 * K];
-    __shared__ T lv[NWV][CH];
-    __shared__ uint8_t lx[NWV][CH];
+    __shared__ __align__(16) T lv[NWV][CH + 8];
+    __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
     const int it = blk_item[blockIdx.x];
     if (it < 0) return;
     const int p = i_prob[it];
@@ -297,18 +297,15 @@ __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, c
         for (int k = 0; k < K; ++k) acc[k].zero();
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
-            for (int i = lane; i < cnt; i += G) {
-                bv[i] = ent_val[gb + q0 + i];
-                bx[i] = ent_vox[gb + q0 + i];
-            }
+            const int sh = stage_entries<T, G>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int e0 = beg > q0 ? beg : q0;
             const int e1 = end < q0 + cnt ? end : q0 + cnt;
             for (int e = e0; e < e1; ++e) {
-                const T a = bv[e - q0];
-                const T *xv = xl + (int)bx[e - q0] * K;
+                const T a = bv[sh + e];
+                const T *xv = xl + (int)bx[sh + e] * K;
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k].add(a, xv[k]);
             }

@@ -371,7 +371,7 @@ int ensure_fp32(plsqr_batch *b) {
                        b->slots, b->sval, b->sval32);
     BT_LAUNCH(b);
     if (b->rtiled) {
-        if ((rc = balloc(b, &b->rt_val32, (size_t)b->ents))) return rc;
+        if ((rc = balloc(b, &b->rt_val32, (size_t)b->ents + 4))) return rc;
         hipLaunchKernelGGL((kb_convert<float>), dim3(grid_for(b->ents, 256, b->cu * 8)), dim3(256), 0, b->stream,
                            b->ents, b->rt_val, b->rt_val32);
         BT_LAUNCH(b);
@@ -864,7 +864,8 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     if (b->rtiled &&
         ((rc = balloc(b, &b->rt_gbase, (size_t)b->items + 1)) || (rc = balloc(b, &b->rt_ebase, (size_t)b->groups + 1)) ||
          (rc = balloc(b, &b->rt_len, (size_t)b->groups * plsqr::RowTiles::G)) ||
-         (rc = balloc(b, &b->rt_vox, (size_t)b->ents)) || (rc = balloc(b, &b->rt_val, (size_t)b->ents)) ||
+         // entries carry 4 of slack: the 16-byte staging loads may read past the last group
+         (rc = balloc(b, &b->rt_vox, (size_t)b->ents + 4)) || (rc = balloc(b, &b->rt_val, (size_t)b->ents + 4)) ||
          (rc = balloc(b, &b->rt_rptr, (size_t)b->rows + 1)) || (rc = balloc(b, &b->rt_rseg, (size_t)b->segs)) ||
          (rc = balloc(b, &b->segv, (size_t)b->segs * K))))
         return rc;

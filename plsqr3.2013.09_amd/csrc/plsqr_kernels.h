@@ -11,6 +11,8 @@
 // results are bitwise reproducible from run to run.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -83,6 +85,27 @@ __device__ double reduce_partials(const double *p, int count, int stride, double
 // the add separate).  Each FP32 dot product is thus as accurate as if it were
 // computed in twice the working precision, at two FP64 operations per term.
 __device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc += (double)a * (double)x; }
+
+// Copies the cnt (value, voxel) entries at global index base into a
+// wavefront's LDS buffers with 16-byte loads (two FP64 or four FP32 values,
+// and the matching voxel bytes per lane and step).  The copy starts at the
+// 16-byte boundary at or below base, so the caller finds entry i at
+// bv[shift + i], shift = the return value < V; the buffers hold cnt + 2V - 2
+// entries, and the global arrays carry V - 1 entries of slack past their end.
+template <class T, int G>
+__device__ __forceinline__ int stage_entries(const T *__restrict__ val, const uint8_t *__restrict__ vox, long long base,
+                                             int cnt, T *bv, uint8_t *bx, int lane) {
+    constexpr int V = 16 / (int)sizeof(T);
+    using VoxV = typename std::conditional<V == 2, uint16_t, uint32_t>::type;
+    const long long a0 = base & ~(long long)(V - 1);
+    const int sh = (int)(base - a0);
+    const int nv = (sh + cnt + V - 1) / V;
+    for (int i = lane; i < nv; i += G) {
+        *reinterpret_cast<uint4 *>(bv + i * V) = *reinterpret_cast<const uint4 *>(val + a0 + (long long)i * V);
+        *reinterpret_cast<VoxV *>(bx + i * V) = *reinterpret_cast<const VoxV *>(vox + a0 + (long long)i * V);
+    }
+    return sh;
+}
 
 // cblas.h:53-63
 __device__ __forceinline__ double d2norm(double a, double b) {

@@ -84,8 +84,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     static_assert(G == kWave, "one segment group per wavefront");
     static_assert(CH % G == 0, "whole wavefront copy steps");
     __shared__ float xl[128];
-    __shared__ float lv[NWV][CH];
-    __shared__ uint8_t lx[NWV][CH];
+    __shared__ __align__(16) float lv[NWV][CH + 8];
+    __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
     if (sc->done) return;
     const int it = blockIdx.x;
     const int g0 = grp_base[it], g1 = grp_base[it + 1];
@@ -113,17 +113,13 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
         double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
-#pragma unroll 4
-            for (int i = lane; i < cnt; i += G) {
-                bv[i] = ent_val[gb + q0 + i];
-                bx[i] = ent_vox[gb + q0 + i];
-            }
+            const int sh = stage_entries<float, G>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int e0 = beg > q0 ? beg : q0;
             const int e1 = end < q0 + cnt ? end : q0 + cnt;
-            for (int e = e0; e < e1; ++e) dot32_add(acc, bv[e - q0], xl[bx[e - q0]]);
+            for (int e = e0; e < e1; ++e) dot32_add(acc, bv[sh + e], xl[bx[sh + e]]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
