@@ -278,7 +278,7 @@ def main():
                     help="mixed precision: inner FP32 iterations per FP64 residual refinement")
     ap.add_argument("--max-iters", type=int, default=5000)
     ap.add_argument("--aprod-reps", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--aprod-only", action="store_true",
                     help="only time aprod launches and a device copy of known size "
