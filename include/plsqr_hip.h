@@ -283,6 +283,15 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
                             double *x_dev, double *w_dev, plsqr_solve_out *out,
                             double *trace);
 
+/* Plain FP64 solve as plsqr_lsqr_solve that also returns the standard-error
+ * estimates of the reference's wantse branch (lsqr.c:644-670 accumulate
+ * var += (w/rho)^2 per iteration; lsqr.c:878-891 scale se = rnorm / sqrt(t)
+ * * sqrt(var) with t = m - n when m > n, m when damp > 0, else 1; m = m_k +
+ * m_d).  The reference itself always passes se = NULL.  x_host and se_host:
+ * n doubles, caller's column order.  PLSQR_ERR_ARG for the mixed precision. */
+int plsqr_lsqr_solve_se(plsqr_op *op, const double *b_host, const plsqr_solve_opts *opts,
+                        double *x_host, double *se_host, plsqr_solve_out *out);
+
 /* Residual and normal-equation residual in FP64 on the device:
  * r = [b;0] - A x,  rnorm = ||r||,  arnorm = ||A^T r||,  anorm_f = ||A||_F.
  * Used to confirm a test2 reading with ||A^T r|| / (||A||_F ||r||). */

@@ -156,6 +156,17 @@ class Operator:
             info["w"] = w
         return (x, info, tr) if trace else (x, info)
 
+    def lsqr_se(self, b, itn, tol=0.0, damp=0.0):
+        """Plain FP64 solve with the standard-error estimates: (x, se, info)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.size == self.m_k
+        x = np.empty(self.n, np.float64)
+        se = np.empty(self.n, np.float64)
+        out = SolveOutC()
+        o = self.make_opts(itn, tol, PREC_FP64, 25, damp)
+        _check(self.lib.plsqr_lsqr_solve_se(self.h, dptr(b), C.byref(o), dptr(x), dptr(se), C.byref(out)), self.h)
+        return x, se, out.as_dict()
+
     def lsqr_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
                     damp=0.0, w_ptr=None):
         out = SolveOutC()

@@ -135,6 +135,8 @@ SIGNATURES = {
     "plsqr_aprod_split": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p]),
     "plsqr_aprod_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "plsqr_aprod_f32": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
+    "plsqr_lsqr_solve_se": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p, c_double_p,
+                                      C.POINTER(SolveOutC)]),
     "plsqr_aprod_device_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "plsqr_lsqr_solve": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p,
                                    c_double_p, C.POINTER(SolveOutC), c_double_p]),

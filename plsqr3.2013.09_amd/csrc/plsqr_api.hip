@@ -1148,15 +1148,17 @@ static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, dou
 
 // u_full: NULL for u = [b;0], else the full m-vector to start from (a warm
 // start's residual r0; the solve then returns the correction).
+// se: NULL, or n doubles (device order) that receive the standard-error
+// estimates of lsqr.c:644-670, 878-891.
 static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, double *x, double *w,
-                      plsqr_solve_out *out, double *trace_host, const double *u_full = nullptr) {
+                      plsqr_solve_out *out, double *trace_host, const double *u_full = nullptr,
+                      double *se = nullptr) {
     using namespace plsqr;
     int rc;
     const int itn = std::max(0, o->itn);
     const bool want_trace = trace_host != nullptr;
     if (want_trace && (rc = ensure_trace(op, itn + 1))) return rc;
     double *trace_d = want_trace ? op->trace_d : nullptr;
-    double *se = nullptr;  // standard errors are not exposed through this entry point
     Scal h;
     memset(&h, 0, sizeof(h));
     h.damp = o->damp;
@@ -1180,6 +1182,7 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->v);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, x);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, w);
+    if (se) hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, se);
     LAUNCH_CHECK(op);
     // beta = ||u||
     if (op->exact) {
@@ -1272,6 +1275,15 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
     LAUNCH_CHECK(op);
     HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
     HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if (se) {
+        // se = rnorm / sqrt(m - n, or m when damped, or 1) * sqrt(var)  (lsqr.c:878-891)
+        double t = 1.0;
+        if (op->m > op->n) t = (double)(op->m - op->n);
+        if (h.damp > 0.0) t = (double)op->m;
+        t = h.rnorm / std::sqrt(t);
+        hipLaunchKernelGGL(k_se_final, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n, t, se);
+        LAUNCH_CHECK(op);
+    }
     if (out) {
         memset(out, 0, sizeof(*out));
         int istop = h.istop;
@@ -1589,6 +1601,35 @@ int plsqr_lsqr_solve(plsqr_op *op, const double *b_host, const plsqr_solve_opts 
         }
         HIP_TRY(op, hipStreamSynchronize(op->stream));
     }
+    return PLSQR_OK;
+}
+
+int plsqr_lsqr_solve_se(plsqr_op *op, const double *b_host, const plsqr_solve_opts *opts, double *x_host,
+                        double *se_host, plsqr_solve_out *out) {
+    using namespace plsqr;
+    if (!op) return PLSQR_ERR_ARG;
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!b_host || !opts || !x_host || !se_host) return set_err(op, PLSQR_ERR_ARG, "b, opts, x and se are required");
+    plsqr_solve_opts o;
+    default_opts(opts, &o);
+    if (o.precision != PLSQR_PREC_FP64) return set_err(op, PLSQR_ERR_ARG, "standard errors come with the FP64 solve");
+    for (int i = 0; i < op->m_k; ++i)
+        if (!std::isfinite(b_host[i])) return set_err(op, PLSQR_ERR_NONFINITE, "b holds a NaN or Inf at " + std::to_string(i));
+    int rc;
+    if (!op->se && (rc = dalloc(op, &op->se, op->n))) return rc;
+    HIP_TRY(op, hipMemcpyAsync(op->bst, b_host, sizeof(double) * op->m_k, hipMemcpyHostToDevice, op->stream));
+    const double *b_dev = op->bst;
+    if ((rc = b_internal(op, b_dev, &b_dev))) return rc;
+    if ((rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, nullptr, nullptr, op->se))) return rc;
+    const double *xs = op->x, *ss = op->se;
+    if (op->permuted) {
+        if ((rc = to_external(op, op->x, op->hx)) || (rc = to_external(op, op->se, op->hx2))) return rc;
+        xs = op->hx;
+        ss = op->hx2;
+    }
+    HIP_TRY(op, hipMemcpyAsync(x_host, xs, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipMemcpyAsync(se_host, ss, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
     return PLSQR_OK;
 }
 

@@ -5,6 +5,8 @@ reference; default operator within 1e-12 relative per aprod and 1e-10
 relative in x after the reference's iteration count (the tolerance the
 plain FP64 path is held to).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -81,6 +83,29 @@ def test_aprod_f32_matches_oracle(P, O, MG, gpu_available, name):
         with pytest.raises(P.PlsqrError) as e:
             op.aprod_f32(1, x.copy(), y.copy())
         assert e.value.code == 6
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_standard_errors_match_oracle(P, O, MG, gpu_available, name):
+    """plsqr_lsqr_solve_se against the oracle's wantse branch: bitwise on the
+    exact-order operator after 10 iterations; within 1e-10 on the default
+    operator after 5 (before the iterates become rounding-sensitive)."""
+    announce(f"standard errors {name}")
+    s = MG.make_system(P, MG.CASES[name])
+    orc = O.Oracle(s)
+    xo10, res10, _ = orc.lsqr(s.b, 10, want_se=True)
+    xo5, res5, _ = orc.lsqr(s.b, 5, want_se=True)
+    with P.Operator(s, P.OP_EXACT_ORDER) as op:
+        x, se, info = op.lsqr_se(s.b, 10)
+    assert np.array_equal(x, xo10) and np.array_equal(se, res10["se"])
+    with P.Operator(s) as op:
+        x, se, info = op.lsqr_se(s.b, 5)
+        # the mixed precision carries no standard errors: PLSQR_ERR_ARG
+        o = op.make_opts(5, 0.0, P.PREC_MIXED)
+        xx, ss = np.empty(s.n), np.empty(s.n)
+        b = np.ascontiguousarray(s.b)
+        assert op.lib.plsqr_lsqr_solve_se(op.h, P.dptr(b), ctypes.byref(o), P.dptr(xx), P.dptr(ss), None) == 1
+    assert rel(x, xo5) < 1e-10 and rel(se, res5["se"]) < 1e-10, (rel(x, xo5), rel(se, res5["se"]))
 
 
 def test_aprod_entry_points_agree(P, MG, gpu_available):

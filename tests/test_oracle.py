@@ -4,6 +4,8 @@
   * against closed forms (dnrm2 = sqrt(sum x^2), d2norm = hypot),
   * against dense brute force and the adjoint identity for the SpMVs,
   * against scipy.sparse.linalg.lsqr for the LSQR recurrences,
+  * the standard-error branch (wantse) against a textbook Golub-Kahan
+    bidiagonalisation with full reorthogonalisation,
   * and with planted mistakes, to show each check can fail.
 """
 import math
@@ -233,3 +235,62 @@ def test_norm_rounding_alone_moves_x(P, O, MG):
     fs = lambda a: math.sqrt(math.fsum(a * a))
     d = rel(run(fs, fs), g["x"])
     assert 2e-11 < d < 1e-9, d
+
+
+def golub_kahan_se(A, bf, k):
+    """Textbook Golub-Kahan bidiagonalisation of A from bf, k steps with full
+    reorthogonalisation, and the LSQR quantities it fixes: x_k = V_k y_k with
+    y_k = argmin ||B_k y - beta1 e1||, rnorm_k = that minimum, and the
+    variance estimate var_k = diag(V_k (B_k^T B_k)^-1 V_k^T) that LSQR
+    accumulates as sum_j (w_j / rho_j)^2 (Paige and Saunders 1982)."""
+    beta1 = np.linalg.norm(bf)
+    U = [bf / beta1]
+    v = A.T @ U[0]
+    al = [np.linalg.norm(v)]
+    V = [v / al[0]]
+    be = [beta1]
+    for _ in range(k):
+        u = A @ V[-1] - al[-1] * U[-1]
+        for q in U:
+            u -= (q @ u) * q
+        be.append(np.linalg.norm(u))
+        U.append(u / be[-1])
+        v = A.T @ U[-1] - be[-1] * V[-1]
+        for q in V:
+            v -= (q @ v) * q
+        al.append(np.linalg.norm(v))
+        V.append(v / al[-1])
+    B = np.zeros((k + 1, k))
+    for i in range(k):
+        B[i, i] = al[i]
+        B[i + 1, i] = be[i + 1]
+    Vk = np.array(V[:k]).T
+    rhs = np.zeros(k + 1)
+    rhs[0] = beta1
+    y = np.linalg.lstsq(B, rhs, rcond=None)[0]
+    var = np.einsum("ij,jk,ik->i", Vk, np.linalg.inv(B.T @ B), Vk)
+    return Vk @ y, np.linalg.norm(B @ y - rhs), var
+
+
+@pytest.mark.parametrize("args", [(3, 3, 3, 60, 0, 7, 0.01, 0.3), (4, 3, 3, 80, 1, 11, 0.02, 0.2),
+                                  (5, 4, 3, 200, 0, 13, 0.01, 0.05)])
+def test_standard_errors_match_reorthogonalised_bidiagonalisation(P, O, args):
+    """se = rnorm / sqrt(m - n) * sqrt(var) (lsqr.c:644-670, 878-891).  For
+    the first few iterations the plain recurrences keep orthogonality to
+    rounding, so the oracle must match the reorthogonalised textbook values
+    to ~1e-14; the bar is 1e-12.  Scaling by sqrt(m) instead of sqrt(m - n),
+    or dropping the 1/rho factor, misses it by 10% or more."""
+    s = P.generate(*args)
+    A = dense(s)
+    m, n = A.shape
+    bf = np.concatenate([s.b, np.zeros(s.m_d)])
+    orc = O.Oracle(s)
+    for k in (1, 2, 5):
+        x, res, _ = orc.lsqr(s.b, k, want_se=True)
+        xg, rn, var = golub_kahan_se(A, bf, k)
+        se_g = rn / math.sqrt(m - n) * np.sqrt(var)
+        assert rel(x, xg) < 1e-12, (k, rel(x, xg))
+        assert abs(res["rnorm"] - rn) <= 1e-12 * rn
+        assert np.abs(res["se"] - se_g).max() <= 1e-12 * np.abs(se_g).max(), k
+        wrong = rn / math.sqrt(m) * np.sqrt(var)
+        assert np.abs(wrong - se_g).max() > 0.05 * np.abs(se_g).max()
