@@ -28,10 +28,12 @@
  *   orc_lsqr               : scipy.sparse.linalg.lsqr on small systems; the
  *                            golden x (bitwise) and alpha/beta trace written
  *                            by the compiled FP64 reference.
- *   orc_lsqr standard errors (se) : parity unpinned against the reference,
- *                            which always passes se = NULL (lsqr.c:80); checked
- *                            against the closed form diag((A^T A)^-1)-style
- *                            estimate only through its definition.
+ *   orc_lsqr standard errors (se) : the reference always passes se = NULL
+ *                            (lsqr.c:80), so no reference output exists; pinned
+ *                            against a textbook Golub-Kahan bidiagonalisation
+ *                            with full reorthogonalisation (numpy), whose
+ *                            rnorm / sqrt(m - n) * sqrt(diag(V (B^T B)^-1 V^T))
+ *                            the first iterations must reproduce to rounding.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off).
  */
