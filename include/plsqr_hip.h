@@ -146,6 +146,27 @@ int plsqr_read_reference_files(const char *dir,
                                const char *b_k, int row_k, int row_d, int col,
                                plsqr_system *out);
 
+/* Kernel re-layout (PLSQR3_tools/kernel_format/Ker2PLSQR3.c:358-415 reads
+ * the rows in order and hands every entry to its column; :595-606 writes
+ * the column-major records, rows ascending within a column), done on the
+ * current HIP device as a stable radix sort of the entries by column.
+ *   in : m_k rows, row_ptr (m_k+1, from 0), col1 (1-based columns, any order
+ *        within a row), val;
+ *   out: col_ptr (n+1), row_idx (0-based, ascending within a column), val_out
+ *        (nnz each): the kernel CSC of plsqr_system.
+ * Columns outside 1..n -> PLSQR_ERR_BOUNDS; NaN/Inf -> PLSQR_ERR_NONFINITE. */
+int plsqr_kernel_rows_to_csc(int m_k, int n, const int *row_ptr, const int *col1, const double *val,
+                             int *col_ptr, int *row_idx, double *val_out);
+
+/* The tool itself: reads the kernel rows named in dir/list_file (one "file
+ * nnz" pair per line, FileIO.c:353-402; file r holds row r as nnz 16-byte
+ * records {int32 aux, int32 col (1-based), float64 val}), re-lays them with
+ * plsqr_kernel_rows_to_csc and writes dir/out_bin and dir/out_info in the
+ * formats above.  n_col: the number of columns (Ker2PLSQR3's argv[3]; the
+ * per-column counts of its argv[4] are computed instead of read). */
+int plsqr_ker2plsqr3(const char *dir, const char *list_file, int n_col, const char *out_bin,
+                     const char *out_info);
+
 /* Raw float64[n] in global column order, the layout of mpiio_write_result
  * (plsqr3.c:1986-2013) for the FP64 build. */
 int plsqr_write_x(const char *path, const double *x, int n);

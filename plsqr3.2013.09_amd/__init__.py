@@ -20,6 +20,7 @@ from ._lib import (  # noqa: F401
     ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT, OP_EXACT_ORDER,
     OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH, PlsqrError,
     SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, fptr, gen_field, gen_rhs, generate,
+    kernel_rows_to_csc, ker2plsqr3,
     generate_batch, generate_config, load, read_x, write_x, _check,
 )
 

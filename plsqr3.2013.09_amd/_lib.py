@@ -135,6 +135,9 @@ SIGNATURES = {
     "plsqr_aprod_split": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p]),
     "plsqr_aprod_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "plsqr_aprod_f32": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
+    "plsqr_kernel_rows_to_csc": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p,
+                                           c_double_p]),
+    "plsqr_ker2plsqr3": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_char_p]),
     "plsqr_lsqr_solve_se": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p, c_double_p,
                                       C.POINTER(SolveOutC)]),
     "plsqr_aprod_device_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -421,6 +424,28 @@ def generate_config(name, **override):
     cfg.update(override)
     cfg.pop("itn", None)
     return generate(**cfg)
+
+
+def kernel_rows_to_csc(m_k, n, row_ptr, col1, val):
+    """Kernel rows (row_ptr from 0, 1-based columns in any order within a row)
+    to the kernel CSC on the GPU: (col_ptr, row_idx 0-based, val)."""
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col1 = np.ascontiguousarray(col1, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    nnz = int(row_ptr[-1]) if row_ptr.size else 0
+    cp = np.empty(n + 1, np.int32)
+    ri = np.empty(max(nnz, 1), np.int32)
+    vo = np.empty(max(nnz, 1), np.float64)
+    _check(load().plsqr_kernel_rows_to_csc(int(m_k), int(n), iptr(row_ptr), iptr(col1), dptr(val), iptr(cp),
+                                           iptr(ri), dptr(vo)))
+    return cp, ri[:nnz], vo[:nnz]
+
+
+def ker2plsqr3(directory, list_file, n_col, out_bin, out_info):
+    """The Ker2PLSQR3 step on the GPU: row files listed in directory/list_file
+    to the column-sorted kernel files directory/out_bin and out_info."""
+    _check(load().plsqr_ker2plsqr3(directory.encode(), list_file.encode(), int(n_col), out_bin.encode(),
+                                   out_info.encode()))
 
 
 def read_x(path, n):

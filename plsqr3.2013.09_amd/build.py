@@ -32,7 +32,8 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
-SOURCES = (("plsqr_api.hip", DEVICE), ("plsqr_batch.hip", DEVICE), ("plsqr_cnn.hip", DEVICE), ("tomo_io.cpp", []),
+SOURCES = (("plsqr_api.hip", DEVICE), ("plsqr_batch.hip", DEVICE), ("plsqr_cnn.hip", DEVICE),
+           ("plsqr_relayout.hip", DEVICE), ("tomo_io.cpp", []),
            ("tile_build.cpp", []), ("batch_build.cpp", []))
 
 

@@ -776,3 +776,7 @@ int plsqr_read_x(const char *path, double *x, int n) {
 }
 
 }  // extern "C"
+
+namespace plsqr {
+int global_fail(int code, const char *msg) { return fail(code, msg); }
+}  // namespace plsqr

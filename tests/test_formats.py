@@ -191,3 +191,74 @@ def test_tiled_layout_self_check_on_host(P, monkeypatch):
                 assert e.code == 4 and "self-check" not in str(e), str(e)
             else:  # a GPU is present: the operator was built
                 op.close()
+
+
+def kernel_rows(s, seed):
+    """The kernel of s as rows (Ker2PLSQR3's input): row pointers, 1-based
+    columns shuffled inside every row, values."""
+    import scipy.sparse as sp
+    A = sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)).tocsr()
+    rng = np.random.default_rng(seed)
+    cols, vals = A.indices.copy(), A.data.copy()
+    for r in range(s.m_k):
+        b, e = A.indptr[r], A.indptr[r + 1]
+        p = rng.permutation(e - b)
+        cols[b:e], vals[b:e] = cols[b:e][p], vals[b:e][p]
+    return A.indptr.astype(np.int32), (cols + 1).astype(np.int32), vals
+
+
+def test_kernel_relayout_rejects_bad_rows_on_the_host(P):
+    """Ker2PLSQR3's input checks run before any device call."""
+    rp = np.array([0, 2, 3], np.int32)
+    with pytest.raises(P.PlsqrError) as e:
+        P.kernel_rows_to_csc(2, 4, rp, np.array([1, 5, 2], np.int32), np.ones(3))
+    assert e.value.code == 2  # column 5 > n
+    with pytest.raises(P.PlsqrError) as e:
+        P.kernel_rows_to_csc(2, 4, np.array([0, 2, 1], np.int32), np.array([1, 2, 3], np.int32), np.ones(3))
+    assert e.value.code == 7  # row_ptr decreases
+    with pytest.raises(P.PlsqrError) as e:
+        P.kernel_rows_to_csc(2, 4, rp, np.array([1, 2, 3], np.int32), np.array([1.0, np.nan, 2.0]))
+    assert e.value.code == 8
+
+
+def test_ker2plsqr3_missing_list_is_an_io_error(P, tmp_path):
+    with pytest.raises(P.PlsqrError) as e:
+        P.ker2plsqr3(str(tmp_path), "no_such.list", 10, "out.bin", "out.info")
+    assert e.value.code == 3
+
+
+@pytest.mark.gpu
+def test_kernel_relayout_reproduces_the_csc(P, gpu_available):
+    """Rows with their columns shuffled, re-laid on the GPU: the kernel CSC
+    comes back bit for bit (rows ascending inside every column, as
+    Ker2PLSQR3.c:595-606 writes them)."""
+    print("kernel relayout", flush=True)
+    for args, seed in (((16, 16, 16, 20000, 0, 2013, 0.01, 0.1), 1), ((9, 7, 20, 3000, 1, 4, 0.02, 0.05), 2)):
+        s = P.generate(*args)
+        rp, c1, v = kernel_rows(s, seed)
+        cp, ri, vo = P.kernel_rows_to_csc(s.m_k, s.n, rp, c1, v)
+        assert np.array_equal(cp, s.k_col_ptr) and np.array_equal(ri, s.k_row_idx) and np.array_equal(vo, s.k_val)
+
+
+@pytest.mark.gpu
+def test_ker2plsqr3_files_load_as_the_reference_system(P, gpu_available, tmp_path):
+    """One row file per ray plus the list, through the tool, then the
+    reference loader: the same system as the generator's."""
+    print("ker2plsqr3 files", flush=True)
+    s = P.generate(9, 7, 5, 600, 0, 21, 0.01, 0.1)
+    rp, c1, v = kernel_rows(s, 3)
+    lines = []
+    for r in range(s.m_k):
+        rec = np.zeros(rp[r + 1] - rp[r], dtype=[("aux", "<i4"), ("col", "<i4"), ("val", "<f8")])
+        rec["aux"], rec["col"], rec["val"] = r + 1, c1[rp[r]:rp[r + 1]], v[rp[r]:rp[r + 1]]
+        rec.tofile(str(tmp_path / f"row{r}.bin"))
+        lines.append(f"row{r}.bin {rec.size}")
+    (tmp_path / "rows.list").write_text("\n".join(lines) + "\n")
+    names = s.write_reference_files(str(tmp_path))
+    P.ker2plsqr3(str(tmp_path), "rows.list", s.n, "matrix_bycol.mm2.bin", "matrix_bycol.mm2.info")
+    names2 = dict(names, ker_f="matrix_bycol.mm2.bin", ker_i="matrix_bycol.mm2.info")
+    t = P.System.read_reference_files(str(tmp_path), names2, s.m_k, s.m_d, s.n)
+    assert np.array_equal(t.k_col_ptr, s.k_col_ptr) and np.array_equal(t.k_row_idx, s.k_row_idx)
+    assert np.array_equal(t.k_val, s.k_val)
+    assert (tmp_path / "matrix_bycol.mm2.bin").read_bytes() == (tmp_path / names["ker_f"]).read_bytes()
+    assert (tmp_path / "matrix_bycol.mm2.info").read_text() == (tmp_path / names["ker_i"]).read_text()
