@@ -19,6 +19,7 @@ import ctypes as C
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 
@@ -128,6 +129,52 @@ def read_traffic(workload):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "plsqr3_ref_fp64")
+MPIEXEC = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
+
+
+def cpu_baseline_reference(P, s, iters_to_tol, sample_iters):
+    """The reference itself (oracle/_ref: the PLSQR3 sources compiled here at
+    -O3, FP64 configuration) on one MPI rank: the system written in its file
+    formats, a run of sample_iters iterations, and its own "Parallel LSQR
+    time" (which excludes loading), scaled to the GPU solve's iteration
+    count.  None when the binary or mpiexec is missing or the run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    if not (os.path.exists(REF_BIN) and os.path.exists(MPIEXEC)):
+        return None
+    work = tempfile.mkdtemp(prefix="plsqr_ref_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        t0 = time.perf_counter()
+        names = s.write_reference_files(work)
+        write_s = time.perf_counter() - t0
+        cmd = [MPIEXEC, "-np", "1", REF_BIN, "-dir", work]
+        for key, flag in (("ker_f", "-ker_f"), ("ker_i", "-ker_i"), ("damp_f", "-damp_f"), ("damp_i", "-damp_i"),
+                          ("damp_f_bycol", "-damp_f_bycol"), ("damp_i_bycol", "-damp_i_bycol"), ("b_k", "-b_k")):
+            cmd += [flag, names[key]]
+        cmd += ["-row_k", str(s.m_k), "-row_d", str(s.m_d), "-col", str(s.n), "-itn", str(sample_iters)]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        t = re.findall(r"Parallel LSQR time=(\S+) s", r.stdout)
+        if r.returncode != 0 or not t:
+            print(f"[bench] reference run failed ({r.returncode}); using the port", file=sys.stderr, flush=True)
+            return None
+        lsqr_s = float(t[-1])
+    except (OSError, subprocess.SubprocessError) as e:
+        print(f"[bench] reference run failed ({e}); using the port", file=sys.stderr, flush=True)
+        return None
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    per_it = lsqr_s / (sample_iters + 0.5)
+    return {"value": per_it * (iters_to_tol + 0.5), "unit": "s", "cores": 1, "kind": "reference",
+            "sample": f"PLSQR3 itself (compiled from its sources at -O3, FP64, mpiexec -np 1) on the same system "
+                      f"from its files: {sample_iters} iterations, Parallel LSQR time {lsqr_s:.1f} s "
+                      f"({per_it:.3f} s/iteration; files written in {write_s:.0f} s, run {wall:.0f} s incl. "
+                      f"loading), scaled to {iters_to_tol} iterations"}
 
 
 def cpu_baseline(P, s, iters_to_tol, sample_iters):
@@ -278,7 +325,13 @@ def main():
                     help="mixed precision: inner FP32 iterations per FP64 residual refinement")
     ap.add_argument("--max-iters", type=int, default=5000)
     ap.add_argument("--aprod-reps", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-iters", type=int, default=40,
+                    help="iterations of the oracle (C port) timed for the CPU baseline")
+    ap.add_argument("--cpu-kind", default="auto", choices=["auto", "reference", "port"],
+                    help="CPU baseline: the reference compiled from its sources (oracle/_ref), the port, "
+                         "or the reference when it is there")
+    ap.add_argument("--ref-iters", type=int, default=40,
+                    help="iterations of the reference run timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--aprod-only", action="store_true",
                     help="only time aprod launches and a device copy of known size "
@@ -414,7 +467,10 @@ def main():
                 "algorithmic_bytes": dom_b, "avg_ms": round(dom_ms, 4)}
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(P, s, iters, a.cpu_iters)
+        if a.cpu_kind in ("auto", "reference") and not a.exact_order:
+            cpu = cpu_baseline_reference(P, s, iters, a.ref_iters)
+        if cpu is None and a.cpu_kind != "reference":
+            cpu = cpu_baseline(P, s, iters, a.cpu_iters)
     out = {
         "metric": METRIC, "value": round(wall / a.steps, 6) if reached else None, "unit": "s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 3),
