@@ -245,6 +245,7 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
     plan->cch_col.clear();
     plan->cch_prob.clear();
     int mx = 0;
+    std::vector<int> rch_slice;  // slice of every row chunk (-1: damping rows)
     for (int p = 0; p < P; ++p) {
         const ProblemLayout &L = plan->prob[(size_t)p];
         const long long r0 = plan->row0[(size_t)p], c0 = plan->col0[(size_t)p];
@@ -270,15 +271,24 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         plan->pofs[(size_t)p + 1] = plan->pofs[(size_t)p] + (long long)L.S * L.n;
         plan->nslice[(size_t)p] = L.S;
         mx = std::max(mx, L.tiles.max_seg_per_item);
-        for (int r = 0; r < L.m_k; r += row_chunk) {
-            plan->rch_row.push_back((int)(r0 + r));
-            plan->rch_prob.push_back(p);
-            plan->rch_kind.push_back(0);
+        // kernel-row chunks inside the row slices, so that every chunk's rays
+        // belong to one slice (whose XCD holds their segment partials)
+        {
+            std::vector<int> sb = L.tiles.slice_bounds;
+            if (sb.size() < 2) sb = {0, L.m_k};
+            for (size_t q = 0; q + 1 < sb.size(); ++q)
+                for (int r = sb[q]; r < sb[q + 1]; r += row_chunk) {
+                    plan->rch_row.push_back((int)(r0 + r));
+                    plan->rch_prob.push_back(p);
+                    plan->rch_kind.push_back(0);
+                    rch_slice.push_back((int)q);
+                }
         }
         for (int r = 0; r < L.m_d; r += damp_chunk) {
             plan->rch_row.push_back((int)(r0 + L.m_k + r));
             plan->rch_prob.push_back(p);
             plan->rch_kind.push_back(1);
+            rch_slice.push_back(-1);
         }
         plan->rch0[(size_t)p + 1] = (int)plan->rch_prob.size();
         for (int c = 0; c < L.n; c += col_chunk) {
@@ -322,6 +332,38 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         plan->blk_item.assign(len * X, -1);
         for (int x = 0; x < X; ++x)
             for (size_t i = 0; i < queue[(size_t)x].size(); ++i) plan->blk_item[i * X + x] = queue[(size_t)x][i];
+        // row chunks: a slice's kernel-row chunks to its XCD's queue, damping
+        // chunks to the queue with the fewest rows
+        std::vector<std::vector<int>> xcd_of((size_t)P);
+        for (int p = 0; p < P; ++p) xcd_of[(size_t)p].assign((size_t)plan->prob[(size_t)p].S, 0);
+        {
+            std::vector<long long> ld((size_t)X, 0);
+            for (const Sl &q : sl) {
+                const int x = (int)(std::min_element(ld.begin(), ld.end()) - ld.begin());
+                ld[(size_t)x] += q.work;
+                xcd_of[(size_t)q.p][(size_t)q.s] = x;
+            }
+        }
+        std::vector<std::vector<int>> rq((size_t)X);
+        std::vector<long long> rows_in((size_t)X, 0);
+        const int nch = (int)plan->rch_prob.size();
+        for (int ch = 0; ch < nch; ++ch) {
+            const long long nr = (ch + 1 < nch ? plan->rch_row[(size_t)ch + 1]
+                                               : plan->row0[(size_t)plan->rch_prob[(size_t)ch] + 1]) -
+                                 plan->rch_row[(size_t)ch];
+            int x;
+            if (rch_slice[(size_t)ch] >= 0)
+                x = xcd_of[(size_t)plan->rch_prob[(size_t)ch]][(size_t)rch_slice[(size_t)ch]];
+            else
+                x = (int)(std::min_element(rows_in.begin(), rows_in.end()) - rows_in.begin());
+            rq[(size_t)x].push_back(ch);
+            rows_in[(size_t)x] += nr;
+        }
+        size_t rlen = 0;
+        for (const auto &v : rq) rlen = std::max(rlen, v.size());
+        plan->blk_rch.assign(rlen * X, -1);
+        for (int x = 0; x < X; ++x)
+            for (size_t i = 0; i < rq[(size_t)x].size(); ++i) plan->blk_rch[i * X + x] = rq[(size_t)x][i];
     }
     plan->rows = plan->row0[(size_t)P];
     plan->cols = plan->col0[(size_t)P];

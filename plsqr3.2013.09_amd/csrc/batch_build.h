@@ -54,6 +54,9 @@ struct BatchPlan {
     // and its items are queued there together, so the slice's rows of u
     // stay in that XCD's L2 while its items run.
     std::vector<int> blk_item;
+    // mode-1 per-ray sums: workgroup b runs row chunk blk_rch[b] (-1: none),
+    // a slice's kernel-row chunks on the XCD that ran the slice's items
+    std::vector<int> blk_rch;
     // pieces, global numbering
     std::vector<int> rch_row, rch_prob;  // row chunks: rch_row has count + 1 entries
     std::vector<uint8_t> rch_kind;       // 0 kernel rows, 1 damping rows

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, c
 // segment partials added in tile order in FP64, LC lanes per ray); damping
 // rows as kb_rows.  part[ch*K + k] as kb_rows.
 template <class T, int K, int LC, int BS>
-__global__ __launch_bounds__(BS) void kb_rows_comb(int nch, const int *__restrict__ ch_row,
+__global__ __launch_bounds__(BS) void kb_rows_comb(const int *__restrict__ blk_ch, const int *__restrict__ ch_row,
                                                    const int *__restrict__ ch_prob, const uint8_t *__restrict__ ch_kind,
                                                    const int *__restrict__ ray_ptr, const int *__restrict__ ray_seg,
                                                    const T *__restrict__ segv, const int *__restrict__ rp,
@@ -336,7 +336,11 @@ __global__ __launch_bounds__(BS) void kb_rows_comb(int nch, const int *__restric
     constexpr int R = BS / LC;
     const int lane = threadIdx.x % LC;
     const int slot = threadIdx.x / LC;
-    for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    // workgroup b takes row chunk blk_ch[b] (-1: none): a slice's kernel-row
+    // chunks run on the XCD whose L2 holds the slice's segment partials
+    {
+        const int ch = blk_ch[blockIdx.x];
+        if (ch < 0) return;
         const int p = ch_prob[ch];
         T s1[K], s2[K];
         bool live[K];
@@ -349,7 +353,7 @@ __global__ __launch_bounds__(BS) void kb_rows_comb(int nch, const int *__restric
             s2[k] = (T)(-s.alpha);
             any = any || live[k];
         }
-        if (!any) continue;  // uniform across the block
+        if (!any) return;  // uniform across the block
         const int r0 = ch_row[ch], r1 = ch_row[ch + 1];
         double ssq[K];
 #pragma unroll

@@ -59,6 +59,8 @@ struct plsqr_batch {
     int *dcp = nullptr, *dri = nullptr;
     double *dcv = nullptr;
     int *i_prob = nullptr, *i_c0 = nullptr, *i_c1 = nullptr, *blk_item = nullptr, *seg_ray = nullptr;
+    int *blk_rch = nullptr;  // row chunk of each mode-1 per-ray-sum workgroup
+    int nblk_r = 0;
     long long *i_pbase = nullptr;
     long long *slot_base = nullptr, *seg_start = nullptr, *cut_base = nullptr;
     uint16_t *vox_len = nullptr, *sseg = nullptr, *vox_cut = nullptr;
@@ -181,12 +183,13 @@ int rows_k(plsqr_batch *b, const Shape &s, const T *av, const T *xv, T *yv, cons
                            b->i_prob, b->i_c0, b->i_c1, b->seg_start, b->rt_gbase, b->rt_ebase, b->rt_len, b->rt_vox,
                            ev, xv, sc, (T *)b->segv);
         BT_LAUNCH(b);
+        const dim3 gr(b->nblk_r);
         if (s.bs1 == 128)
-            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 128>), g, dim3(128), 0, b->stream, b->nrch, b->rch_row,
+            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 128>), gr, dim3(128), 0, b->stream, b->blk_rch, b->rch_row,
                                b->rch_prob, b->rch_kind, b->rt_rptr, b->rt_rseg, (const T *)b->segv, b->rp,
                                b->ci, av, xv, yv, sc, b->part);
         else
-            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 256>), g, dim3(256), 0, b->stream, b->nrch, b->rch_row,
+            hipLaunchKernelGGL((kb_rows_comb<T, K, 4, 256>), gr, dim3(256), 0, b->stream, b->blk_rch, b->rch_row,
                                b->rch_prob, b->rch_kind, b->rt_rptr, b->rt_rseg, (const T *)b->segv, b->rp,
                                b->ci, av, xv, yv, sc, b->part);
         BT_LAUNCH(b);
@@ -814,6 +817,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
     b->cuts = plan.cuts;
     b->items = plan.items;
     b->nblk = (int)plan.blk_item.size();
+    b->nblk_r = (int)plan.blk_rch.size();
     b->partials = plan.partials;
     b->nrch = (int)plan.rch_prob.size();
     b->ncch = (int)plan.cch_prob.size();
@@ -840,7 +844,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->dri, (size_t)b->dnnz)) || (rc = balloc(b, &b->dcv, (size_t)b->dnnz)) ||
         (rc = balloc(b, &b->i_prob, (size_t)b->items)) || (rc = balloc(b, &b->i_c0, (size_t)b->items)) ||
         (rc = balloc(b, &b->i_c1, (size_t)b->items)) || (rc = balloc(b, &b->i_pbase, (size_t)b->items)) ||
-        (rc = balloc(b, &b->blk_item, (size_t)b->nblk)) || (rc = balloc(b, &b->pp, (size_t)b->partials * K)) ||
+        (rc = balloc(b, &b->blk_item, (size_t)b->nblk)) || (rc = balloc(b, &b->blk_rch, (size_t)b->nblk_r)) || (rc = balloc(b, &b->pp, (size_t)b->partials * K)) ||
         (rc = balloc(b, &b->d_pofs, (size_t)P + 1)) || (rc = balloc(b, &b->d_nslice, (size_t)P)) ||
         (rc = balloc(b, &b->slot_base, (size_t)b->items + 1)) || (rc = balloc(b, &b->seg_start, (size_t)b->items + 1)) ||
         (rc = balloc(b, &b->vox_len, (size_t)b->items * TW)) || (rc = balloc(b, &b->seg_ray, (size_t)b->segs)) ||
@@ -969,6 +973,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = upload(b, b->seg_start + b->items, tail_sg.data(), 1)) ||
         (rc = upload(b, b->cut_base + b->items, tail_cb.data(), 1)) ||
         (rc = upload(b, b->blk_item, plan.blk_item.data(), plan.blk_item.size())) ||
+        (rc = upload(b, b->blk_rch, plan.blk_rch.data(), plan.blk_rch.size())) ||
         (rc = upload(b, b->d_pofs, plan.pofs.data(), plan.pofs.size())) ||
         (rc = upload(b, b->rch_row, plan.rch_row.data(), plan.rch_row.size())) ||
         (rc = upload(b, b->rch_prob, plan.rch_prob.data(), plan.rch_prob.size())) ||
