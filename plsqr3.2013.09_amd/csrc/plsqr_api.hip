@@ -158,13 +158,25 @@ int env_int(const char *name, int dflt) {
 // -------------------------------------------------------------------------
 constexpr int BS = 256;
 
+// Mode-1 pass 1 on the bricked layout (FP64 or the FP32 copy), 512 entries
+// staged per LDS round
+template <class T>
+void launch_pass1(plsqr_op *op, const T *val, const T *x, const Scal *sc, T *segv) {
+    using namespace plsqr;
+    const dim3 g(op->tT * op->tS), b(BS);
+    if constexpr (std::is_same<T, double>::value)
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,
+                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);
+    else
+        hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,
+                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);
+}
+
 int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double *part) {
     using namespace plsqr;
     dim3 g(op->G1), b(BS);
     if (op->rtiled) {
-        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
-                           op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x, sc,
-                           op->rt_segv);
+        launch_pass1<double>(op, op->rt_val, x, sc, op->rt_segv);
         LAUNCH_CHECK(op);
         switch (op->rt_lc) {
             case 1: hipLaunchKernelGGL((k_rows_tiled_combine<1, BS>), g, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, y, sc, part); break;
@@ -205,9 +217,7 @@ constexpr int TILE_CAP = 2048;
 int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
     using namespace plsqr;
     if (op->rtiled) {
-        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->tS,
-                           op->t_start, op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val, x,
-                           (const Scal *)nullptr, op->rt_segv);
+        launch_pass1<double>(op, op->rt_val, x, (const Scal *)nullptr, op->rt_segv);
         hipLaunchKernelGGL((k_rows_tiled_residual<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
                            op->tS, op->bounds, op->rt_rptr, op->rt_rseg, op->rt_segv, op->rp, op->ci, op->rv, x, b, r);
     } else {
@@ -935,9 +945,7 @@ static int aprod32_internal(plsqr_op *op, int mode) {
     const Scal *sp = op->sc_plain;
     if (mode == 1) {
         if (op->rtiled) {
-            hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->tS,
-                               op->t_start, op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox,
-                               op->rt_val32, op->v32, sp, (float *)op->rt_segv);
+            launch_pass1<float>(op, op->rt_val32, op->v32, sp, (float *)op->rt_segv);
             hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
                                op->tS, op->bounds, op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp,
                                op->ci, op->rv32, op->v32, op->u32, sp, op->part);
@@ -1335,9 +1343,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     using namespace plsqr;
     dim3 g1(op->G1), b(BS);
     if (op->rtiled) {
-        hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), dim3(op->tT * op->tS), b, 0, op->stream, op->tS, op->t_start,
-                           op->t_seg, op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, op->rt_val32, op->v32,
-                           (const Scal *)op->sc, (float *)op->rt_segv);
+        launch_pass1<float>(op, op->rt_val32, op->v32, (const Scal *)op->sc, (float *)op->rt_segv);
         hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds,
                            op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp, op->ci, op->rv32, op->v32,
                            op->u32, (const Scal *)op->sc, op->part);

@@ -260,7 +260,6 @@ __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, c
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(CH % G == 0, "whole wavefront copy steps");
     __shared__ T xl[128 * K];
     __shared__ __align__(16) T lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
@@ -297,7 +296,7 @@ __global__ __launch_bounds__(BS) void kb_seg(const int *__restrict__ blk_item, c
         for (int k = 0; k < K; ++k) acc[k].zero();
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
-            const int sh = stage_entries<T, G>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
+            const int sh = stage_entries<T, G, CH>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

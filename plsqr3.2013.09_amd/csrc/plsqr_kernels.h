@@ -86,23 +86,38 @@ __device__ double reduce_partials(const double *p, int count, int stride, double
 // computed in twice the working precision, at two FP64 operations per term.
 __device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc += (double)a * (double)x; }
 
-// Copies the cnt (value, voxel) entries at global index base into a
+// Copies the cnt <= CH (value, voxel) entries at global index base into a
 // wavefront's LDS buffers with 16-byte loads (two FP64 or four FP32 values,
 // and the matching voxel bytes per lane and step).  The copy starts at the
 // 16-byte boundary at or below base, so the caller finds entry i at
 // bv[shift + i], shift = the return value < V; the buffers hold cnt + 2V - 2
 // entries, and the global arrays carry V - 1 entries of slack past their end.
-template <class T, int G>
+// A lane issues all of its (at most K) loads before its first LDS store, so
+// the copy waits for memory once.
+template <class T, int G, int CH>
 __device__ __forceinline__ int stage_entries(const T *__restrict__ val, const uint8_t *__restrict__ vox, long long base,
                                              int cnt, T *bv, uint8_t *bx, int lane) {
     constexpr int V = 16 / (int)sizeof(T);
+    constexpr int K = (CH / V + 1 + G - 1) / G;  // 16-byte steps per lane: cnt <= CH spans <= CH/V + 1
     using VoxV = typename std::conditional<V == 2, uint16_t, uint32_t>::type;
     const long long a0 = base & ~(long long)(V - 1);
     const int sh = (int)(base - a0);
     const int nv = (sh + cnt + V - 1) / V;
-    for (int i = lane; i < nv; i += G) {
-        *reinterpret_cast<uint4 *>(bv + i * V) = *reinterpret_cast<const uint4 *>(val + a0 + (long long)i * V);
-        *reinterpret_cast<VoxV *>(bx + i * V) = *reinterpret_cast<const VoxV *>(vox + a0 + (long long)i * V);
+    uint4 rv[K];
+    VoxV rx[K];
+    // steps past nv (cnt >= 1, so nv >= 1) repeat the last step: the same
+    // bytes loaded and stored again, so no step needs a branch
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = lane + k * G < nv ? lane + k * G : nv - 1;
+        rv[k] = *reinterpret_cast<const uint4 *>(val + a0 + (long long)i * V);
+        rx[k] = *reinterpret_cast<const VoxV *>(vox + a0 + (long long)i * V);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = lane + k * G < nv ? lane + k * G : nv - 1;
+        *reinterpret_cast<uint4 *>(bv + i * V) = rv[k];
+        *reinterpret_cast<VoxV *>(bx + i * V) = rx[k];
     }
     return sh;
 }

@@ -82,7 +82,6 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(CH % G == 0, "whole wavefront copy steps");
     __shared__ float xl[128];
     __shared__ __align__(16) float lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
@@ -113,7 +112,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
         double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
-            const int sh = stage_entries<float, G>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
+            const int sh = stage_entries<float, G, CH>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -129,7 +129,6 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(CH % G == 0, "whole wavefront copy steps");
     __shared__ double xl[128];
     __shared__ __align__(16) double lv[NWV][CH + 4];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 4];
@@ -160,7 +159,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
         double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
             const int cnt = tot - q0 < CH ? tot - q0 : CH;
-            const int sh = stage_entries<double, G>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
+            const int sh = stage_entries<double, G, CH>(ent_val, ent_vox, gb + q0, cnt, bv, bx, lane) - q0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -340,6 +339,7 @@ __device__ __forceinline__ void cols_tiled_item(
     double *__restrict__ P, double *g, double (*part)[TW]) {
     static_assert(CAP % CUT == 0, "rounds end on round cuts");
     constexpr int NW = BS / TW;
+    constexpr int U = 4;  // slots per thread in flight (8 measured no faster, 2 slower)
     const int s = it % S;
     const int t = it / S;
     const int c0 = tile_start[t];
@@ -379,16 +379,16 @@ __device__ __forceinline__ void cols_tiled_item(
         __syncthreads();
         const double *sv = sval + base + v;
         const uint16_t *ss = sseg + base + v;
-        for (; j + 3 * NW < len; j += 4 * NW) {
-            const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
-            const int g0 = ld_stream<NT>(ss + o0), g1 = ld_stream<NT>(ss + o1), g2 = ld_stream<NT>(ss + o2),
-                      g3 = ld_stream<NT>(ss + o3);
-            const double a0 = ld_stream<NT>(sv + o0), a1 = ld_stream<NT>(sv + o1), a2 = ld_stream<NT>(sv + o2),
-                         a3 = ld_stream<NT>(sv + o3);
-            acc += a0 * g[g0];
-            acc += a1 * g[g1];
-            acc += a2 * g[g2];
-            acc += a3 * g[g3];
+        for (; j + (U - 1) * NW < len; j += U * NW) {
+            const long long o = (long long)j * TW;
+            int gi[U];
+            double a[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o + q * NW * TW);
+#pragma unroll
+            for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o + q * NW * TW);
+#pragma unroll
+            for (int q = 0; q < U; ++q) acc += a[q] * g[gi[q]];
         }
         for (; j < len; j += NW) {
             const long long o = (long long)j * TW;
@@ -425,16 +425,16 @@ __device__ __forceinline__ void cols_tiled_item(
             if (hi < ns && v < tw) jend = (int)vox_cut[(size_t)(cut_base[it] + hi / CUT - 1) * TW + v];
             const double *sv = sval + base + v;
             const uint16_t *ss = sseg + base + v;
-            for (; j + 3 * NW < jend; j += 4 * NW) {
-                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
-                const int g0 = ld_stream<NT>(ss + o0) - r0, g1 = ld_stream<NT>(ss + o1) - r0,
-                          g2 = ld_stream<NT>(ss + o2) - r0, g3 = ld_stream<NT>(ss + o3) - r0;
-                const double a0 = ld_stream<NT>(sv + o0), a1 = ld_stream<NT>(sv + o1), a2 = ld_stream<NT>(sv + o2),
-                             a3 = ld_stream<NT>(sv + o3);
-                acc += a0 * g[g0];
-                acc += a1 * g[g1];
-                acc += a2 * g[g2];
-                acc += a3 * g[g3];
+            for (; j + (U - 1) * NW < jend; j += U * NW) {
+                const long long o = (long long)j * TW;
+                int gi[U];
+                double a[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o + q * NW * TW) - r0;
+#pragma unroll
+                for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o + q * NW * TW);
+#pragma unroll
+                for (int q = 0; q < U; ++q) acc += a[q] * g[gi[q]];
             }
             for (; j < jend; j += NW) {
                 const long long o = (long long)j * TW;
