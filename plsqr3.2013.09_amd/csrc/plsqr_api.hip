@@ -67,6 +67,7 @@ struct plsqr_op {
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
     bool nt_loads = false;
+    int tw = 128;  // voxels per tile (brick) of the tiled layouts: 128 or 256
     bool rows_nt = false;  // streaming loads for the mode-1 matrix data
     // mode 1 on the bricked layout (segment partials + per-ray combine)
     bool rtiled = false;
@@ -158,18 +159,26 @@ int env_int(const char *name, int dflt) {
 // -------------------------------------------------------------------------
 constexpr int BS = 256;
 
-// Mode-1 pass 1 on the bricked layout (FP64 or the FP32 copy), 512 entries
-// staged per LDS round
+// Mode-1 pass 1 on the bricked layout (FP64 or the FP32 copy).  Entries are
+// staged 512 per LDS round on 128-voxel tiles and 448 on 256-voxel tiles, so
+// a workgroup's LDS stays under 20 KB (8 workgroups per CU).
 template <class T>
 void launch_pass1(plsqr_op *op, const T *val, const T *x, const Scal *sc, T *segv) {
     using namespace plsqr;
     const dim3 g(op->tT * op->tS), b(BS);
-    if constexpr (std::is_same<T, double>::value)
-        hipLaunchKernelGGL((k_rows_tiled_c<BS, 512>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,
+#define PASS1(CH, TW)                                                                                                 \
+    if constexpr (std::is_same<T, double>::value)                                                                     \
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,         \
+                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);                     \
+    else                                                                                                              \
+        hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,       \
                            op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);
-    else
-        hipLaunchKernelGGL((k_rows_tiled_c32<BS, 512>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,
-                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);
+    if (op->tw == 256) {
+        PASS1(448, 256)
+    } else {
+        PASS1(512, 128)
+    }
+#undef PASS1
 }
 
 int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double *part) {
@@ -211,7 +220,6 @@ int launch_rows(plsqr_op *op, const double *x, double *y, const Scal *sc, double
 }
 
 constexpr int TILE_W = 128;
-constexpr int TILE_CAP = 2048;
 
 // r = [b;0] - A x in the device order (x: n, b: m_k kernel rows, r: m)
 int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
@@ -233,15 +241,20 @@ int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
 int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
     using namespace plsqr;
     if (op->tiled) {
-        if (op->nt_loads) {
-            hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, true>), dim3(op->tT * op->tS), dim3(BS), 0,
-                               op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
-                               op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
+#define COLS(TW, B, CAP, NT)                                                                                          \
+    hipLaunchKernelGGL((k_cols_tiled<TW, B, CAP, NT>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n, op->tS,   \
+                       op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval, op->t_sseg, op->t_cut,  \
+                       op->t_vcut, u, sc, op->P);
+        // 4 threads per voxel; LDS for 8192 (4096) gathered segments, so few
+        // items need a second round
+        if (op->tw == 256) {
+            COLS(256, 1024, 8192, false)
+        } else if (op->nt_loads) {
+            COLS(TILE_W, 512, 4096, true)
         } else {
-            hipLaunchKernelGGL((k_cols_tiled<TILE_W, BS, TILE_CAP, false>), dim3(op->tT * op->tS), dim3(BS), 0,
-                               op->stream, op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray,
-                               op->t_sval, op->t_sseg, op->t_cut, op->t_vcut, u, sc, op->P);
+            COLS(TILE_W, 512, 4096, false)
         }
+#undef COLS
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
@@ -554,7 +567,9 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
     }
     // ---- tiled mode-2 layout
     if (L) {
+        if (L->TW != 128 && L->TW != 256) return set_err(op, PLSQR_ERR_STATE, "tile width must be 128 or 256");
         op->tiled = true;
+        op->tw = L->TW;
         op->tS = L->S;
         op->tT = L->T;
         op->nsl = L->S;
@@ -765,10 +780,15 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     std::unique_ptr<plsqr::RowTiles> R;
     std::unique_ptr<plsqr::RaySorted> rays;
     const plsqr_system *sys = s;  // s, or its ray-sorted renumbering
+    int tw = TILE_W;
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
         std::vector<int> perm, tiles;
         if (grid) {
-            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 4, 8, env_int("PLSQR_BRICKS", 1), perm, tiles);
+            // bricks of 4 x 8 x 8 voxels (PLSQR_BRICK=128: 4 x 4 x 8)
+            const bool big = env_int("PLSQR_BRICK", 256) != 128;
+            tw = big ? 256 : TILE_W;
+            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, big ? 8 : 4, 8, env_int("PLSQR_BRICKS", 1), perm,
+                               tiles);
         } else {
             perm.resize((size_t)s->n);
             for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
@@ -785,7 +805,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         }
         L.reset(new plsqr::TileLayout());
         std::string why;
-        if (!plsqr::build_tile_layout(sys, perm, tiles, hb, TILE_W, L.get(), &why)) {
+        if (!plsqr::build_tile_layout(sys, perm, tiles, hb, tw, L.get(), &why)) {
             L.reset();  // falls back to the sliced CSC
         } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(sys, *L, &why)) {
             return fail(set_err(op, PLSQR_ERR_STATE, why));
@@ -939,6 +959,21 @@ int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
 
 static int ensure_fp32(plsqr_op *op);
 
+// FP32 mode 2 on the tiled layout (partials into op->P)
+static void launch_cols_tiled32(plsqr_op *op, const Scal *sc) {
+    using namespace plsqr;
+#define COLS32(TW, B, CAP)                                                                                            \
+    hipLaunchKernelGGL((k_cols_tiled32<TW, B, CAP>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,            \
+                       op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,   \
+                       op->t_cut, op->t_vcut, op->u32, sc, op->P);
+    if (op->tw == 256) {
+        COLS32(256, 1024, 8192)
+    } else {
+        COLS32(TILE_W, 512, 4096)
+    }
+#undef COLS32
+}
+
 // FP32 aprod on internal-order u32 / v32 with the plain Scal
 static int aprod32_internal(plsqr_op *op, int mode) {
     using namespace plsqr;
@@ -954,9 +989,7 @@ static int aprod32_internal(plsqr_op *op, int mode) {
                                op->ci, op->rv32, op->v32, op->u32, sp, op->part);
         }
     } else {
-        hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
-                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32,
-                           op->t_sseg, op->t_cut, op->t_vcut, op->u32, sp, op->P);
+        launch_cols_tiled32(op, sp);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, op->u32, sp, op->v32, op->part);
     }
@@ -1354,9 +1387,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     }
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
-    hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream, op->n,
-                       op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,
-                       op->t_cut, op->t_vcut, op->u32, (const Scal *)op->sc, op->P);
+    launch_cols_tiled32(op, (const Scal *)op->sc);
     hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
                        op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
     LAUNCH_CHECK(op);
@@ -1470,9 +1501,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         hipLaunchKernelGGL((k_ssq2<BS>), dim3(gs), dim3(BS), 0, op->stream, op->m_k, op->m, op->hy, op->part);
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
-        hipLaunchKernelGGL((k_cols_tiled32<TILE_W, BS, TILE_CAP>), dim3(op->tT * op->tS), dim3(BS), 0, op->stream,
-                           op->n, op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32,
-                           op->t_sseg, op->t_cut, op->t_vcut, op->u32, (const Scal *)op->sc, op->P);
+        launch_cols_tiled32(op, (const Scal *)op->sc);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
         LAUNCH_CHECK(op);

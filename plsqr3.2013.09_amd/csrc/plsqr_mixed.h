@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
 // Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
 // k_rows_tiled_c): FP32 entries and x tile, FP64 sum per segment, the
 // partial s + c rounded once to FP32.
-template <int BS, int CH>
+template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
                                                        const long long *__restrict__ seg_start,
                                                        const int *__restrict__ grp_base,
@@ -82,7 +82,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    __shared__ float xl[128];
+    static_assert(BS >= TW, "one thread per voxel loads the tile of x");
+    __shared__ float xl[TW];
     __shared__ __align__(16) float lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
     if (sc->done) return;

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 }
 
 // Mode 1 on the bricked layout, pass 1: one workgroup per (tile, row slice)
-// item stages the tile of x (at most 128 values) in LDS.  The item's
+// item stages the tile of x (at most TW values) in LDS.  The item's
 // segments (one ray's entries in the tile) are in ray order, each segment's
 // entries stored together in ascending voxel order.  A wavefront takes a group of 64
 // segments (one per lane), copies the group's entries into its own LDS
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 // of its segment found in the buffer (its first entry from a wavefront
 // prefix sum of the lane lengths), in entry order.  No padding is stored.
 // The partial goes to segv[seg_start[item] + segment index].
-template <int BS, int CH>
+template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
                                                      const long long *__restrict__ seg_start,
                                                      const int *__restrict__ grp_base,
@@ -129,7 +129,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    __shared__ double xl[128];
+    static_assert(BS >= TW, "one thread per voxel loads the tile of x");
+    __shared__ double xl[TW];
     __shared__ __align__(16) double lv[NWV][CH + 4];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 4];
     if (sc && sc->done) return;

@@ -1,7 +1,8 @@
 // Host-side construction of the tiled mode-2 layout (internal to libplsqr_hip.so).
 //
 // Columns are renumbered so that a tile of consecutive new column indices is
-// a compact brick of voxels (4 x 4 x 8 when the grid is known).  For every
+// a compact brick of voxels (4 x 8 x 8 when the grid is known; the batched
+// operator uses 4 x 4 x 8).  For every
 // (row slice s, tile t) "item" the kernel gathers u once per ray that crosses
 // the tile (a segment) and reuses it for every voxel of the tile the ray
 // touches.

@@ -85,6 +85,38 @@ def test_aprod_f32_matches_oracle(P, O, MG, gpu_available, name):
         assert e.value.code == 6
 
 
+def test_aprod_mode2_rounds_match_oracle(P, O, gpu_available):
+    """An item (brick x row slice) crossed by more rays than the mode-2
+    kernels stage in LDS at once (8192 segments on 4 x 8 x 8 bricks) is
+    summed in rounds.  320,000 rays on a 16^3 grid put the average item above
+    that; FP64 and FP32 operators against the oracle."""
+    s = P.generate(nx=16, ny=16, nz=16, n_rays=320000, geometry=0, seed=11, sigma=0.01, lam=0.1)
+    cols = np.repeat(np.arange(s.n), np.diff(s.k_col_ptr))
+    x_, y_, z_ = cols % 16, (cols // 16) % 16, cols // 256
+    brick = (z_ // 8) * 8 + (y_ // 8) * 4 + x_ // 4          # 4 x 2 x 2 bricks of 4 x 8 x 8
+    pairs = np.unique(s.k_row_idx.astype(np.int64) * 16 + brick).size
+    assert pairs / (16 * 8) > 8192, pairs
+    announce(f"aprod rounds: {pairs} segments over 128 items")
+    orc = O.Oracle(s)
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    with P.Operator(s) as op:
+        for mode in (1, 2):
+            xg, yg = x.copy(), y.copy()
+            op.aprod(mode, xg, yg)
+            xo, yo = x.copy(), y.copy()
+            orc.aprod(mode, xo, yo)
+            got, want = (yg, yo) if mode == 1 else (xg, xo)
+            assert rel(got, want) < 1e-13, (mode, rel(got, want))
+            x32, y32 = x.astype(np.float32), y.astype(np.float32)
+            op.aprod_f32(mode, x32, y32)
+            xo, yo = x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)
+            orc.aprod(mode, xo, yo)
+            got, want = (y32, yo) if mode == 1 else (x32, xo)
+            assert rel(got.astype(np.float64), want) < 1e-6, (mode, rel(got.astype(np.float64), want))
+
+
 @pytest.mark.parametrize("name", SMALL)
 def test_standard_errors_match_oracle(P, O, MG, gpu_available, name):
     """plsqr_lsqr_solve_se against the oracle's wantse branch: bitwise on the
