@@ -415,6 +415,8 @@ typedef struct plsqr_batch_tuning {
     int    block_cols;      /* mode 2 workgroup size: 128, 256 or 512          */
     double ms_rows, ms_cols; /* measured time per launch of the chosen shapes  */
     int    candidates;      /* shapes timed (0: defaults, not measured)        */
+    int    gather_kib;      /* mode 2: LDS per workgroup for the gathered ray
+                               values, 32 or 64 KiB (0 in set_tuning: 32)     */
 } plsqr_batch_tuning;
 
 /* systems[p]: the p-th system in the loaded layout (b and x_true unused).

@@ -332,17 +332,19 @@ class Batch:
         return t.as_dict()
 
     def set_tuning(self, precision=PREC_FP64, rows_per_wave=4, nnz_per_lane=4, block_rows=256,
-                   block_cols=256):
+                   block_cols=256, gather_kib=32):
         t = BatchTuningC()
-        t.rows_per_wave, t.nnz_per_lane, t.block_rows, t.block_cols = (
-            int(rows_per_wave), int(nnz_per_lane), int(block_rows), int(block_cols))
+        t.rows_per_wave, t.nnz_per_lane, t.block_rows, t.block_cols, t.gather_kib = (
+            int(rows_per_wave), int(nnz_per_lane), int(block_rows), int(block_cols), int(gather_kib))
         self._check(self.lib.plsqr_batch_set_tuning(self.h, int(precision), C.byref(t)))
 
 
 # launch shapes the batch autotuner chooses among
 BATCH_SHAPES = [dict(rows_per_wave=r, nnz_per_lane=u, block_rows=b, block_cols=256)
                 for r in (16, 8, 4, 2) for b in (128, 256) for u in (2, 4)] + \
-               [dict(rows_per_wave=4, nnz_per_lane=4, block_rows=256, block_cols=c) for c in (128, 512)]
+               [dict(rows_per_wave=4, nnz_per_lane=4, block_rows=256, block_cols=c) for c in (128, 512)] + \
+               [dict(rows_per_wave=4, nnz_per_lane=4, block_rows=256, block_cols=c, gather_kib=64)
+                for c in (128, 256, 512)]
 
 
 class WarmStartNet:

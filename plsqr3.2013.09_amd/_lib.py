@@ -92,7 +92,7 @@ class BatchTuningC(C.Structure):
     _fields_ = [
         ("rows_per_wave", C.c_int), ("nnz_per_lane", C.c_int), ("block_rows", C.c_int),
         ("block_cols", C.c_int), ("ms_rows", C.c_double), ("ms_cols", C.c_double),
-        ("candidates", C.c_int),
+        ("candidates", C.c_int), ("gather_kib", C.c_int),
     ]
 
     def as_dict(self):

@@ -25,11 +25,10 @@ constexpr int TW = 128;         // voxels per tile (one 4 x 4 x 8 brick)
 constexpr int ROW_CHUNK = 512;  // kernel rows per mode-1 piece
 constexpr int DAMP_CHUNK = 2048;
 constexpr int COL_CHUNK = 1024;
-constexpr int LDS_GATHER = 32768;  // bytes of LDS for the gathered ray values of a tile
 constexpr int SEG_CH = 512;        // mode-1 segment entries staged per wavefront copy
 
 struct Shape {
-    int L = 16, bs1 = 256, U = 4, bs2 = 256;
+    int L = 16, bs1 = 256, U = 4, bs2 = 256, kib2 = 32;
     double ms1 = 0.0, ms2 = 0.0;
     int candidates = 0;
 };
@@ -210,19 +209,22 @@ int rows_k(plsqr_batch *b, const Shape &s, const T *av, const T *xv, T *yv, cons
     return set_err(b, PLSQR_ERR_ARG, "unsupported mode-1 launch shape");
 }
 
+// kib2 KiB of LDS hold CAP = kib2 * 1024 / (K * sizeof(T)) gathered
+// segments; items with more are staged in rounds
 template <class T, int K>
 int cols_k(plsqr_batch *b, const Shape &s, const T *sv, const T *uv, T *vv, const Scal *sc) {
-    constexpr int CAP = LDS_GATHER / (K * (int)sizeof(T));
     const dim3 g(b->nblk);
     bool done = false;
-#define COLS_CASE(BS_)                                                                                          \
-    if (s.bs2 == BS_) {                                                                                        \
+#define COLS_CASE(BS_, KIB_)                                                                                     \
+    if (s.bs2 == BS_ && s.kib2 == KIB_) {                                                                       \
+        constexpr int CAP = KIB_ * 1024 / (K * (int)sizeof(T));                                                 \
         hipLaunchKernelGGL((kb_cols<T, K, TW, BS_, CAP, plsqr::TileLayout::CUT>), g, dim3(BS_), 0, b->stream,     \
                            b->blk_item, b->i_prob, b->i_c0, b->i_c1, b->i_pbase, b->slot_base, b->seg_start,       \
                            b->vox_len, b->seg_ray, sv, b->sseg, b->cut_base, b->vox_cut, uv, sc, b->pp);          \
         done = true;                                                                                           \
     }
-    COLS_CASE(128) COLS_CASE(256) COLS_CASE(512)
+    COLS_CASE(128, 32) COLS_CASE(256, 32) COLS_CASE(512, 32)
+    COLS_CASE(128, 64) COLS_CASE(256, 64) COLS_CASE(512, 64)
 #undef COLS_CASE
     if (!done) return set_err(b, PLSQR_ERR_ARG, "unsupported mode-2 launch shape");
     BT_LAUNCH(b);
@@ -717,10 +719,12 @@ int shape_from_tuning(const plsqr_batch_tuning *t, Shape *s) {
     if (t->nnz_per_lane != 2 && t->nnz_per_lane != 4) return PLSQR_ERR_ARG;
     if (t->block_rows != 128 && t->block_rows != 256) return PLSQR_ERR_ARG;
     if (t->block_cols != 128 && t->block_cols != 256 && t->block_cols != 512) return PLSQR_ERR_ARG;
+    if (t->gather_kib != 0 && t->gather_kib != 32 && t->gather_kib != 64) return PLSQR_ERR_ARG;
     s->L = 64 / rpw;
     s->U = t->nnz_per_lane;
     s->bs1 = t->block_rows;
     s->bs2 = t->block_cols;
+    s->kib2 = t->gather_kib ? t->gather_kib : 32;
     return PLSQR_OK;
 }
 
@@ -732,6 +736,7 @@ void tuning_from_shape(const Shape &s, plsqr_batch_tuning *t) {
     t->ms_rows = s.ms1;
     t->ms_cols = s.ms2;
     t->candidates = s.candidates;
+    t->gather_kib = s.kib2;
 }
 
 // median time per launch of fn over reps launches (after one warm-up)
@@ -780,18 +785,22 @@ int autotune(plsqr_batch *b, int prec, int reps, const T *av, const T *sv, T *u,
                     best.U = U;
                 }
             }
-    for (int bs : {128, 256, 512}) {
-        Shape s = best;
-        s.bs2 = bs;
-        double ms;
-        if ((rc = fill<T>(b, v, b->cols * b->K, T(1)))) return rc;
-        if ((rc = time_launches(b, reps, [&] { return launch_cols<T>(b, s, sv, u, v, b->sc_plain); }, &ms))) return rc;
-        ++n;
-        if (ms < best2) {
-            best2 = ms;
-            best.bs2 = bs;
+    for (int bs : {128, 256, 512})
+        for (int kib : {32, 64}) {
+            Shape s = best;
+            s.bs2 = bs;
+            s.kib2 = kib;
+            double ms;
+            if ((rc = fill<T>(b, v, b->cols * b->K, T(1)))) return rc;
+            if ((rc = time_launches(b, reps, [&] { return launch_cols<T>(b, s, sv, u, v, b->sc_plain); }, &ms)))
+                return rc;
+            ++n;
+            if (ms < best2) {
+                best2 = ms;
+                best.bs2 = bs;
+                best.kib2 = kib;
+            }
         }
-    }
     best.ms1 = best1;
     best.ms2 = best2;
     best.candidates = n;
