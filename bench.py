@@ -105,9 +105,15 @@ def aprod_bytes(m_k, m_d, n, nnz_k, nnz_d, s_v=8, s_x=8, s_y=8):
     return b1, b2
 
 
-def time_aprod(hip, op, mode, x_dev, y_dev, reps, f32=False):
-    """Average ms of one aprod launch (FP64, or the mixed solver's FP32 operator)."""
-    call = op.aprod_device_f32 if f32 else op.aprod_device
+def time_aprod(hip, op, mode, x_dev, y_dev, reps, f32=False, internal=True):
+    """Average ms of one aprod launch (FP64, or the mixed solver's FP32
+    operator).  internal: on vectors in the handle's order, i.e. the SpMV
+    kernels alone; otherwise through the caller-order entry point, which adds
+    the vector reordering at the API boundary."""
+    if internal:
+        call = op.aprod_device_internal_f32 if f32 else op.aprod_device_internal
+    else:
+        call = op.aprod_device_f32 if f32 else op.aprod_device
     stream = op.lib.plsqr_op_get_stream(op.h)
     for _ in range(3):
         call(mode, x_dev, y_dev)
@@ -400,8 +406,11 @@ def main():
             print(f"[bench] streaming-sum calibration skipped: {e}", file=sys.stderr)
         ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
         ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
+        cm1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps, internal=False)
+        cm2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps, internal=False)
         b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
-        line = {"aprod_only": True, "mode1_ms": ms1, "mode2_ms": ms2, "mode1_bytes": b1, "mode2_bytes": b2}
+        line = {"aprod_only": True, "mode1_ms": ms1, "mode2_ms": ms2, "mode1_bytes": b1, "mode2_bytes": b2,
+                "mode1_ms_caller_order": cm1, "mode2_ms_caller_order": cm2}
         if not a.exact_order:
             f1 = time_aprod(hip, op, 1, x32_dev, y32_dev, a.aprod_reps, f32=True)
             f2 = time_aprod(hip, op, 2, x32_dev, y32_dev, a.aprod_reps, f32=True)
@@ -431,15 +440,22 @@ def main():
           f"test2={infos[-1]['arnorm']:.3e} (device loop {infos[-1]['elapsed_ms']:.2f} ms)",
           file=sys.stderr, flush=True)
 
-    # aprod kernels, timed with events on the handle's stream
+    # aprod kernels, timed with events on the handle's stream: on vectors in
+    # the handle's order (the SpMV kernels alone, as in the solve loop), and
+    # through the caller-order entry point (plus the reordering at the API
+    # boundary)
     if prec == P.PREC_MIXED:
         # the mixed solve's inner iterations run the FP32 operator
         ms1 = time_aprod(hip, op, 1, x32_dev, y32_dev, a.aprod_reps, f32=True)
         ms2 = time_aprod(hip, op, 2, x32_dev, y32_dev, a.aprod_reps, f32=True)
+        cm1 = time_aprod(hip, op, 1, x32_dev, y32_dev, a.aprod_reps, f32=True, internal=False)
+        cm2 = time_aprod(hip, op, 2, x32_dev, y32_dev, a.aprod_reps, f32=True, internal=False)
         b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d, 4, 4, 4)
     else:
         ms1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps)
         ms2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps)
+        cm1 = time_aprod(hip, op, 1, x_dev, y_dev, a.aprod_reps, internal=False)
+        cm2 = time_aprod(hip, op, 2, x_dev, y_dev, a.aprod_reps, internal=False)
         b1, b2 = aprod_bytes(s.m_k, s.m_d, s.n, s.nnz_k, s.nnz_d)
     gbs1, gbs2 = b1 / ms1 / 1e6, b2 / ms2 / 1e6
     gbs = (b1 + b2) / (ms1 + ms2) / 1e6
@@ -484,6 +500,7 @@ def main():
                    "operator": "exact-order" if a.exact_order else "default"},
         "ms_per_iteration": round(infos[-1]["elapsed_ms"] / max(iters, 1), 4),
         "aprod_gbs": {"mode1": round(gbs1, 1), "mode2": round(gbs2, 1), "both": round(gbs, 1)},
+        "aprod_caller_order_ms": {"mode1": round(cm1, 4), "mode2": round(cm2, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "setup_s": {"generate": round(gen_s, 2), "operator_create": round(setup_s, 2)},

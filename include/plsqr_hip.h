@@ -254,6 +254,21 @@ int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev);
 int plsqr_aprod_f32(plsqr_op *op, int mode, float *x, float *y);
 int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev);
 
+/* The handle's own vector order.  The default operator renumbers the columns
+ * (voxel bricks) and the kernel rows (ray order), and the caller-order entry
+ * points above convert x and y at every call, as the reference's aprod
+ * gathers x into its compressed local numbering at every call
+ * (aprod_impl.c:145-175, vector_comm.c GatherVecXCompressed).  A caller that
+ * keeps its device vectors in the handle's order converts once:
+ *   space 0: column space (n doubles), 1: row space (m_k+m_d doubles);
+ *   the two pointers must differ (PLSQR_ERR_ARG otherwise);
+ * and then calls the _internal aprod, which launches the SpMV kernels only.
+ * All enqueue on the handle's stream and return without synchronising. */
+int plsqr_op_to_internal(plsqr_op *op, int space, const double *ext_dev, double *int_dev);
+int plsqr_op_to_external(plsqr_op *op, int space, const double *int_dev, double *ext_dev);
+int plsqr_aprod_device_internal(plsqr_op *op, int mode, double *x_dev, double *y_dev);
+int plsqr_aprod_device_internal_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident LSQR                                                    */
 /* ---------------------------------------------------------------------- */

@@ -113,6 +113,23 @@ class Operator:
         _check(self.lib.plsqr_aprod_device_f32(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
                self.h)
 
+    def to_internal(self, space, ext_ptr, int_ptr):
+        """Device vector (ints: pointers) from the caller's order to the
+        handle's; space 0: columns (n), 1: rows (m)."""
+        _check(self.lib.plsqr_op_to_internal(self.h, int(space), C.c_void_p(ext_ptr), C.c_void_p(int_ptr)), self.h)
+
+    def to_external(self, space, int_ptr, ext_ptr):
+        _check(self.lib.plsqr_op_to_external(self.h, int(space), C.c_void_p(int_ptr), C.c_void_p(ext_ptr)), self.h)
+
+    def aprod_device_internal(self, mode, x_ptr, y_ptr):
+        """aprod on device vectors in the handle's order: the SpMV kernels only."""
+        _check(self.lib.plsqr_aprod_device_internal(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
+               self.h)
+
+    def aprod_device_internal_f32(self, mode, x_ptr, y_ptr):
+        _check(self.lib.plsqr_aprod_device_internal_f32(self.h, int(mode), C.c_void_p(x_ptr), C.c_void_p(y_ptr)),
+               self.h)
+
     def set_vectors(self, u=None, v=None):
         _check(self.lib.plsqr_op_set_vectors(self.h, dptr(u) if u is not None else None,
                                              dptr(v) if v is not None else None), self.h)

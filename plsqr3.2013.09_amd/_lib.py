@@ -141,6 +141,10 @@ SIGNATURES = {
     "plsqr_lsqr_solve_se": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p, c_double_p,
                                       C.POINTER(SolveOutC)]),
     "plsqr_aprod_device_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "plsqr_op_to_internal": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "plsqr_op_to_external": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "plsqr_aprod_device_internal": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "plsqr_aprod_device_internal_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "plsqr_lsqr_solve": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p,
                                    c_double_p, C.POINTER(SolveOutC), c_double_p]),
     "plsqr_lsqr_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveOptsC),

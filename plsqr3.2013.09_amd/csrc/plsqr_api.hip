@@ -960,12 +960,12 @@ int plsqr_aprod_device(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
 static int ensure_fp32(plsqr_op *op);
 
 // FP32 mode 2 on the tiled layout (partials into op->P)
-static void launch_cols_tiled32(plsqr_op *op, const Scal *sc) {
+static void launch_cols_tiled32(plsqr_op *op, const float *u, const Scal *sc) {
     using namespace plsqr;
 #define COLS32(TW, B, CAP)                                                                                            \
     hipLaunchKernelGGL((k_cols_tiled32<TW, B, CAP>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,            \
                        op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,   \
-                       op->t_cut, op->t_vcut, op->u32, sc, op->P);
+                       op->t_cut, op->t_vcut, u, sc, op->P);
     if (op->tw == 256) {
         COLS32(256, 1024, 8192)
     } else {
@@ -974,24 +974,24 @@ static void launch_cols_tiled32(plsqr_op *op, const Scal *sc) {
 #undef COLS32
 }
 
-// FP32 aprod on internal-order u32 / v32 with the plain Scal
-static int aprod32_internal(plsqr_op *op, int mode) {
+// FP32 aprod on internal-order vectors (x: n, y: m floats) with the plain Scal
+static int aprod32_internal(plsqr_op *op, int mode, float *x, float *y) {
     using namespace plsqr;
     const Scal *sp = op->sc_plain;
     if (mode == 1) {
         if (op->rtiled) {
-            launch_pass1<float>(op, op->rt_val32, op->v32, sp, (float *)op->rt_segv);
+            launch_pass1<float>(op, op->rt_val32, x, sp, (float *)op->rt_segv);
             hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
                                op->tS, op->bounds, op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp,
-                               op->ci, op->rv32, op->v32, op->u32, sp, op->part);
+                               op->ci, op->rv32, x, y, sp, op->part);
         } else {
             hipLaunchKernelGGL((k_rows32<16, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m, op->rp,
-                               op->ci, op->rv32, op->v32, op->u32, sp, op->part);
+                               op->ci, op->rv32, x, y, sp, op->part);
         }
     } else {
-        launch_cols_tiled32(op, sp);
+        launch_cols_tiled32(op, y, sp);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
-                           op->P, op->dcp, op->dri, op->dcv, op->u32, sp, op->v32, op->part);
+                           op->P, op->dcp, op->dri, op->dcv, y, sp, x, op->part);
     }
     LAUNCH_CHECK(op);
     return PLSQR_OK;
@@ -1036,7 +1036,7 @@ int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
         HIP_TRY(op, hipMemcpyAsync(op->u32, y_dev, sizeof(float) * op->m, hipMemcpyDeviceToDevice, op->stream));
     }
     LAUNCH_CHECK(op);
-    if ((rc = aprod32_internal(op, mode))) return rc;
+    if ((rc = aprod32_internal(op, mode, op->v32, op->u32))) return rc;
     // back to the caller's order: only the output moves
     if (mode == 2) {
         if (op->permuted)
@@ -1053,6 +1053,51 @@ int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
     }
     LAUNCH_CHECK(op);
     return PLSQR_OK;
+}
+
+static int check_internal_args(plsqr_op *op, const void *a, const void *b) {
+    if (op->allocs.empty()) return set_err(op, PLSQR_ERR_STATE, "handle failed to initialise");
+    if (!a || !b || a == b) return set_err(op, PLSQR_ERR_ARG, "two distinct device vectors required");
+    return PLSQR_OK;
+}
+
+int plsqr_op_to_internal(plsqr_op *op, int space, const double *ext_dev, double *int_dev) {
+    if (!op) return PLSQR_ERR_ARG;
+    int rc;
+    if ((rc = check_internal_args(op, ext_dev, int_dev))) return rc;
+    if (space == 1) return rows_to_internal(op, ext_dev, int_dev);
+    if (space != 0) return set_err(op, PLSQR_ERR_ARG, "space must be 0 (columns) or 1 (rows)");
+    if (op->permuted) return to_internal(op, ext_dev, int_dev);
+    HIP_TRY(op, hipMemcpyAsync(int_dev, ext_dev, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_op_to_external(plsqr_op *op, int space, const double *int_dev, double *ext_dev) {
+    if (!op) return PLSQR_ERR_ARG;
+    int rc;
+    if ((rc = check_internal_args(op, int_dev, ext_dev))) return rc;
+    if (space == 1) return rows_to_external(op, int_dev, ext_dev);
+    if (space != 0) return set_err(op, PLSQR_ERR_ARG, "space must be 0 (columns) or 1 (rows)");
+    if (op->permuted) return to_external(op, int_dev, ext_dev);
+    HIP_TRY(op, hipMemcpyAsync(ext_dev, int_dev, sizeof(double) * op->n, hipMemcpyDeviceToDevice, op->stream));
+    return PLSQR_OK;
+}
+
+int plsqr_aprod_device_internal(plsqr_op *op, int mode, double *x_dev, double *y_dev) {
+    if (!op) return PLSQR_ERR_ARG;
+    int rc;
+    if ((rc = check_internal_args(op, x_dev, y_dev))) return rc;
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    return mode == 1 ? aprod1_dev(op, x_dev, y_dev) : aprod2_dev(op, x_dev, y_dev);
+}
+
+int plsqr_aprod_device_internal_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
+    if (!op) return PLSQR_ERR_ARG;
+    int rc;
+    if ((rc = check_internal_args(op, x_dev, y_dev))) return rc;
+    if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
+    if ((rc = ensure_aprod32(op))) return rc;
+    return aprod32_internal(op, mode, x_dev, y_dev);
 }
 
 int plsqr_aprod_f32(plsqr_op *op, int mode, float *x, float *y) {
@@ -1387,7 +1432,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     }
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
-    launch_cols_tiled32(op, (const Scal *)op->sc);
+    launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
     hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
                        op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
     LAUNCH_CHECK(op);
@@ -1501,7 +1546,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         hipLaunchKernelGGL((k_ssq2<BS>), dim3(gs), dim3(BS), 0, op->stream, op->m_k, op->m, op->hy, op->part);
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
-        launch_cols_tiled32(op, (const Scal *)op->sc);
+        launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
                            op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
         LAUNCH_CHECK(op);

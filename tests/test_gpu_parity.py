@@ -161,6 +161,76 @@ def test_aprod_entry_points_agree(P, MG, gpu_available):
                 assert np.array_equal(got_x, a_x) and np.array_equal(got_y, a_y)
 
 
+def _hip():
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("plsqr_bench", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.Hip()
+
+
+def test_internal_order_aprod_matches_caller_order(P, MG, gpu_available):
+    """Vectors converted once into the handle's order (plsqr_op_to_internal),
+    the internal-order aprod, and the conversions back give the caller-order
+    aprod's results bitwise: the same kernels, with the reordering moved out.
+    FP64 and FP32; the handle's permutations are read off by converting index
+    vectors."""
+    announce("internal-order aprod")
+    hip = _hip()
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn1"])
+    n, m = s.n, s.m_k + s.m_d
+    rng = np.random.default_rng(29)
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    dx, dy, ix, iy = hip.malloc(8 * n), hip.malloc(8 * m), hip.malloc(8 * n), hip.malloc(8 * m)
+    try:
+        with P.Operator(s) as op:
+            for mode in (1, 2):
+                hip.h2d(dx, x), hip.h2d(dy, y)
+                op.aprod_device(mode, dx, dy)
+                hip.sync()
+                ax, ay = np.empty(n), np.empty(m)
+                hip.d2h(ax, dx), hip.d2h(ay, dy)
+                hip.h2d(dx, x), hip.h2d(dy, y)
+                op.to_internal(0, dx, ix), op.to_internal(1, dy, iy)
+                op.aprod_device_internal(mode, ix, iy)
+                op.to_external(0, ix, dx), op.to_external(1, iy, dy)
+                hip.sync()
+                bx, by = np.empty(n), np.empty(m)
+                hip.d2h(bx, dx), hip.d2h(by, dy)
+                assert np.array_equal(ax, bx) and np.array_equal(ay, by), mode
+            # internal[i] = external[perm[i]]
+            hip.h2d(dx, np.arange(n, dtype=np.float64)), hip.h2d(dy, np.arange(m, dtype=np.float64))
+            op.to_internal(0, dx, ix), op.to_internal(1, dy, iy)
+            hip.sync()
+            pc, pr = np.empty(n), np.empty(m)
+            hip.d2h(pc, ix), hip.d2h(pr, iy)
+            pc, pr = pc.astype(np.int64), pr.astype(np.int64)
+            assert np.array_equal(np.sort(pc), np.arange(n)) and np.array_equal(np.sort(pr), np.arange(m))
+            assert not np.array_equal(pc, np.arange(n))      # the bricks renumber the columns
+            x32, y32 = x.astype(np.float32), y.astype(np.float32)
+            for mode in (1, 2):
+                a_x, a_y = x32.copy(), y32.copy()
+                op.aprod_f32(mode, a_x, a_y)
+                hip.h2d(dx, x32[pc]), hip.h2d(dy, y32[pr])
+                op.aprod_device_internal_f32(mode, dx, dy)
+                hip.sync()
+                bx, by = np.empty(n, np.float32), np.empty(m, np.float32)
+                hip.d2h(bx, dx), hip.d2h(by, dy)
+                cx, cy = np.empty_like(bx), np.empty_like(by)
+                cx[pc], cy[pr] = bx, by
+                assert np.array_equal(cx, a_x) and np.array_equal(cy, a_y), mode
+            for bad in (lambda: op.to_internal(2, dx, ix), lambda: op.to_internal(0, dx, dx),
+                        lambda: op.aprod_device_internal(3, ix, iy), lambda: op.aprod_device_internal(1, ix, ix)):
+                with pytest.raises(P.PlsqrError) as e:
+                    bad()
+                assert e.value.code == 1
+    finally:
+        for p in (dx, dy, ix, iy):
+            hip.free(p)
+
+
 def test_aprod_rejects_bad_arguments(P, MG, gpu_available):
     announce("aprod errors")
     s = MG.make_system(P, MG.CASES["grid4_rays12_itn10"])
