@@ -278,6 +278,15 @@ enum {
                               FP64 scalars and an FP64 residual refinement          */
 };
 
+/* Preconditioning of the mixed-precision solve (no counterpart in the
+ * reference, whose lsqr() always runs on A itself). */
+enum {
+    PLSQR_PRECOND_NONE = 0,      /* LSQR on A                                          */
+    PLSQR_PRECOND_COLSCALE = 1   /* LSQR on A D with D = diag(1 / ||A e_j||) (columns of
+                                    [A_k; damping] scaled to unit 2-norm; zero columns
+                                    keep 1), x = D y; the FP64 checks of test2 stay on A */
+};
+
 typedef struct plsqr_solve_opts {
     int    itn;            /* iteration count (-itn); the loop runs exactly itn
                               iterations when tol == 0, like lsqr.c:833-834      */
@@ -289,6 +298,11 @@ typedef struct plsqr_solve_opts {
     double atol, btol, conlim; /* istop tests (lsqr.c:759-775); reference values
                               1e-10, 1e-10, 100 when all three are 0            */
     int    itnlim;         /* istop=5 threshold; 1000 (lsqr.c:50) when 0          */
+    int    precond;        /* mixed only: PLSQR_PRECOND_*                         */
+    double anorm_ref;      /* mixed with PLSQR_PRECOND_COLSCALE: the ||A|| of the
+                              FP64 test2 = ||A^T r|| / (anorm_ref ||r||); 0 = ||A||_F.
+                              Passing the anorm a plain FP64 solve reported makes
+                              the stop exactly as strict as that solve's         */
 } plsqr_solve_opts;
 
 typedef struct plsqr_solve_out {

@@ -18,7 +18,8 @@ import numpy as np
 from ._lib import (  # noqa: F401
     BATCH_CONFIGS, BATCH_MAX_RHS, BATCH_NO_TUNE, BATCH_TUNE_MIXED, BatchTuningC, GridC, CONFIGS,
     ERR_NAMES, GEOM_OPPOSITE_FACES, GEOM_TOP_SOURCES, LIB_PATH, OP_DEFAULT, OP_EXACT_ORDER,
-    OP_WITH_FP32, PREC_FP64, PREC_MIXED, SIGNATURES, TRACE_FIELDS, TRACE_WIDTH, PlsqrError,
+    OP_WITH_FP32, PREC_FP64, PREC_MIXED, PRECOND_COLSCALE, PRECOND_NONE, SIGNATURES, TRACE_FIELDS,
+    TRACE_WIDTH, PlsqrError,
     SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, fptr, gen_field, gen_rhs, generate,
     kernel_rows_to_csc, ker2plsqr3,
     generate_batch, generate_config, load, read_x, write_x, _check,
@@ -147,16 +148,18 @@ class Operator:
 
     # -- LSQR -------------------------------------------------------------
     @staticmethod
-    def make_opts(itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+    def make_opts(itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0, precond=PRECOND_NONE,
+                  anorm_ref=0.0):
         o = SolveOptsC()
         o.itn, o.tol, o.precision, o.refine_every, o.damp = int(itn), float(tol), int(precision), \
             int(refine_every), float(damp)
         o.atol = o.btol = o.conlim = 0.0
         o.itnlim = 0
+        o.precond, o.anorm_ref = int(precond), float(anorm_ref)
         return o
 
     def lsqr(self, b, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0,
-             trace=False, want_w=False):
+             trace=False, want_w=False, precond=PRECOND_NONE, anorm_ref=0.0):
         """Returns (x, info dict[, trace array (itn+1, 8)])."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         assert b.size == self.m_k
@@ -164,7 +167,7 @@ class Operator:
         w = np.empty(self.n, np.float64) if want_w else None
         out = SolveOutC()
         tr = np.full((itn + 1, TRACE_WIDTH), np.nan) if trace else None
-        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref)
         _check(self.lib.plsqr_lsqr_solve(self.h, dptr(b), C.byref(o), dptr(x),
                                          dptr(w) if w is not None else None, C.byref(out),
                                          tr.ctypes.data_as(C.POINTER(C.c_double))
@@ -186,33 +189,35 @@ class Operator:
         return x, se, out.as_dict()
 
     def lsqr_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
-                    damp=0.0, w_ptr=None):
+                    damp=0.0, w_ptr=None, precond=PRECOND_NONE, anorm_ref=0.0):
         out = SolveOutC()
-        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref)
         _check(self.lib.plsqr_lsqr_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
                                                 C.c_void_p(x_ptr),
                                                 C.c_void_p(w_ptr) if w_ptr else None,
                                                 C.byref(out), None), self.h)
         return out.as_dict()
 
-    def lsqr_x0(self, b, x0, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+    def lsqr_x0(self, b, x0, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0,
+                precond=PRECOND_NONE, anorm_ref=0.0):
         """Warm start from x0 (None = zero start); returns (x, info) with info["used_x0"]."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         x = np.empty(self.n, np.float64)
         out = SolveOutC()
         used = C.c_int(0)
-        o = self.make_opts(itn, tol, precision, refine_every, damp)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref)
         _check(self.lib.plsqr_lsqr_solve_x0(self.h, dptr(b), dptr(x0), C.byref(o), dptr(x), C.byref(out),
                                             C.byref(used)), self.h)
         info = out.as_dict()
         info["used_x0"] = bool(used.value)
         return x, info
 
-    def lsqr_x0_device(self, b_ptr, x0_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25):
+    def lsqr_x0_device(self, b_ptr, x0_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
+                       precond=PRECOND_NONE, anorm_ref=0.0):
         out = SolveOutC()
         used = C.c_int(0)
-        o = self.make_opts(itn, tol, precision, refine_every, 0.0)
+        o = self.make_opts(itn, tol, precision, refine_every, 0.0, precond, anorm_ref)
         _check(self.lib.plsqr_lsqr_solve_x0_device(self.h, C.c_void_p(b_ptr),
                                                    C.c_void_p(x0_ptr) if x0_ptr else None, C.byref(o),
                                                    C.c_void_p(x_ptr), C.byref(out), C.byref(used)), self.h)

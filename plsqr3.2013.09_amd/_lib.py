@@ -30,6 +30,8 @@ OP_WITH_FP32 = 2
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PRECOND_NONE = 0
+PRECOND_COLSCALE = 1
 
 TRACE_WIDTH = 8
 TRACE_FIELDS = ("alpha", "beta", "rnorm", "test1", "test2", "anorm", "acond", "xnorm")
@@ -71,7 +73,7 @@ class SolveOptsC(C.Structure):
         ("itn", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
         ("refine_every", C.c_int), ("damp", C.c_double),
         ("atol", C.c_double), ("btol", C.c_double), ("conlim", C.c_double),
-        ("itnlim", C.c_int),
+        ("itnlim", C.c_int), ("precond", C.c_int), ("anorm_ref", C.c_double),
     ]
 
 

@@ -83,6 +83,11 @@ struct plsqr_op {
     float *fx = nullptr, *fy = nullptr;  // caller-order FP32 staging of the host FP32 aprod
     plsqr::Scal *sc_plain = nullptr;     // ib = 1, alpha = -1, vscale = 1: plain y += A x, x += A^T y
     double *dx = nullptr;
+    // column scaling of the mixed solve: d = 1 / ||A e_j|| (device column
+    // order), ||A||_F, and the scales the FP32 combine applies during a solve
+    double *cscale = nullptr;
+    double fro = 0.0;
+    const double *cs_active = nullptr;
     // warm starts: x0 in the device column order and r0 = [b;0] - A x0
     double *x0i = nullptr, *r0 = nullptr;
     // ray order: kernel rows renumbered so that rays crossing the same tiles
@@ -1241,6 +1246,8 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
                       double *se = nullptr) {
     using namespace plsqr;
     int rc;
+    if (o->precond != PLSQR_PRECOND_NONE)
+        return set_err(op, PLSQR_ERR_ARG, "column scaling is offered on the mixed-precision solve");
     const int itn = std::max(0, o->itn);
     const bool want_trace = trace_host != nullptr;
     if (want_trace && (rc = ensure_trace(op, itn + 1))) return rc;
@@ -1434,7 +1441,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
     launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
     hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
-                       op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
+                       op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part, op->cs_active);
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4);
     hipLaunchKernelGGL((k_update32<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
@@ -1480,6 +1487,50 @@ static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, doubl
     return PLSQR_OK;
 }
 
+// Column scales d_j = 1 / ||A e_j|| of [A_k; D] (device column order) and
+// ||A||_F, once per handle: the tiled mode 2 run on the squared values with
+// u = 1 gives every column's sum of squares in a fixed order.
+static int ensure_colscale(plsqr_op *op) {
+    using namespace plsqr;
+    if (op->cscale) return PLSQR_OK;
+    if (!op->tiled) return set_err(op, PLSQR_ERR_STATE, "column scaling needs the tiled operator");
+    int rc;
+    double *cs = nullptr, *sq_t = nullptr, *sq_d = nullptr, *ones = nullptr;
+    if ((rc = dalloc(op, &cs, op->n)) || (rc = dalloc(op, &sq_t, op->t_nslot)) ||
+        (rc = dalloc(op, &sq_d, op->nnz_d)) || (rc = dalloc(op, &ones, op->m)))
+        return rc;
+    const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
+    hipLaunchKernelGGL(k_square, dim3(grid_for(op->t_nslot, 256, op->cu * 16)), dim3(256), 0, op->stream,
+                       op->t_nslot, (const double *)op->t_sval, sq_t);
+    hipLaunchKernelGGL(k_square, dim3(grid_for(op->nnz_d, 256, op->cu * 16)), dim3(256), 0, op->stream, op->nnz_d,
+                       (const double *)op->dcv, sq_d);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, 1.0, ones);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, cs);
+    LAUNCH_CHECK(op);
+    std::swap(op->t_sval, sq_t);
+    std::swap(op->dcv, sq_d);
+    rc = launch_cols(op, ones, cs, nullptr, nullptr);
+    std::swap(op->t_sval, sq_t);
+    std::swap(op->dcv, sq_d);
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    dfree(op, sq_t, op->t_nslot);
+    dfree(op, sq_d, op->nnz_d);
+    dfree(op, ones, op->m);
+    if (rc) return rc;
+    double fro2 = 0.0;
+    const long long nnz = op->nnz_k + op->nnz_d;
+    const int ga = std::min(op->part_len, grid_for(nnz, BS, op->cu * 4));
+    hipLaunchKernelGGL((k_ssq<BS>), dim3(ga), dim3(BS), 0, op->stream, nnz, (const double *)op->rv, op->part);
+    LAUNCH_CHECK(op);
+    if ((rc = host_sum(op, op->part, ga, &fro2))) return rc;
+    hipLaunchKernelGGL(k_colscale, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n, cs);
+    LAUNCH_CHECK(op);
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    op->fro = std::sqrt(fro2);
+    op->cscale = cs;
+    return PLSQR_OK;
+}
+
 // Mixed precision: an inner LSQR in FP32 (values, u, v; FP64-accumulated dot products; FP64
 // scalars, w and correction dx) solves min ||A dx - r|| for r = [b;0] - A x.
 // Every refine_every inner iterations the FP64 residual of xt = x + dx is
@@ -1489,6 +1540,9 @@ static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, doubl
 // Krylov basis, and is restarted on the refined residual (x = xt) only when
 // its own recurrence stops (its estimate of test2 reached tol) while the FP64
 // test2 has not.  itn bounds the inner iterations in total.
+// With PLSQR_PRECOND_COLSCALE the inner LSQR runs on A D (D = diag of the
+// column scales) and the FP64 test2 uses anorm_ref, or ||A||_F when that is
+// 0, in place of the inner estimate, which is then one of ||A D||.
 // x_init: NULL for x = 0, else the starting x in the device column order.
 static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out,
                        const double *x_init = nullptr) {
@@ -1497,6 +1551,11 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     if ((rc = ensure_fp32(op))) return rc;
     if (!op->x0i && ((rc = dalloc(op, &op->x0i, op->n)) || (rc = dalloc(op, &op->r0, op->m)))) return rc;
     double *xt = op->x0i == x_init ? op->r0 : op->x0i;  // n doubles not aliasing x_init
+    const bool colscale = o->precond == PLSQR_PRECOND_COLSCALE;
+    if (o->precond != PLSQR_PRECOND_NONE && !colscale) return set_err(op, PLSQR_ERR_ARG, "unknown preconditioner");
+    if (colscale && (rc = ensure_colscale(op))) return rc;
+    op->cs_active = colscale ? op->cscale : nullptr;
+    const double anorm_fixed = colscale ? (o->anorm_ref > 0.0 ? o->anorm_ref : op->fro) : 0.0;
     const int itn = std::max(0, o->itn);
     const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
@@ -1548,7 +1607,8 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
         launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
-                           op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part);
+                           op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part,
+                           op->cs_active);
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4,
                            (double *)nullptr);
@@ -1590,7 +1650,8 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
             LAUNCH_CHECK(op);
             if ((rc = true_norms(op, b_dev, xt, &rn, &an))) return rc;
             ++checks;
-            test2 = (rn > 0.0 && anorm_max > 0.0) ? an / (anorm_max * rn) : 0.0;
+            const double anorm_t = colscale ? anorm_fixed : anorm_max;
+            test2 = (rn > 0.0 && anorm_t > 0.0) ? an / (anorm_t * rn) : 0.0;
             if (test2 <= o->tol || rn == 0.0) converged = true;
             if (converged || total >= itn || h.done || launched == 0) {
                 // x = xt; op->hy now holds its residual for a restart
@@ -1612,12 +1673,13 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     (void)hipHostFree(done_h);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    op->cs_active = nullptr;
     if (out) {
         memset(out, 0, sizeof(*out));
         out->itn = total;
         out->inner_total = total;
         out->refinements = checks;
-        out->anorm = anorm_max;
+        out->anorm = colscale ? anorm_fixed : anorm_max;
         out->rnorm = rn;
         out->arnorm = test2;
         out->arnorm_true = an;

@@ -1204,6 +1204,8 @@ int plsqr_batch_solve_device(plsqr_batch *b, const double *b_dev, const plsqr_so
     if (!b_dev || !opts || !x_dev || !out) return set_err(b, PLSQR_ERR_ARG, "b, opts, x and out are required");
     plsqr_solve_opts o;
     default_opts(opts, &o);
+    if (o.precond != PLSQR_PRECOND_NONE)
+        return set_err(b, PLSQR_ERR_ARG, "the batched solver has no preconditioner");
     if (o.precision == PLSQR_PREC_FP64) {
         rc = solve_fp64(b, b_dev, &o, out, total_ms);
     } else if (o.precision == PLSQR_PREC_MIXED) {

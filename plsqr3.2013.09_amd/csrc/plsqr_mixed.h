@@ -286,11 +286,17 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
 // v32[c] <- fl32(v32*vscale) + (tk + td): tk the FP64 slice partials, td the
 // damping column (FP64 values, FP32 u); FP64 partial sums of squares.
 template <int BS>
+//
+// cs (may be NULL): the column scales d of LSQR on A D.  The solver then
+// keeps v^ = D v (and w^ = D w, so x += t1 w^ directly): mode 1 of A D on v
+// is mode 1 of A on v^, and v^ = D^2 A^T u - beta v^; the norm partials are
+// those of v = v^ / d.
 __global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, const double *__restrict__ P,
                                                        const int *__restrict__ dcp, const int *__restrict__ dri,
                                                        const double *__restrict__ dav, const float *__restrict__ u,
                                                        const Scal *__restrict__ sc, float *__restrict__ v,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part,
+                                                       const double *__restrict__ cs = nullptr) {
     __shared__ double sh[BS / kWave];
     if (sc->done || !sc->beta_pos) return;
     const double ib = sc->ib;
@@ -302,9 +308,17 @@ __global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, 
         for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
         double td = 0.0;
         for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * ((double)ud[dri[k]] * ib);
-        const float vn = v[c] * vs + (float)(tk + td);
-        v[c] = vn;
-        ssq += (double)vn * (double)vn;
+        if (cs) {
+            const double d = cs[c];
+            const float vn = v[c] * vs + (float)(d * d * (tk + td));
+            v[c] = vn;
+            const double q = (double)vn / d;
+            ssq += q * q;
+        } else {
+            const float vn = v[c] * vs + (float)(tk + td);
+            v[c] = vn;
+            ssq += (double)vn * (double)vn;
+        }
     }
     const double a = block_sum<BS>(ssq, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = a;
@@ -349,6 +363,19 @@ __global__ void k_init_vw32(int n, const Scal *__restrict__ sc, float *__restric
         }
         w[i] = (double)vi;
     }
+}
+
+// d = 1 / sqrt(sum of squares of the column), 1 for an empty column
+__global__ void k_colscale(int n, double *__restrict__ d) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double q = d[i];
+        d[i] = q > 0.0 ? 1.0 / sqrt(q) : 1.0;
+    }
+}
+
+__global__ void k_square(long long len, const double *__restrict__ a, double *__restrict__ b) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long long)gridDim.x * 256)
+        b[i] = a[i] * a[i];
 }
 
 __global__ void k_to_f32(long long len, const double *__restrict__ a, float *__restrict__ b) {

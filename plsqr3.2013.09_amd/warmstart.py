@@ -17,7 +17,7 @@ import time
 
 import numpy as np
 
-from . import Batch, Operator, PREC_FP64, PREC_MIXED, WarmStartNet, gen_field
+from . import Batch, Operator, PREC_FP64, PREC_MIXED, PRECOND_COLSCALE, WarmStartNet, gen_field
 
 
 def make_pairs(batch, system, grid, seeds, sigma):
@@ -74,10 +74,12 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
-def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None):
+def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False):
     """Per held-out right-hand side: FP64 from zero (the reference method),
-    FP64 from x0, mixed precision from zero and from x0.  Times are the
-    device loop times of each solve."""
+    FP64 from x0, mixed precision from zero and from x0; with colscale also
+    the mixed solves on A D (column scaling), their FP64 test2 taking the
+    anorm the plain FP64 solve of the same b reported, so that they stop as
+    strictly as it does.  Times are the device loop times of each solve."""
     rows = []
     for j, b in enumerate(B):
         r = {}
@@ -92,6 +94,14 @@ def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None):
         xmw, imw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, precision=PREC_MIXED, refine_every=refine_every)
         r["mixed_warm"] = dict(itn=imw["itn"], ms=imw["elapsed_ms"], used_x0=imw["used_x0"],
                                xerr=rel(xmw, xr), test2=imw["arnorm"])
+        if colscale:
+            kw = dict(precision=PREC_MIXED, refine_every=refine_every, precond=PRECOND_COLSCALE,
+                      anorm_ref=i64["anorm"])
+            xc, ic = op.lsqr_x0(b, None, max_iters, tol=tol, **kw)
+            r["mixed_zero_cs"] = dict(itn=ic["itn"], ms=ic["elapsed_ms"], xerr=rel(xc, xr), test2=ic["arnorm"])
+            xcw, icw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, **kw)
+            r["mixed_warm_cs"] = dict(itn=icw["itn"], ms=icw["elapsed_ms"], used_x0=icw["used_x0"],
+                                      xerr=rel(xcw, xr), test2=icw["arnorm"])
         r["x0_err"] = rel(X0[j], xr)
         del t0
         rows.append(r)
@@ -100,7 +110,9 @@ def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None):
 
 def summarize(rows):
     out = {}
-    for key in ("fp64_zero", "fp64_warm", "mixed_zero", "mixed_warm"):
+    keys = [k for k in ("fp64_zero", "fp64_warm", "mixed_zero", "mixed_warm", "mixed_zero_cs", "mixed_warm_cs")
+            if k in rows[0]]
+    for key in keys:
         its = np.array([r[key]["itn"] for r in rows], float)
         ms = np.array([r[key]["ms"] for r in rows], float)
         out[key] = {"itn_median": float(np.median(its)), "itn_mean": float(its.mean()),
@@ -111,13 +123,13 @@ def summarize(rows):
             out[key]["used_x0"] = int(sum(bool(r[key]["used_x0"]) for r in rows))
     out["x0_err_median"] = float(np.median([r["x0_err"] for r in rows]))
     base = out["fp64_zero"]["ms_mean"]
-    for key in ("fp64_warm", "mixed_zero", "mixed_warm"):
+    for key in keys[1:]:
         out[key]["speedup_vs_fp64_zero"] = base / out[key]["ms_mean"] if out[key]["ms_mean"] > 0 else None
     return out
 
 
 def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, refine_every, log=print,
-        net_seed=3, lr=2e-3):
+        net_seed=3, lr=2e-3, colscale=False):
     """The whole pipeline on one system; returns a result dict and the trained net."""
     t = {}
     t0 = time.perf_counter()
@@ -146,7 +158,7 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     t["predict_s"] = time.perf_counter() - t0
     t["predict_ms_per_sample"] = 1000 * t["predict_s"] / max(n_eval, 1)
     t0 = time.perf_counter()
-    rows = evaluate(op, B[n_train:], X0, tol, refine_every)
+    rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale)
     t["eval_s"] = time.perf_counter() - t0
     op.close()
     res = {"summary": summarize(rows), "rows": rows, "times": t,

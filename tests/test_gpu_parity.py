@@ -374,3 +374,74 @@ def test_mixed_precision_refinement(P, O, MG, gpu_available):
     # as close to the converged solution as the FP64 solve stopped at the same test2
     e_m, e_64 = rel(xm, x_tight), rel(x64, x_tight)
     assert e_m < 3 * e_64 + 1e-7, (e_m, e_64)
+
+
+def lsqr_scaled_np(A, b, k, d):
+    """k iterations of textbook LSQR (Paige and Saunders 1982, damp = 0) on
+    A diag(d) in float64; returns x = d * y."""
+    u = b.copy()
+    beta = np.linalg.norm(u)
+    u /= beta
+    v = d * (A.T @ u)
+    alpha = np.linalg.norm(v)
+    v /= alpha
+    w, y, phibar, rhobar = v.copy(), np.zeros_like(v), beta, alpha
+    for _ in range(k):
+        u = A @ (d * v) - alpha * u
+        beta = np.linalg.norm(u)
+        u /= beta
+        v = d * (A.T @ u) - beta * v
+        alpha = np.linalg.norm(v)
+        v /= alpha
+        rho = np.hypot(rhobar, beta)
+        c, sn = rhobar / rho, beta / rho
+        theta, rhobar, phi, phibar = sn * alpha, -c * alpha, c * phibar, sn * phibar
+        y += (phi / rho) * w
+        w = v - (theta / rho) * w
+    return d * y
+
+
+def test_mixed_column_scaling(P, O, MG, gpu_available):
+    """Mixed precision on A D, D = diag(1 / ||A e_j||): the first inner
+    iterations follow textbook LSQR on A D (and not on A); the solve stops on
+    the FP64 test2 of A with ||A||_F, or with the anorm of a plain FP64
+    solve; x as close to the converged solution as the FP64 solve's."""
+    announce("mixed precision, column scaling")
+    import scipy.sparse as sp
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    bb = np.concatenate([s.b, np.zeros(s.m_d)])
+    cn = np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel())
+    d = np.where(cn > 0, 1.0 / np.where(cn > 0, cn, 1.0), 1.0)
+    fro = sp.linalg.norm(A)
+    x_tight, _, _ = O.Oracle(s).lsqr(s.b, 300)
+    k = 8
+    x_np_scaled = lsqr_scaled_np(A, bb, k, d)
+    x_np_plain = lsqr_scaled_np(A, bb, k, np.ones(s.n))
+    with P.Operator(s) as op:
+        xs_k, _ = op.lsqr(s.b, k, tol=0.0, precision=P.PREC_MIXED, refine_every=k, precond=P.PRECOND_COLSCALE)
+        xp_k, _ = op.lsqr(s.b, k, tol=0.0, precision=P.PREC_MIXED, refine_every=k)
+        x64, i64 = op.lsqr(s.b, 2000, tol=1e-6)
+        xs, is_ = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                          precond=P.PRECOND_COLSCALE)
+        xs2, _ = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                         precond=P.PRECOND_COLSCALE)
+        xr, ir = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                         precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"])
+        with pytest.raises(P.PlsqrError):
+            op.lsqr(s.b, 10, precond=P.PRECOND_COLSCALE)  # FP64 has no preconditioner
+    e_same, e_other = rel(xs_k, x_np_scaled), rel(xs_k, x_np_plain)
+    print(f"{k} inner iterations: vs LSQR on A D {e_same:.2e}, vs LSQR on A {e_other:.2e}; "
+          f"unscaled vs LSQR on A {rel(xp_k, x_np_plain):.2e}", flush=True)
+    assert e_same < 1e-3 and e_same < 0.05 * e_other, (e_same, e_other)
+    assert rel(xp_k, x_np_plain) < 1e-3
+    assert np.array_equal(xs, xs2)
+    np.testing.assert_allclose(is_["anorm"], fro, rtol=1e-12)
+    for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"])):
+        r = bb - A @ x
+        assert info["arnorm"] <= 1e-6
+        assert np.linalg.norm(A.T @ r) / (an * np.linalg.norm(r)) <= 1e-6 * 1.001
+        assert rel(x, x_tight) < 3 * rel(x64, x_tight) + 1e-7, (rel(x, x_tight), rel(x64, x_tight))
+    print(f"iterations: FP64 {i64['itn']}, mixed+colscale {is_['itn']} (||A||_F), {ir['itn']} "
+          f"(FP64 anorm)", flush=True)
