@@ -299,7 +299,7 @@ def bench_learned(P, hip, a):
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     log(f"[bench] generated {a.workload} ({gen_s:.1f} s); {a.train_pairs} training pairs, {a.eval_pairs} held out")
     res, net = W.run(s, grid, a.train_pairs, a.eval_pairs, cfg["sigma"], 500000, a.target_tol, a.epochs, a.tol,
-                     a.refine_every, log=log, colscale=a.precond == "col")
+                     a.refine_every, log=log, colscale=a.precond == "col", refine_adaptive=a.refine_adaptive)
     if a.save_params:
         np.save(a.save_params, net.get_params())
     net.close()
@@ -316,7 +316,7 @@ def bench_learned(P, hip, a):
                 "per pair)",
         "config": {"workload": a.workload + "+learned_warm_start", "train_pairs": a.train_pairs,
                    "eval_pairs": a.eval_pairs, "target_tol": a.target_tol, "epochs": a.epochs, "tol": a.tol,
-                   "method": head, "precond": a.precond,
+                   "method": head, "precond": a.precond, "refine_adaptive": a.refine_adaptive,
                    "refine_every": a.refine_every, "parallelism": "single GPU"},
         "summary": S, "best_method": best, "times": res["times"], "loss": res["loss"], "targets": res["targets"],
     }
@@ -336,6 +336,9 @@ def main():
     ap.add_argument("--precond", default="none", choices=["none", "col"],
                     help="mixed precision: LSQR on A D with D the inverse column norms (col); the FP64 "
                          "test2 then uses the anorm of an untimed plain FP64 solve of the same b")
+    ap.add_argument("--refine-adaptive", action="store_true",
+                    help="mixed precision: FP64 checks scheduled from the inner test2 estimate (first after "
+                         "--refine-every iterations, then when the estimate predicts tol, at most 4x apart)")
     ap.add_argument("--max-iters", type=int, default=5000)
     ap.add_argument("--aprod-reps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=40,
@@ -444,13 +447,14 @@ def main():
               f"anorm {anorm_ref:.6e}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec, refine_every=a.refine_every,
-                       precond=pc, anorm_ref=anorm_ref)
+                       precond=pc, anorm_ref=anorm_ref, refine_adaptive=a.refine_adaptive)
     hip.sync()
     infos = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         infos.append(op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec,
-                                    refine_every=a.refine_every, precond=pc, anorm_ref=anorm_ref))
+                                    refine_every=a.refine_every, precond=pc, anorm_ref=anorm_ref,
+                                    refine_adaptive=a.refine_adaptive))
     hip.sync()
     wall = time.perf_counter() - t0
     if ref64 is not None:
@@ -519,7 +523,8 @@ def main():
         "data": "synthetic (seeded straight-ray generator, inputs resident in HBM)",
         "config": {"workload": a.workload, "grid": [67, 67, 67] if "67" in a.workload else None,
                    "rays": s.m_k, "nnz_k": s.nnz_k, "m_d": s.m_d, "nnz_d": s.nnz_d,
-                   "precision": a.precision, "precond": a.precond, "tol": a.tol, "iterations_to_tol": iters,
+                   "precision": a.precision, "precond": a.precond, "refine_adaptive": a.refine_adaptive,
+                   "tol": a.tol, "iterations_to_tol": iters, "fp64_checks": infos[-1]["refinements"],
                    "tol_reached": bool(reached), "parallelism": "single GPU",
                    "operator": "exact-order" if a.exact_order else "default"},
         "ms_per_iteration": round(infos[-1]["elapsed_ms"] / max(iters, 1), 4),

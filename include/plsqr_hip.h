@@ -303,6 +303,11 @@ typedef struct plsqr_solve_opts {
                               FP64 test2 = ||A^T r|| / (anorm_ref ||r||); 0 = ||A||_F.
                               Passing the anorm a plain FP64 solve reported makes
                               the stop exactly as strict as that solve's         */
+    int    refine_adaptive; /* mixed with tol > 0: 1 = after the first check
+                              (refine_every inner iterations), check when the inner
+                              estimate of test2 times the last measured ratio
+                              FP64 test2 / inner estimate reaches tol, at most
+                              4 * refine_every apart; 0 = every refine_every   */
 } plsqr_solve_opts;
 
 typedef struct plsqr_solve_out {

@@ -1571,8 +1571,10 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
-    int *done_h = nullptr;
-    HIP_TRY(op, hipHostMalloc((void **)&done_h, sizeof(int)));
+    Scal *hs = nullptr;  // pinned copy of the inner state for the polls
+    HIP_TRY(op, hipHostMalloc((void **)&hs, sizeof(Scal)));
+    const bool adaptive = o->refine_adaptive != 0 && o->tol > 0.0;
+    double ratio = -1.0;  // FP64 test2 / inner estimate at the last check
     double anorm_max = 0.0, rn = 0.0, an = 0.0, test2 = INFINITY;
     int total = 0, starts = 0, checks = 0;
     bool converged = false;
@@ -1626,16 +1628,20 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         }
         int inner_prev = 0;
         for (;;) {
-            // a chunk of inner iterations, polling the inner stop every 8
-            const int k = std::min(period, itn - total);
+            // a chunk of inner iterations, polling the inner state every 8.
+            // Adaptive: the chunk may run to 4 periods and ends early once the
+            // inner estimate of test2 times the last measured ratio FP64/inner
+            // reaches tol.
+            const int k = std::min(adaptive && ratio > 0.0 ? 4 * period : period, itn - total);
             int launched = 0;
             for (int i = 0; i < k; ++i) {
                 HIP_TRY(op, hipGraphLaunch(exec, op->stream));
                 ++launched;
                 if ((i + 1) % 8 == 0 && i + 1 < k) {
-                    HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+                    HIP_TRY(op, hipMemcpyAsync(hs, op->sc, sizeof(Scal), hipMemcpyDeviceToHost, op->stream));
                     HIP_TRY(op, hipStreamSynchronize(op->stream));
-                    if (*done_h) break;
+                    if (hs->done) break;
+                    if (adaptive && ratio > 0.0 && hs->test2 * ratio <= o->tol) break;
                 }
             }
             HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
@@ -1652,6 +1658,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
             ++checks;
             const double anorm_t = colscale ? anorm_fixed : anorm_max;
             test2 = (rn > 0.0 && anorm_t > 0.0) ? an / (anorm_t * rn) : 0.0;
+            if (h.test2 > 0.0 && std::isfinite(h.test2)) ratio = test2 / h.test2;
             if (test2 <= o->tol || rn == 0.0) converged = true;
             if (converged || total >= itn || h.done || launched == 0) {
                 // x = xt; op->hy now holds its residual for a restart
@@ -1670,7 +1677,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipHostFree(done_h);
+    (void)hipHostFree(hs);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     op->cs_active = nullptr;

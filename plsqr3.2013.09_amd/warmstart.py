@@ -74,7 +74,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
-def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False):
+def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False, refine_adaptive=False):
     """Per held-out right-hand side: FP64 from zero (the reference method),
     FP64 from x0, mixed precision from zero and from x0; with colscale also
     the mixed solves on A D (column scaling), their FP64 test2 taking the
@@ -89,14 +89,14 @@ def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale
         xw, iw = op.lsqr_x0(b, X0[j], max_iters, tol=tol)
         r["fp64_warm"] = dict(itn=iw["itn"], ms=iw["elapsed_ms"], used_x0=iw["used_x0"], xerr=rel(xw, xr))
         t0 = time.perf_counter()
-        xm, im = op.lsqr_x0(b, None, max_iters, tol=tol, precision=PREC_MIXED, refine_every=refine_every)
+        mk = dict(precision=PREC_MIXED, refine_every=refine_every, refine_adaptive=refine_adaptive)
+        xm, im = op.lsqr_x0(b, None, max_iters, tol=tol, **mk)
         r["mixed_zero"] = dict(itn=im["itn"], ms=im["elapsed_ms"], xerr=rel(xm, xr), test2=im["arnorm"])
-        xmw, imw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, precision=PREC_MIXED, refine_every=refine_every)
+        xmw, imw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, **mk)
         r["mixed_warm"] = dict(itn=imw["itn"], ms=imw["elapsed_ms"], used_x0=imw["used_x0"],
                                xerr=rel(xmw, xr), test2=imw["arnorm"])
         if colscale:
-            kw = dict(precision=PREC_MIXED, refine_every=refine_every, precond=PRECOND_COLSCALE,
-                      anorm_ref=i64["anorm"])
+            kw = dict(mk, precond=PRECOND_COLSCALE, anorm_ref=i64["anorm"])
             xc, ic = op.lsqr_x0(b, None, max_iters, tol=tol, **kw)
             r["mixed_zero_cs"] = dict(itn=ic["itn"], ms=ic["elapsed_ms"], xerr=rel(xc, xr), test2=ic["arnorm"])
             xcw, icw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, **kw)
@@ -129,7 +129,7 @@ def summarize(rows):
 
 
 def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, refine_every, log=print,
-        net_seed=3, lr=2e-3, colscale=False):
+        net_seed=3, lr=2e-3, colscale=False, refine_adaptive=False):
     """The whole pipeline on one system; returns a result dict and the trained net."""
     t = {}
     t0 = time.perf_counter()
@@ -158,7 +158,7 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     t["predict_s"] = time.perf_counter() - t0
     t["predict_ms_per_sample"] = 1000 * t["predict_s"] / max(n_eval, 1)
     t0 = time.perf_counter()
-    rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale)
+    rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale, refine_adaptive=refine_adaptive)
     t["eval_s"] = time.perf_counter() - t0
     op.close()
     res = {"summary": summarize(rows), "rows": rows, "times": t,

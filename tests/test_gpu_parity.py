@@ -429,6 +429,9 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
                          precond=P.PRECOND_COLSCALE)
         xr, ir = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
                          precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"])
+        xa, ia = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=10,
+                         precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"], refine_adaptive=True)
+        xm, im = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=10, refine_adaptive=True)
         with pytest.raises(P.PlsqrError):
             op.lsqr(s.b, 10, precond=P.PRECOND_COLSCALE)  # FP64 has no preconditioner
     e_same, e_other = rel(xs_k, x_np_scaled), rel(xs_k, x_np_plain)
@@ -438,10 +441,11 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
     assert rel(xp_k, x_np_plain) < 1e-3
     assert np.array_equal(xs, xs2)
     np.testing.assert_allclose(is_["anorm"], fro, rtol=1e-12)
-    for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"])):
+    for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"]), (xa, ia, i64["anorm"]), (xm, im, im["anorm"])):
         r = bb - A @ x
         assert info["arnorm"] <= 1e-6
         assert np.linalg.norm(A.T @ r) / (an * np.linalg.norm(r)) <= 1e-6 * 1.001
         assert rel(x, x_tight) < 3 * rel(x64, x_tight) + 1e-7, (rel(x, x_tight), rel(x64, x_tight))
     print(f"iterations: FP64 {i64['itn']}, mixed+colscale {is_['itn']} (||A||_F), {ir['itn']} "
-          f"(FP64 anorm)", flush=True)
+          f"(FP64 anorm), adaptive checks {ia['itn']} ({ia['refinements']} checks); mixed adaptive "
+          f"{im['itn']} ({im['refinements']} checks)", flush=True)
