@@ -1,18 +1,24 @@
 """Benchmark: LSQR time-to-tol on the 67^3-grid / 2,000,000-ray system, one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+                    [--precond none|col] [--refine-adaptive] [--learned]
 
 A step is one complete LSQR solve from x = 0 to test2 <= 1e-6 (test2 as
 lsqr.c:742-744) on the device-resident system; W untimed solves, then K timed
 solves between device synchronisations.  value = mean seconds per solve.
+The default line is plain FP64 LSQR (the reference's method); beside it
+("mixed_colscale") the same b solved in mixed precision on A D (column
+scaling) with adaptive FP64 checks, stopping on the same test2 (the FP64
+solve's anorm), K timed solves as well.
 
 Also measured inside the run, on the handle's stream with HIP events: the
 average duration of one aprod of each mode, priced against the algorithmic
 bytes of that launch (each stored value and index read once, each pointer
 array once, each dense input once, each dense output read and written once).
-The CPU baseline is the oracle (the C restatement of the reference, one
-thread) timed on a bounded sample of the same solve and scaled to the same
-iteration count.
+The CPU baseline is the reference itself (compiled from its sources, one
+MPI rank), or the oracle (the C restatement of the reference, one thread)
+when that binary is absent, timed on a bounded sample of the same solve and
+scaled to the plain FP64 solve's iteration count.
 """
 import argparse
 import ctypes as C
@@ -349,6 +355,8 @@ def main():
     ap.add_argument("--ref-iters", type=int, default=40,
                     help="iterations of the reference run timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="FP64 line only: skip the mixed-precision column-scaled solve timed beside it")
     ap.add_argument("--aprod-only", action="store_true",
                     help="only time aprod launches and a device copy of known size "
                          "(the workload for rocprofv3 counter passes)")
@@ -467,6 +475,34 @@ def main():
     print(f"[bench] {a.steps} solves: {ms_step:.2f} ms each, {iters} iterations, "
           f"test2={infos[-1]['arnorm']:.3e} (device loop {infos[-1]['elapsed_ms']:.2f} ms)",
           file=sys.stderr, flush=True)
+    mixed_line = None
+    if prec == P.PREC_FP64 and not a.exact_order and not a.no_mixed:
+        # beside the plain FP64 line: the mixed-precision solve of the same b
+        # on A D (column scaling) with adaptive FP64 checks, stopping on the
+        # same test2 (the anorm of the FP64 solve just timed)
+        x64 = np.empty(s.n)
+        hip.d2h(x64, x_dev)
+        kw = dict(tol=a.tol, precision=P.PREC_MIXED, refine_every=a.refine_every, precond=P.PRECOND_COLSCALE,
+                  anorm_ref=infos[-1]["anorm"], refine_adaptive=True)
+        op.lsqr_device(b_dev, x_dev, a.max_iters, **kw)
+        hip.sync()
+        minfos = []
+        t1 = time.perf_counter()
+        for _ in range(a.steps):
+            minfos.append(op.lsqr_device(b_dev, x_dev, a.max_iters, **kw))
+        hip.sync()
+        mwall = (time.perf_counter() - t1) / a.steps
+        xm = np.empty(s.n)
+        hip.d2h(xm, x_dev)
+        mi = minfos[-1]
+        mixed_line = {"method": "mixed precision on A D (column scaling), adaptive FP64 checks",
+                      "s_per_solve": round(mwall, 6), "iterations_to_tol": mi["itn"],
+                      "tol_reached": bool(mi["arnorm"] <= a.tol), "fp64_checks": mi["refinements"],
+                      "ms_per_iteration": round(mi["elapsed_ms"] / max(mi["itn"], 1), 4),
+                      "speedup_vs_fp64": round((wall / a.steps) / mwall, 3),
+                      "x_rel_diff_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))}
+        print(f"[bench] mixed + column scaling: {1000 * mwall:.2f} ms per solve, {mi['itn']} iterations, "
+              f"{mixed_line['speedup_vs_fp64']}x the FP64 solve", file=sys.stderr, flush=True)
 
     # aprod kernels, timed with events on the handle's stream: on vectors in
     # the handle's order (the SpMV kernels alone, as in the solve loop), and
@@ -510,11 +546,13 @@ def main():
                 "kernel": f"aprod mode {dom_mode} ({kname})",
                 "algorithmic_bytes": dom_b, "avg_ms": round(dom_ms, 4)}
     cpu = None
+    # the reference runs plain LSQR: its time-to-tol is at the plain FP64 solve's iteration count
+    iters_ref = ref64["itn"] if ref64 is not None else iters
     if not a.no_cpu:
         if a.cpu_kind in ("auto", "reference") and not a.exact_order:
-            cpu = cpu_baseline_reference(P, s, iters, a.ref_iters)
+            cpu = cpu_baseline_reference(P, s, iters_ref, a.ref_iters)
         if cpu is None and a.cpu_kind != "reference":
-            cpu = cpu_baseline(P, s, iters, a.cpu_iters)
+            cpu = cpu_baseline(P, s, iters_ref, a.cpu_iters)
     out = {
         "metric": METRIC, "value": round(wall / a.steps, 6) if reached else None, "unit": "s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 3),
@@ -532,6 +570,7 @@ def main():
             "iterations": ref64["itn"], "device_ms": round(ref64["elapsed_ms"], 2), "anorm": ref64["anorm"],
             "speedup_device": round(ref64["elapsed_ms"] / max(infos[-1]["elapsed_ms"], 1e-9), 3),
             "x_rel_diff": ref64["xdiff"]},
+        "mixed_colscale": mixed_line,
         "aprod_gbs": {"mode1": round(gbs1, 1), "mode2": round(gbs2, 1), "both": round(gbs, 1)},
         "aprod_caller_order_ms": {"mode1": round(cm1, 4), "mode2": round(cm2, 4)},
         "roofline": roofline,

@@ -54,6 +54,37 @@ __device__ __forceinline__ double group_sum(double v) {
     return v;
 }
 
+// One lane's share of a ray's segment partials: segments b + lane,
+// b + lane + LC, ... below e, added in that order.  The partial indices are
+// loaded in groups of 8 (the tail of up to 7 under per-load predicates)
+// before any partial, so that a lane has up to 8 gathers in flight.
+template <int LC, class T>
+__device__ __forceinline__ double ray_partials(const int *__restrict__ ray_seg, const T *__restrict__ segv, int b,
+                                               int e, int lane) {
+    double acc = 0.0;
+    int k = b + lane;
+    for (; k + 7 * LC < e; k += 8 * LC) {
+        int i[8];
+        double p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) i[j] = ray_seg[k + j * LC];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p[j] = (double)segv[i[j]];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += p[j];
+    }
+    int i[7];
+    double p[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) i[j] = k + j * LC < e ? ray_seg[k + j * LC] : 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) p[j] = k + j * LC < e ? (double)segv[i[j]] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+        if (k + j * LC < e) acc += p[j];
+    return acc;
+}
+
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
     v = group_sum<kWave>(v);

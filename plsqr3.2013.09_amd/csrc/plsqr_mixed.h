@@ -154,17 +154,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int
     for (int g = blockIdx.x / S; g < groups; g += gridDim.x / S) {
         const int r = r0 + g * R + slot;
         double acc = 0.0;
-        if (r < r1) {
-            const int b = ray_ptr[r], e = ray_ptr[r + 1];
-            int k = b + lane;
-            for (; k + LC < e; k += 2 * LC) {
-                const int i0 = ray_seg[k], i1 = ray_seg[k + LC];
-                const double p0 = segv[i0], p1 = segv[i1];
-                acc += p0;
-                acc += p1;
-            }
-            if (k < e) acc += segv[ray_seg[k]];
-        }
+        if (r < r1) acc = ray_partials<LC>(ray_seg, segv, ray_ptr[r], ray_ptr[r + 1], lane);
         acc = group_sum<LC>(acc);
         if (lane == 0 && r < r1) {
             const float yn = (y[r] * s1) * s2 + (float)acc;
