@@ -282,9 +282,10 @@ enum {
  * reference, whose lsqr() always runs on A itself). */
 enum {
     PLSQR_PRECOND_NONE = 0,      /* LSQR on A                                          */
-    PLSQR_PRECOND_COLSCALE = 1   /* LSQR on A D with D = diag(1 / ||A e_j||) (columns of
-                                    [A_k; damping] scaled to unit 2-norm; zero columns
-                                    keep 1), x = D y; the FP64 checks of test2 stay on A */
+    PLSQR_PRECOND_COLSCALE = 1   /* LSQR on A D with D = diag(||A e_j||^-p), p =
+                                    precond_power (1: columns of [A_k; damping] scaled
+                                    to unit 2-norm; zero columns keep 1), x = D y; the
+                                    FP64 checks of test2 stay on A                     */
 };
 
 typedef struct plsqr_solve_opts {
@@ -308,6 +309,8 @@ typedef struct plsqr_solve_opts {
                               estimate of test2 times the last measured ratio
                               FP64 test2 / inner estimate reaches tol, at most
                               4 * refine_every apart; 0 = every refine_every   */
+    double precond_power;  /* PLSQR_PRECOND_COLSCALE: D = diag(||A e_j||^-precond_power);
+                              0 means 1 (unit column norms)                     */
 } plsqr_solve_opts;
 
 typedef struct plsqr_solve_out {

@@ -149,17 +149,19 @@ class Operator:
     # -- LSQR -------------------------------------------------------------
     @staticmethod
     def make_opts(itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0, precond=PRECOND_NONE,
-                  anorm_ref=0.0, refine_adaptive=False):
+                  anorm_ref=0.0, refine_adaptive=False, precond_power=0.0):
         o = SolveOptsC()
         o.itn, o.tol, o.precision, o.refine_every, o.damp = int(itn), float(tol), int(precision), \
             int(refine_every), float(damp)
         o.atol = o.btol = o.conlim = 0.0
         o.itnlim = 0
         o.precond, o.anorm_ref, o.refine_adaptive = int(precond), float(anorm_ref), int(bool(refine_adaptive))
+        o.precond_power = float(precond_power)
         return o
 
     def lsqr(self, b, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0,
-             trace=False, want_w=False, precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False):
+             trace=False, want_w=False, precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False,
+             precond_power=0.0):
         """Returns (x, info dict[, trace array (itn+1, 8)])."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         assert b.size == self.m_k
@@ -167,7 +169,8 @@ class Operator:
         w = np.empty(self.n, np.float64) if want_w else None
         out = SolveOutC()
         tr = np.full((itn + 1, TRACE_WIDTH), np.nan) if trace else None
-        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive,
+                           precond_power)
         _check(self.lib.plsqr_lsqr_solve(self.h, dptr(b), C.byref(o), dptr(x),
                                          dptr(w) if w is not None else None, C.byref(out),
                                          tr.ctypes.data_as(C.POINTER(C.c_double))
@@ -189,9 +192,11 @@ class Operator:
         return x, se, out.as_dict()
 
     def lsqr_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
-                    damp=0.0, w_ptr=None, precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False):
+                    damp=0.0, w_ptr=None, precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False,
+                    precond_power=0.0):
         out = SolveOutC()
-        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive,
+                           precond_power)
         _check(self.lib.plsqr_lsqr_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
                                                 C.c_void_p(x_ptr),
                                                 C.c_void_p(w_ptr) if w_ptr else None,
@@ -199,14 +204,16 @@ class Operator:
         return out.as_dict()
 
     def lsqr_x0(self, b, x0, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0,
-                precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False):
+                precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False,
+                precond_power=0.0):
         """Warm start from x0 (None = zero start); returns (x, info) with info["used_x0"]."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         x = np.empty(self.n, np.float64)
         out = SolveOutC()
         used = C.c_int(0)
-        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive)
+        o = self.make_opts(itn, tol, precision, refine_every, damp, precond, anorm_ref, refine_adaptive,
+                           precond_power)
         _check(self.lib.plsqr_lsqr_solve_x0(self.h, dptr(b), dptr(x0), C.byref(o), dptr(x), C.byref(out),
                                             C.byref(used)), self.h)
         info = out.as_dict()
@@ -214,10 +221,12 @@ class Operator:
         return x, info
 
     def lsqr_x0_device(self, b_ptr, x0_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
-                       precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False):
+                       precond=PRECOND_NONE, anorm_ref=0.0, refine_adaptive=False,
+                       precond_power=0.0):
         out = SolveOutC()
         used = C.c_int(0)
-        o = self.make_opts(itn, tol, precision, refine_every, 0.0, precond, anorm_ref, refine_adaptive)
+        o = self.make_opts(itn, tol, precision, refine_every, 0.0, precond, anorm_ref, refine_adaptive,
+                           precond_power)
         _check(self.lib.plsqr_lsqr_solve_x0_device(self.h, C.c_void_p(b_ptr),
                                                    C.c_void_p(x0_ptr) if x0_ptr else None, C.byref(o),
                                                    C.c_void_p(x_ptr), C.byref(out), C.byref(used)), self.h)

@@ -74,7 +74,7 @@ class SolveOptsC(C.Structure):
         ("refine_every", C.c_int), ("damp", C.c_double),
         ("atol", C.c_double), ("btol", C.c_double), ("conlim", C.c_double),
         ("itnlim", C.c_int), ("precond", C.c_int), ("anorm_ref", C.c_double),
-        ("refine_adaptive", C.c_int),
+        ("refine_adaptive", C.c_int), ("precond_power", C.c_double),
     ]
 
 

@@ -85,7 +85,8 @@ struct plsqr_op {
     double *dx = nullptr;
     // column scaling of the mixed solve: d = 1 / ||A e_j|| (device column
     // order), ||A||_F, and the scales the FP32 combine applies during a solve
-    double *cscale = nullptr;
+    double *cscale = nullptr, *colss = nullptr;  // scales, column sums of squares
+    double cs_power = 0.0;                        // exponent the scales were made for
     double fro = 0.0;
     const double *cs_active = nullptr;
     // warm starts: x0 in the device column order and r0 = [b;0] - A x0
@@ -1487,17 +1488,28 @@ static int true_norms(plsqr_op *op, const double *b_dev, const double *xt, doubl
     return PLSQR_OK;
 }
 
-// Column scales d_j = 1 / ||A e_j|| of [A_k; D] (device column order) and
-// ||A||_F, once per handle: the tiled mode 2 run on the squared values with
-// u = 1 gives every column's sum of squares in a fixed order.
-static int ensure_colscale(plsqr_op *op) {
+// Column scales d_j = ||A e_j||^-power of [A_k; D] (device column order)
+// and ||A||_F.  The column sums of squares are formed once per handle: the
+// tiled mode 2 run on the squared values with u = 1 gives every one in a
+// fixed order; the scales are remade when the power changes.
+static int ensure_colscale(plsqr_op *op, double power) {
     using namespace plsqr;
-    if (op->cscale) return PLSQR_OK;
+    if (op->cscale && op->cs_power == power) return PLSQR_OK;
     if (!op->tiled) return set_err(op, PLSQR_ERR_STATE, "column scaling needs the tiled operator");
+    if (!(power > 0.0) || !std::isfinite(power)) return set_err(op, PLSQR_ERR_ARG, "precond_power must be > 0");
     int rc;
+    if (op->colss && op->cscale) {
+        hipLaunchKernelGGL(k_colscale, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                           (const double *)op->colss, power, op->cscale);
+        LAUNCH_CHECK(op);
+        HIP_TRY(op, hipStreamSynchronize(op->stream));
+        op->cs_power = power;
+        return PLSQR_OK;
+    }
     double *cs = nullptr, *sq_t = nullptr, *sq_d = nullptr, *ones = nullptr;
-    if ((rc = dalloc(op, &cs, op->n)) || (rc = dalloc(op, &sq_t, op->t_nslot)) ||
-        (rc = dalloc(op, &sq_d, op->nnz_d)) || (rc = dalloc(op, &ones, op->m)))
+    if ((!op->colss && (rc = dalloc(op, &op->colss, op->n))) || (rc = dalloc(op, &cs, op->n)) ||
+        (rc = dalloc(op, &sq_t, op->t_nslot)) || (rc = dalloc(op, &sq_d, op->nnz_d)) ||
+        (rc = dalloc(op, &ones, op->m)))
         return rc;
     const int gz = grid_for(std::max(op->m, op->n), 256, op->cu * 4);
     hipLaunchKernelGGL(k_square, dim3(grid_for(op->t_nslot, 256, op->cu * 16)), dim3(256), 0, op->stream,
@@ -1505,11 +1517,11 @@ static int ensure_colscale(plsqr_op *op) {
     hipLaunchKernelGGL(k_square, dim3(grid_for(op->nnz_d, 256, op->cu * 16)), dim3(256), 0, op->stream, op->nnz_d,
                        (const double *)op->dcv, sq_d);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, 1.0, ones);
-    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, cs);
+    hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->colss);
     LAUNCH_CHECK(op);
     std::swap(op->t_sval, sq_t);
     std::swap(op->dcv, sq_d);
-    rc = launch_cols(op, ones, cs, nullptr, nullptr);
+    rc = launch_cols(op, ones, op->colss, nullptr, nullptr);
     std::swap(op->t_sval, sq_t);
     std::swap(op->dcv, sq_d);
     HIP_TRY(op, hipStreamSynchronize(op->stream));
@@ -1523,17 +1535,22 @@ static int ensure_colscale(plsqr_op *op) {
     hipLaunchKernelGGL((k_ssq<BS>), dim3(ga), dim3(BS), 0, op->stream, nnz, (const double *)op->rv, op->part);
     LAUNCH_CHECK(op);
     if ((rc = host_sum(op, op->part, ga, &fro2))) return rc;
-    hipLaunchKernelGGL(k_colscale, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n, cs);
+    hipLaunchKernelGGL(k_colscale, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, op->n,
+                       (const double *)op->colss, power, cs);
     LAUNCH_CHECK(op);
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     op->fro = std::sqrt(fro2);
     op->cscale = cs;
+    op->cs_power = power;
     return PLSQR_OK;
 }
 
 // Mixed precision: an inner LSQR in FP32 (values, u, v; FP64-accumulated dot products; FP64
 // scalars, w and correction dx) solves min ||A dx - r|| for r = [b;0] - A x.
-// Every refine_every inner iterations the FP64 residual of xt = x + dx is
+// Every refine_every inner iterations (refine_adaptive: first after
+// refine_every, then when the inner test2 estimate times the last measured
+// ratio FP64 / inner reaches tol, at most 4 refine_every apart) the FP64
+// residual of xt = x + dx is
 // formed and its test2 = ||A^T r|| / (anorm ||r||) checked in FP64 (anorm:
 // the largest LSQR estimate of ||A|| so far); the solve stops there when it
 // is <= tol.  The inner run is not interrupted by the check: it keeps its
@@ -1541,8 +1558,9 @@ static int ensure_colscale(plsqr_op *op) {
 // its own recurrence stops (its estimate of test2 reached tol) while the FP64
 // test2 has not.  itn bounds the inner iterations in total.
 // With PLSQR_PRECOND_COLSCALE the inner LSQR runs on A D (D = diag of the
-// column scales) and the FP64 test2 uses anorm_ref, or ||A||_F when that is
-// 0, in place of the inner estimate, which is then one of ||A D||.
+// column scales ||A e_j||^-precond_power) and the FP64 test2 uses anorm_ref,
+// or ||A||_F when that is 0, in place of the inner estimate, which is then
+// one of ||A D||.
 // x_init: NULL for x = 0, else the starting x in the device column order.
 static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out,
                        const double *x_init = nullptr) {
@@ -1553,7 +1571,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     double *xt = op->x0i == x_init ? op->r0 : op->x0i;  // n doubles not aliasing x_init
     const bool colscale = o->precond == PLSQR_PRECOND_COLSCALE;
     if (o->precond != PLSQR_PRECOND_NONE && !colscale) return set_err(op, PLSQR_ERR_ARG, "unknown preconditioner");
-    if (colscale && (rc = ensure_colscale(op))) return rc;
+    if (colscale && (rc = ensure_colscale(op, o->precond_power == 0.0 ? 1.0 : o->precond_power))) return rc;
     op->cs_active = colscale ? op->cscale : nullptr;
     const double anorm_fixed = colscale ? (o->anorm_ref > 0.0 ? o->anorm_ref : op->fro) : 0.0;
     const int itn = std::max(0, o->itn);

@@ -355,11 +355,11 @@ __global__ void k_init_vw32(int n, const Scal *__restrict__ sc, float *__restric
     }
 }
 
-// d = 1 / sqrt(sum of squares of the column), 1 for an empty column
-__global__ void k_colscale(int n, double *__restrict__ d) {
+// d = (sum of squares of the column)^(-power/2), 1 for an empty column
+__global__ void k_colscale(int n, const double *__restrict__ colss, double power, double *__restrict__ d) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double q = d[i];
-        d[i] = q > 0.0 ? 1.0 / sqrt(q) : 1.0;
+        const double q = colss[i];
+        d[i] = q > 0.0 ? (power == 1.0 ? 1.0 / sqrt(q) : pow(q, -0.5 * power)) : 1.0;
     }
 }
 

@@ -74,7 +74,8 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
-def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False, refine_adaptive=False):
+def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False, refine_adaptive=False,
+             precond_power=1.0):
     """Per held-out right-hand side: FP64 from zero (the reference method),
     FP64 from x0, mixed precision from zero and from x0; with colscale also
     the mixed solves on A D (column scaling), their FP64 test2 taking the
@@ -96,7 +97,7 @@ def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale
         r["mixed_warm"] = dict(itn=imw["itn"], ms=imw["elapsed_ms"], used_x0=imw["used_x0"],
                                xerr=rel(xmw, xr), test2=imw["arnorm"])
         if colscale:
-            kw = dict(mk, precond=PRECOND_COLSCALE, anorm_ref=i64["anorm"])
+            kw = dict(mk, precond=PRECOND_COLSCALE, anorm_ref=i64["anorm"], precond_power=precond_power)
             xc, ic = op.lsqr_x0(b, None, max_iters, tol=tol, **kw)
             r["mixed_zero_cs"] = dict(itn=ic["itn"], ms=ic["elapsed_ms"], xerr=rel(xc, xr), test2=ic["arnorm"])
             xcw, icw = op.lsqr_x0(b, X0[j], max_iters, tol=tol, **kw)
@@ -129,7 +130,7 @@ def summarize(rows):
 
 
 def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, refine_every, log=print,
-        net_seed=3, lr=2e-3, colscale=False, refine_adaptive=False):
+        net_seed=3, lr=2e-3, colscale=False, refine_adaptive=False, precond_power=1.0):
     """The whole pipeline on one system; returns a result dict and the trained net."""
     t = {}
     t0 = time.perf_counter()
@@ -158,7 +159,8 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     t["predict_s"] = time.perf_counter() - t0
     t["predict_ms_per_sample"] = 1000 * t["predict_s"] / max(n_eval, 1)
     t0 = time.perf_counter()
-    rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale, refine_adaptive=refine_adaptive)
+    rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale, refine_adaptive=refine_adaptive,
+                    precond_power=precond_power)
     t["eval_s"] = time.perf_counter() - t0
     op.close()
     res = {"summary": summarize(rows), "rows": rows, "times": t,

@@ -432,6 +432,15 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
         xa, ia = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=10,
                          precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"], refine_adaptive=True)
         xm, im = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=10, refine_adaptive=True)
+        # D = diag(||A e_j||^-1.5): the first iterations follow LSQR on that A D
+        x15_k, _ = op.lsqr(s.b, k, tol=0.0, precision=P.PREC_MIXED, refine_every=k, precond=P.PRECOND_COLSCALE,
+                           precond_power=1.5)
+        x15, i15 = op.lsqr(s.b, 2000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                           precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"], refine_adaptive=True,
+                           precond_power=1.5)
+        xs_k2, _ = op.lsqr(s.b, k, tol=0.0, precision=P.PREC_MIXED, refine_every=k, precond=P.PRECOND_COLSCALE)
+        with pytest.raises(P.PlsqrError):
+            op.lsqr(s.b, 10, precision=P.PREC_MIXED, precond=P.PRECOND_COLSCALE, precond_power=-1.0)
         with pytest.raises(P.PlsqrError):
             op.lsqr(s.b, 10, precond=P.PRECOND_COLSCALE)  # FP64 has no preconditioner
     e_same, e_other = rel(xs_k, x_np_scaled), rel(xs_k, x_np_plain)
@@ -439,9 +448,15 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
           f"unscaled vs LSQR on A {rel(xp_k, x_np_plain):.2e}", flush=True)
     assert e_same < 1e-3 and e_same < 0.05 * e_other, (e_same, e_other)
     assert rel(xp_k, x_np_plain) < 1e-3
+    assert np.array_equal(xs_k, xs_k2)  # scales remade for power 1 after power 1.5
+    d15 = d ** 1.5
+    e15, e15_other = rel(x15_k, lsqr_scaled_np(A, bb, k, d15)), rel(x15_k, x_np_scaled)
+    print(f"power 1.5: vs LSQR on A D^1.5 {e15:.2e}, vs LSQR on A D {e15_other:.2e}", flush=True)
+    assert e15 < 1e-3 and e15 < 0.05 * e15_other, (e15, e15_other)
     assert np.array_equal(xs, xs2)
     np.testing.assert_allclose(is_["anorm"], fro, rtol=1e-12)
-    for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"]), (xa, ia, i64["anorm"]), (xm, im, im["anorm"])):
+    for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"]), (xa, ia, i64["anorm"]), (xm, im, im["anorm"]),
+                        (x15, i15, i64["anorm"])):
         r = bb - A @ x
         assert info["arnorm"] <= 1e-6
         assert np.linalg.norm(A.T @ r) / (an * np.linalg.norm(r)) <= 1e-6 * 1.001
