@@ -228,14 +228,24 @@ def bench_batch(P, hip, a):
     b_dev = hip.malloc(8 * b.size)
     x_dev = hip.malloc(8 * sum(s.n for s in systems) * K)
     hip.h2d(b_dev, b)
+    pc, ref64 = P.PRECOND_NONE, None
+    if a.precond == "col":
+        if prec != P.PREC_MIXED:
+            sys.exit("--precond col needs --precision mixed")
+        # every pair's scaled solve stops as strictly as its plain FP64 solve:
+        # its test2 takes the anorm that solve reports
+        ref64 = bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol)
+        bt.set_anorm_refs([[i["anorm"] for i in row] for row in ref64[0]])
+        pc = P.PRECOND_COLSCALE
+        print(f"[bench] plain FP64 batch: {ref64[1]:.1f} ms", file=sys.stderr, flush=True)
+    kw = dict(tol=a.tol, precision=prec, refine_every=a.refine_every, precond=pc, precond_power=a.precond_power)
     for _ in range(a.warmup):
-        bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec, refine_every=a.refine_every)
+        bt.solve_device(b_dev, x_dev, a.max_iters, **kw)
     hip.sync()
     runs = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        runs.append(bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol, precision=prec,
-                                    refine_every=a.refine_every))
+        runs.append(bt.solve_device(b_dev, x_dev, a.max_iters, **kw))
     hip.sync()
     wall = time.perf_counter() - t0
     infos, tot = runs[-1]
@@ -274,12 +284,14 @@ def bench_batch(P, hip, a):
         "config": {"workload": a.workload, "problems": len(systems), "rhs_per_problem": K,
                    "grids": [sp["grid"][0] for sp in specs], "nnz_k": nnz, "precision": a.precision,
                    "refine_every": a.refine_every if prec == P.PREC_MIXED else None, "tol": a.tol,
+                   "precond": a.precond, "precond_power": a.precond_power if a.precond == "col" else None,
                    "tol_reached": bool(reached), "parallelism": "single GPU"},
         "per_problem_ms": {"p50": round(float(np.percentile(per_problem, 50)), 2),
                            "p99": round(float(np.percentile(per_problem, 99)), 2),
                            "max": round(float(per_problem.max()), 2)},
         "iterations": {"min": int(iters.min()), "median": int(np.median(iters)), "max": int(iters.max())},
         "device_ms": round(tot, 2),
+        "plain_fp64_batch_ms": None if ref64 is None else round(ref64[1], 2),
         "tuning": tune,
         "cpu_baseline": cpu,
         "setup_s": {"generate": round(gen_s, 2), "batch_create": round(setup_s, 2)},

@@ -492,9 +492,19 @@ int plsqr_batch_aprod(plsqr_batch *h, int mode, int precision, double *x, double
  * refine_every inner iterations and stops on the FP64 test2 of the true
  * residual).  out: n_problems * n_rhs results in the array order above;
  * out[i].elapsed_ms is the device time from the start of the solve until
- * pair i stopped, and *total_ms (may be NULL) the time of the whole batch. */
+ * pair i stopped, and *total_ms (may be NULL) the time of the whole batch.
+ * MIXED accepts precond = PLSQR_PRECOND_COLSCALE (precond_power as for
+ * plsqr_lsqr_solve): each problem's inner LSQR runs on A_p D_p and pair i's
+ * FP64 test2 takes the anorm set by plsqr_batch_set_anorm_refs, or
+ * ||A_p||_F; refine_adaptive and anorm_ref are not used here. */
 int plsqr_batch_solve(plsqr_batch *h, const double *b_host, const plsqr_solve_opts *opts,
                       double *x_host, plsqr_solve_out *out, double *total_ms);
+/* The anorm of the FP64 test2 of every (problem, right-hand side) pair of a
+ * column-scaled batch solve (refs: n_problems * n_rhs values in the order of
+ * out, e.g. the out[i].anorm of a plain FP64 batch solve of the same b, so
+ * that each pair stops exactly as strictly; a value <= 0 means ||A_p||_F).
+ * refs == NULL clears them.  No counterpart in the reference. */
+int plsqr_batch_set_anorm_refs(plsqr_batch *h, const double *refs);
 /* Same on device arrays (b_dev and x_dev in the array layout above). */
 int plsqr_batch_solve_device(plsqr_batch *h, const double *b_dev, const plsqr_solve_opts *opts,
                              double *x_dev, plsqr_solve_out *out, double *total_ms);

@@ -327,23 +327,33 @@ class Batch:
         self._check(self.lib.plsqr_batch_aprod(self.h, int(mode), int(precision), dptr(x), dptr(y)))
         return self._unpack(x, self.n), self._unpack(y, self.m)
 
-    def solve(self, B, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0):
+    def set_anorm_refs(self, refs):
+        """Per-(problem, rhs) anorm of a column-scaled solve's FP64 test2 ((P, K)
+        array, e.g. [[info["anorm"] ...] ...] of an FP64 batch solve), or None."""
+        if refs is None:
+            self._check(self.lib.plsqr_batch_set_anorm_refs(self.h, None))
+            return
+        r = np.ascontiguousarray(refs, dtype=np.float64).reshape(self.P * self.K)
+        self._check(self.lib.plsqr_batch_set_anorm_refs(self.h, dptr(r)))
+
+    def solve(self, B, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0, precond=PRECOND_NONE,
+              precond_power=0.0):
         """B: list of (K, m_k) arrays.  Returns (X list of (K, n), infos [p][k] dicts, total_ms)."""
         b = self._pack(B, self.m_k)
         x = np.empty(sum(self.n) * self.K, np.float64)
         out = (SolveOutC * (self.P * self.K))()
         tot = C.c_double()
-        o = Operator.make_opts(itn, tol, precision, refine_every, damp)
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power)
         self._check(self.lib.plsqr_batch_solve(self.h, dptr(b), C.byref(o), dptr(x), out, C.byref(tot)))
         infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]
         return self._unpack(x, self.n), infos, tot.value
 
     def solve_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
-                     damp=0.0):
+                     damp=0.0, precond=PRECOND_NONE, precond_power=0.0):
         """b_ptr, x_ptr: device arrays in the library's batch layout.  Returns (infos, total_ms)."""
         out = (SolveOutC * (self.P * self.K))()
         tot = C.c_double()
-        o = Operator.make_opts(itn, tol, precision, refine_every, damp)
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power)
         self._check(self.lib.plsqr_batch_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
                                                       C.c_void_p(x_ptr), out, C.byref(tot)))
         infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]

@@ -193,6 +193,7 @@ SIGNATURES = {
     "plsqr_batch_aprod": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_double_p]),
     "plsqr_batch_solve": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p,
                                     C.POINTER(SolveOutC), c_double_p]),
+    "plsqr_batch_set_anorm_refs": (C.c_int, [C.c_void_p, c_double_p]),
     "plsqr_batch_solve_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveOptsC),
                                            C.c_void_p, C.POINTER(SolveOutC), c_double_p]),
 }

@@ -549,7 +549,9 @@ __global__ __launch_bounds__(BS) void kb_combine(int ncc, const int *__restrict_
                                                  const int *__restrict__ dcp, const int *__restrict__ dri,
                                                  const double *__restrict__ dcv, const T *__restrict__ u,
                                                  T *__restrict__ v, const Scal *__restrict__ sc,
-                                                 double *__restrict__ part) {
+                                                 double *__restrict__ part, const double *__restrict__ cs) {
+    // cs (may be NULL): column scales d of LSQR on A D; v holds v^ = D v, so
+    // v^ = d^2 A^T u - beta v^ and the norm partials are those of v^ / d
     __shared__ double sh[K][BS / kWave];
     for (int ch = blockIdx.x; ch < ncc; ch += gridDim.x) {
         const int p = cc_prob[ch];
@@ -589,12 +591,25 @@ __global__ __launch_bounds__(BS) void kb_combine(int ncc, const int *__restrict_
                 for (int k = 0; k < K; ++k) td[k] += d * ((double)q.a[k] * ib[k]);
             }
             VK<T, K> vv = ldk<T, K>(v, (size_t)c);
+            if (cs) {
+                const double dd = cs[c];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (live[k]) {
-                    const T vn = vv.a[k] * vsT[k] + (T)(tk.a[k] + td[k]);
-                    vv.a[k] = vn;
-                    ssq[k] += (double)vn * (double)vn;
+                for (int k = 0; k < K; ++k) {
+                    if (live[k]) {
+                        const T vn = vv.a[k] * vsT[k] + (T)(dd * dd * (tk.a[k] + td[k]));
+                        vv.a[k] = vn;
+                        const double q = (double)vn / dd;
+                        ssq[k] += q * q;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (live[k]) {
+                        const T vn = vv.a[k] * vsT[k] + (T)(tk.a[k] + td[k]);
+                        vv.a[k] = vn;
+                        ssq[k] += (double)vn * (double)vn;
+                    }
                 }
             }
             stk<T, K>(v, (size_t)c, vv);
@@ -887,6 +902,26 @@ template <class T>
 __global__ void kb_fill(long long len, T val, T *__restrict__ a) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
         a[i] = val;
+}
+
+// column sums of squares from the first of K interleaved mode-2 results
+// (colss[c] = v[c*K]) and column scales cs = colss^(-power/2), 1 for an
+// empty column
+__global__ void kb_take_first(long long cols, int K, const double *__restrict__ v, double *__restrict__ colss) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (long long)gridDim.x * blockDim.x)
+        colss[c] = v[c * K];
+}
+
+__global__ void kb_colscale(long long cols, const double *__restrict__ colss, double power, double *__restrict__ cs) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (long long)gridDim.x * blockDim.x) {
+        const double q = colss[c];
+        cs[c] = q > 0.0 ? (power == 1.0 ? 1.0 / sqrt(q) : pow(q, -0.5 * power)) : 1.0;
+    }
+}
+
+__global__ void kb_square(long long len, const double *__restrict__ a, double *__restrict__ b) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i] * a[i];
 }
 
 // xt = x + dx

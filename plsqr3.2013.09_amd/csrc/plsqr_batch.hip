@@ -93,6 +93,13 @@ struct plsqr_batch {
     int part_len = 0;
     Scal *sc = nullptr, *sc_plain = nullptr;
     int *live_d = nullptr, *live_h = nullptr;
+    // column scaling of the mixed solve: cs (cols) holds the scales during a
+    // scaled solve and 1 otherwise (the FP32 combine always reads it);
+    // colss the column sums of squares, fro_p each problem's ||A_p||_F,
+    // anorm_refs the caller's per-(problem, rhs) anorm for the FP64 test2
+    double *cs = nullptr, *colss = nullptr;
+    bool cs_scaled = false;  // cs holds scales (a scaled solve ran or stopped early)
+    std::vector<double> fro_p, anorm_refs;
     Shape shape[2];
     GraphCache graphs[2];
     std::vector<hipEvent_t> evs;
@@ -134,6 +141,17 @@ int balloc(plsqr_batch *b, T **p, size_t count) {
     b->allocs.push_back((void *)*p);
     b->bytes += sizeof(T) * count;
     return PLSQR_OK;
+}
+
+template <class T>
+void bfree(plsqr_batch *b, T *p, size_t count) {
+    if (!p) return;
+    auto it = std::find(b->allocs.begin(), b->allocs.end(), (void *)p);
+    if (it != b->allocs.end()) {
+        b->allocs.erase(it);
+        b->bytes -= sizeof(T) * std::max<size_t>(count, 1);
+    }
+    (void)hipFree(p);
 }
 
 template <class T>
@@ -230,7 +248,8 @@ int cols_k(plsqr_batch *b, const Shape &s, const T *sv, const T *uv, T *vv, cons
     BT_LAUNCH(b);
     hipLaunchKernelGGL((kb_combine<T, K, 256>), dim3(b->ncch), dim3(256), 0, b->stream, b->ncch, b->cch_col,
                        b->cch_prob, b->d_pofs, b->d_nslice, b->d_col0, b->d_ncol, (const double *)b->pp, b->dcp,
-                       b->dri, b->dcv, uv, vv, sc, b->part);
+                       b->dri, b->dcv, uv, vv, sc, b->part,
+                       std::is_same<T, float>::value ? (const double *)b->cs : (const double *)nullptr);
     BT_LAUNCH(b);
     return PLSQR_OK;
 }
@@ -367,7 +386,8 @@ int fill(plsqr_batch *b, T *a, long long len, T val) {
 int ensure_fp32(plsqr_batch *b) {
     if (b->rv32) return PLSQR_OK;
     int rc;
-    if ((rc = balloc(b, &b->rv32, (size_t)b->nnz)) || (rc = balloc(b, &b->sval32, (size_t)b->slots)) ||
+    if ((rc = balloc(b, &b->cs, (size_t)b->cols)) || (rc = fill(b, b->cs, b->cols, 1.0)) ||
+        (rc = balloc(b, &b->rv32, (size_t)b->nnz)) || (rc = balloc(b, &b->sval32, (size_t)b->slots)) ||
         (rc = balloc(b, &b->u32, (size_t)b->rows * b->K)) || (rc = balloc(b, &b->v32, (size_t)b->cols * b->K)))
         return rc;
     hipLaunchKernelGGL((kb_convert<float>), dim3(grid_for(b->nnz, 256, b->cu * 8)), dim3(256), 0, b->stream, b->nnz,
@@ -572,10 +592,72 @@ int true_norms(plsqr_batch *b, const double *xv, std::vector<double> &sums) {
 // once its FP64 test2 = ||A^T r|| / (anorm ||r||) <= tol, anorm the largest
 // inner estimate), and a restart on the refined residuals (x = xt) when
 // every pair still live has stopped its inner run without meeting tol.
+// Column sums of squares of every problem's [A_k; D] (once per handle: the
+// FP64 mode 2 on the squared values with u = 1), ||A_p||_F, and the scales
+// cs = colss^(-power/2) for a scaled mixed solve.
+int set_colscale(plsqr_batch *b, double power) {
+    int rc;
+    if (!(power > 0.0) || !std::isfinite(power)) return set_err(b, PLSQR_ERR_ARG, "precond_power must be > 0");
+    if (!b->colss) {
+        double *sq_s = nullptr, *sq_d = nullptr;
+        if ((rc = balloc(b, &b->colss, (size_t)b->cols)) || (rc = balloc(b, &sq_s, (size_t)b->slots)) ||
+            (rc = balloc(b, &sq_d, (size_t)std::max(b->dnnz, 1LL))))
+            return rc;
+        hipLaunchKernelGGL(kb_square, dim3(grid_for(b->slots, 256, b->cu * 8)), dim3(256), 0, b->stream, b->slots,
+                           (const double *)b->sval, sq_s);
+        hipLaunchKernelGGL(kb_square, dim3(grid_for(b->dnnz, 256, b->cu * 8)), dim3(256), 0, b->stream, b->dnnz,
+                           (const double *)b->dcv, sq_d);
+        BT_LAUNCH(b);
+        if ((rc = fill(b, b->u64, b->rows * b->K, 1.0)) || (rc = fill(b, b->v64, b->cols * b->K, 0.0))) return rc;
+        std::swap(b->dcv, sq_d);
+        rc = launch_cols<double>(b, b->shape[0], sq_s, b->u64, b->v64, b->sc_plain);
+        std::swap(b->dcv, sq_d);
+        BT_TRY(b, hipStreamSynchronize(b->stream));
+        bfree(b, sq_s, (size_t)b->slots);
+        bfree(b, sq_d, (size_t)std::max(b->dnnz, 1LL));
+        if (rc) return rc;
+        hipLaunchKernelGGL(kb_take_first, dim3(grid_for(b->cols, 256, b->cu * 8)), dim3(256), 0, b->stream, b->cols,
+                           b->K, (const double *)b->v64, b->colss);
+        BT_LAUNCH(b);
+        std::vector<double> h((size_t)b->cols);
+        BT_TRY(b, hipMemcpyAsync(h.data(), b->colss, sizeof(double) * b->cols, hipMemcpyDeviceToHost, b->stream));
+        BT_TRY(b, hipStreamSynchronize(b->stream));
+        b->fro_p.assign((size_t)b->P, 0.0);
+        for (int p = 0; p < b->P; ++p) {
+            double f = 0.0;
+            for (long long c = b->col0[(size_t)p]; c < b->col0[(size_t)p] + b->np[(size_t)p]; ++c) f += h[(size_t)c];
+            b->fro_p[(size_t)p] = std::sqrt(f);
+        }
+    }
+    hipLaunchKernelGGL(kb_colscale, dim3(grid_for(b->cols, 256, b->cu * 8)), dim3(256), 0, b->stream, b->cols,
+                       (const double *)b->colss, power, b->cs);
+    BT_LAUNCH(b);
+    b->cs_scaled = true;
+    return PLSQR_OK;
+}
+
+// cs back to 1 (unscaled FP32 operator)
+int clear_colscale(plsqr_batch *b) {
+    if (!b->cs_scaled) return PLSQR_OK;
+    int rc;
+    if ((rc = fill(b, b->cs, b->cols, 1.0))) return rc;
+    b->cs_scaled = false;
+    return PLSQR_OK;
+}
+
 int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out, double *total_ms) {
     int rc;
     if ((rc = ensure_fp32(b))) return rc;
     if (!b->xt && (rc = balloc(b, &b->xt, (size_t)b->cols * b->K))) return rc;
+    const bool colscale = o->precond == PLSQR_PRECOND_COLSCALE;
+    if (o->precond != PLSQR_PRECOND_NONE && !colscale) return set_err(b, PLSQR_ERR_ARG, "unknown preconditioner");
+    if ((rc = colscale ? set_colscale(b, o->precond_power == 0.0 ? 1.0 : o->precond_power) : clear_colscale(b)))
+        return rc;
+    // the ||A|| of the FP64 test2 of state (p, k) on a scaled solve
+    auto anorm_fixed = [&](int p, int k) {
+        const size_t i = (size_t)p * b->Kr + k;
+        return i < b->anorm_refs.size() && b->anorm_refs[i] > 0.0 ? b->anorm_refs[i] : b->fro_p[(size_t)p];
+    };
     const int itn = std::max(0, o->itn);
     const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int S = b->P * b->K;
@@ -659,8 +741,9 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
                 rn[(size_t)st] = std::sqrt(sums[(size_t)st]);
                 an[(size_t)st] = std::sqrt(sums[(size_t)S + st]);
                 checks[(size_t)st] += 1;
-                t2[(size_t)st] = (rn[(size_t)st] > 0.0 && anorm_max[(size_t)st] > 0.0)
-                                     ? an[(size_t)st] / (anorm_max[(size_t)st] * rn[(size_t)st])
+                const double anorm_t = colscale ? anorm_fixed(st / b->K, st % b->K) : anorm_max[(size_t)st];
+                t2[(size_t)st] = (rn[(size_t)st] > 0.0 && anorm_t > 0.0)
+                                     ? an[(size_t)st] / (anorm_t * rn[(size_t)st])
                                      : 0.0;
                 const bool stuck = q.done && q.itn == 0;  // the inner run could not start
                 if (t2[(size_t)st] <= o->tol || total >= itn || stuck) {
@@ -693,7 +776,7 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             r.itn = inner[st];
             r.inner_total = inner[st];
             r.refinements = checks[st];
-            r.anorm = anorm_max[st];
+            r.anorm = colscale ? anorm_fixed(p, k) : anorm_max[st];
             r.rnorm = rn[st];
             r.bnorm = rn0[st];
             r.arnorm = t2[st];
@@ -704,7 +787,7 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
         }
     if (total_ms) *total_ms = t_all;
     (void)hipEventDestroy(e0);
-    return PLSQR_OK;
+    return clear_colscale(b);
 }
 
 int check_handle(plsqr_batch *b) {
@@ -1169,7 +1252,7 @@ int plsqr_batch_aprod(plsqr_batch *b, int mode, int precision, double *x, double
     const long long ny = b->yb[(size_t)b->P], nx = b->xb[(size_t)b->P];
     const bool mixed = precision == PLSQR_PREC_MIXED;
     if (!mixed && precision != PLSQR_PREC_FP64) return set_err(b, PLSQR_ERR_ARG, "unknown precision");
-    if (mixed && (rc = ensure_fp32(b))) return rc;
+    if (mixed && ((rc = ensure_fp32(b)) || (rc = clear_colscale(b)))) return rc;
     const int pi = mixed ? 1 : 0;
     // both operands to the device
     if ((rc = upload(b, b->stage, x, (size_t)nx))) return rc;
@@ -1197,6 +1280,17 @@ int plsqr_batch_aprod(plsqr_batch *b, int mode, int precision, double *x, double
     return PLSQR_OK;
 }
 
+int plsqr_batch_set_anorm_refs(plsqr_batch *b, const double *refs) {
+    int rc;
+    if ((rc = check_handle(b))) return rc;
+    if (!refs) {
+        b->anorm_refs.clear();
+        return PLSQR_OK;
+    }
+    b->anorm_refs.assign(refs, refs + (size_t)b->P * b->Kr);
+    return PLSQR_OK;
+}
+
 int plsqr_batch_solve_device(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *opts, double *x_dev,
                              plsqr_solve_out *out, double *total_ms) {
     int rc;
@@ -1204,8 +1298,8 @@ int plsqr_batch_solve_device(plsqr_batch *b, const double *b_dev, const plsqr_so
     if (!b_dev || !opts || !x_dev || !out) return set_err(b, PLSQR_ERR_ARG, "b, opts, x and out are required");
     plsqr_solve_opts o;
     default_opts(opts, &o);
-    if (o.precond != PLSQR_PRECOND_NONE)
-        return set_err(b, PLSQR_ERR_ARG, "the batched solver has no preconditioner");
+    if (o.precond != PLSQR_PRECOND_NONE && o.precision != PLSQR_PREC_MIXED)
+        return set_err(b, PLSQR_ERR_ARG, "column scaling is offered on the mixed-precision solve");
     if (o.precision == PLSQR_PREC_FP64) {
         rc = solve_fp64(b, b_dev, &o, out, total_ms);
     } else if (o.precision == PLSQR_PREC_MIXED) {

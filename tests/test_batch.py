@@ -273,3 +273,57 @@ def test_batch_autotune_reports_a_valid_shape(P, gpu_available):
         X1, _, _ = bt.solve(B, 250)
     for a, b in zip(X0, X1):     # converged pairs: the launch shape changes rounding only
         assert rel(a, b) < 1e-12
+
+
+@pytest.mark.gpu
+def test_batch_mixed_column_scaling(P, O, gpu_available):
+    """Column-scaled mixed batch (each problem on A_p D_p, D_p =
+    diag(||A_p e_j||^-1.5)): the first inner iterations follow textbook LSQR
+    on A_p D_p; the stop holds on the FP64 test2 of A_p with the FP64 batch's
+    anorm per pair and with ||A_p||_F; the unscaled FP32 operator is restored
+    afterwards (bitwise the same unscaled solve)."""
+    announce("batch mixed, column scaling")
+    import scipy.sparse as sp
+    from test_gpu_parity import lsqr_scaled_np
+    systems, B = make_batch_inputs(P, SPECS[1:], 2)
+    k_it = 8
+    with P.Batch(systems, 2, flags=P.BATCH_TUNE_MIXED) as bt:
+        Xm0, _, _ = bt.solve(B, 3000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)
+        X64, inf64, _ = bt.solve(B, 3000, tol=1e-6)
+        Xk, _, _ = bt.solve(B, k_it, tol=0.0, precision=P.PREC_MIXED, refine_every=k_it,
+                            precond=P.PRECOND_COLSCALE, precond_power=1.5)
+        bt.set_anorm_refs([[inf64[p][k]["anorm"] for k in range(2)] for p in range(len(systems))])
+        Xs, infs, _ = bt.solve(B, 3000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                               precond=P.PRECOND_COLSCALE, precond_power=1.5)
+        bt.set_anorm_refs(None)
+        Xf, inff, _ = bt.solve(B, 3000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                               precond=P.PRECOND_COLSCALE, precond_power=1.5)
+        Xm1, _, _ = bt.solve(B, 3000, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)
+        with pytest.raises(P.PlsqrError):
+            bt.solve(B, 10, precond=P.PRECOND_COLSCALE)  # FP64 has no preconditioner
+    for p in range(len(systems)):
+        for k in range(2):
+            assert np.array_equal(Xm0[p][k], Xm1[p][k]), (p, k)
+    for p, s in enumerate(systems):
+        A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                       sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+        cn = np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel())
+        d = np.where(cn > 0, np.where(cn > 0, cn, 1.0) ** -1.5, 1.0)
+        fro = sp.linalg.norm(A)
+        x_tight, _, _ = O.Oracle(s).lsqr(B[p][0], 600)
+        for k in range(2):
+            bb = np.concatenate([B[p][k], np.zeros(s.m_d)])
+            e_same = rel(Xk[p][k], lsqr_scaled_np(A, bb, k_it, d))
+            e_other = rel(Xk[p][k], lsqr_scaled_np(A, bb, k_it, np.ones(s.n)))
+            assert e_same < 1e-3 and e_same < 0.05 * e_other, (p, k, e_same, e_other)
+            for X, inf, an in ((Xs, infs, inf64[p][k]["anorm"]), (Xf, inff, fro)):
+                i = inf[p][k]
+                assert i["arnorm"] <= 1e-6, (p, k, i)
+                np.testing.assert_allclose(i["anorm"], an, rtol=1e-12)
+                r = bb - A @ X[p][k]
+                assert np.linalg.norm(A.T @ r) / (an * np.linalg.norm(r)) <= 1e-6 * 1.001
+                if k == 0:
+                    e_s, e_64 = rel(X[p][k], x_tight), rel(X64[p][k], x_tight)
+                    assert e_s < 3 * e_64 + 1e-7, (p, e_s, e_64)
+        print(f"problem {p}: iterations FP64 {[inf64[p][k]['itn'] for k in range(2)]}, scaled mixed "
+              f"{[infs[p][k]['itn'] for k in range(2)]}", flush=True)
