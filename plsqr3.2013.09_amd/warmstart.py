@@ -3,7 +3,9 @@
 Every step runs on the GPU through libplsqr_hip.so:
   * pairs: x_true from the generator's blob field of a seed, b = A_k x_true
     + sigma N(0, 1) (noise from numpy's seeded generator), A^T b, all through
-    the batched operator (4 right-hand sides per launch);
+    the batched operator (4 right-hand sides per launch); the network's input
+    is A^T b divided by the diagonal of A^T A (input_kind "jacobi", the
+    default) or A^T b itself ("atb");
   * targets x*: plain FP64 LSQR (the batched solver, 4 right-hand sides at a
     time) run to test2 <= target_tol;
   * training: WarmStartNet.train (bf16 MFMA kernels, Adam);
@@ -56,6 +58,28 @@ def solve_targets(batch, B, tol, max_iters=5000):
             X.append(Xc[0][k])
             its.append(infos[0][k]["itn"])
     return np.array(X), np.array(its)
+
+
+def column_ssq(system):
+    """Sums of squares of the columns of [A_k; D] (the diagonal of A^T A), from
+    the kernel CSC and the damping CSC on the host."""
+    n = system.n
+    kc = np.diff(system.k_col_ptr.astype(np.int64))
+    dc = np.diff(system.dc_col_ptr.astype(np.int64))
+    ssq = np.bincount(np.repeat(np.arange(n), kc), weights=system.k_val ** 2, minlength=n)
+    ssq += np.bincount(np.repeat(np.arange(n), dc), weights=system.dc_val ** 2, minlength=n)
+    return ssq
+
+
+def net_input(ATB, colss, kind):
+    """The network's input: the back-projection A^T b ("atb"), or A^T b
+    divided by the diagonal of A^T A ("jacobi", one Jacobi step of the normal
+    equations from zero), which removes the ray-density weighting."""
+    if kind == "atb":
+        return ATB
+    if kind == "jacobi":
+        return ATB / np.where(colss > 0, colss, 1.0)
+    raise ValueError(f"unknown network input {kind!r}")
 
 
 def train(net, ATB, X, epochs, batch=4, lr=2e-3, lr_final=5e-4, seed=5):
@@ -130,7 +154,7 @@ def summarize(rows):
 
 
 def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, refine_every, log=print,
-        net_seed=3, lr=2e-3, colscale=False, refine_adaptive=False, precond_power=1.0):
+        net_seed=3, lr=2e-3, colscale=False, refine_adaptive=False, precond_power=1.0, input_kind="jacobi"):
     """The whole pipeline on one system; returns a result dict and the trained net."""
     t = {}
     t0 = time.perf_counter()
@@ -140,6 +164,7 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     seeds = [seed0 + i for i in range(n_train + n_eval)]
     t0 = time.perf_counter()
     B, ATB = make_pairs(bt, system, grid, seeds, sigma)
+    ATB = net_input(ATB, column_ssq(system), input_kind)
     t["pairs_s"] = time.perf_counter() - t0
     log(f"[warm] {len(seeds)} pairs ({t['pairs_s']:.1f} s)")
     t0 = time.perf_counter()
