@@ -383,10 +383,10 @@ def main():
                     help="learned warm start: train the network, then time LSQR from its x0 against "
                          "plain FP64 LSQR from zero on held-out right-hand sides")
     ap.add_argument("--train-pairs", type=int, default=512)
-    ap.add_argument("--eval-pairs", type=int, default=8)
-    ap.add_argument("--target-tol", type=float, default=1e-3,
+    ap.add_argument("--eval-pairs", type=int, default=16)
+    ap.add_argument("--target-tol", type=float, default=1e-4,
                     help="test2 the FP64 training targets x* are solved to")
-    ap.add_argument("--epochs", type=float, default=10)
+    ap.add_argument("--epochs", type=float, default=20)
     ap.add_argument("--save-params", default=None, help="write the trained parameters (.npy)")
     ap.add_argument("--net-input", default="jacobi", choices=["atb", "jacobi"],
                     help="network input: A^T b divided by the diagonal of A^T A (x0 error 0.28 at 67^3), "

@@ -45,11 +45,16 @@ def make_pairs(batch, system, grid, seeds, sigma):
     return np.array(Bs), np.array(ATB)
 
 
-def solve_targets(batch, B, tol, max_iters=5000):
-    """Plain FP64 LSQR solutions of the rows of B, 4 at a time; (X, iterations)."""
+def solve_targets(batch, B, tol, max_iters=5000, log=None):
+    """Plain FP64 LSQR solutions of the rows of B, 4 at a time; (X, iterations).
+    log (optional) receives a progress line about every 30 s."""
     K = batch.K
     X, its = [], []
+    t_last = time.perf_counter()
     for i in range(0, len(B), K):
+        if log is not None and time.perf_counter() - t_last > 30.0:
+            log(f"[warm] targets: {i} of {len(B)} solved")
+            t_last = time.perf_counter()
         chunk = B[i:i + K]
         pad = np.zeros((K, B.shape[1]))
         pad[:len(chunk)] = chunk
@@ -99,7 +104,7 @@ def rel(a, b):
 
 
 def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale=False, refine_adaptive=False,
-             precond_power=1.0):
+             precond_power=1.0, log=None):
     """Per held-out right-hand side: FP64 from zero (the reference method),
     FP64 from x0, mixed precision from zero and from x0; with colscale also
     the mixed solves on A D (column scaling), their FP64 test2 taking the
@@ -130,6 +135,8 @@ def evaluate(op, B, X0, tol, refine_every, max_iters=20000, x_ref=None, colscale
         r["x0_err"] = rel(X0[j], xr)
         del t0
         rows.append(r)
+        if log is not None:
+            log(f"[warm] evaluated {j + 1} of {len(B)}")
     return rows
 
 
@@ -168,7 +175,7 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     t["pairs_s"] = time.perf_counter() - t0
     log(f"[warm] {len(seeds)} pairs ({t['pairs_s']:.1f} s)")
     t0 = time.perf_counter()
-    X, its = solve_targets(bt, B[:n_train], target_tol)
+    X, its = solve_targets(bt, B[:n_train], target_tol, log=log)
     t["targets_s"] = time.perf_counter() - t0
     log(f"[warm] {n_train} FP64 targets to test2 <= {target_tol:g}: iterations median {np.median(its):.0f} "
         f"({t['targets_s']:.1f} s)")
@@ -185,7 +192,7 @@ def run(system, grid, n_train, n_eval, sigma, seed0, target_tol, epochs, tol, re
     t["predict_ms_per_sample"] = 1000 * t["predict_s"] / max(n_eval, 1)
     t0 = time.perf_counter()
     rows = evaluate(op, B[n_train:], X0, tol, refine_every, colscale=colscale, refine_adaptive=refine_adaptive,
-                    precond_power=precond_power)
+                    precond_power=precond_power, log=log)
     t["eval_s"] = time.perf_counter() - t0
     op.close()
     res = {"summary": summarize(rows), "rows": rows, "times": t,
