@@ -981,15 +981,28 @@ static void launch_cols_tiled32(plsqr_op *op, const float *u, const Scal *sc) {
 }
 
 // FP32 aprod on internal-order vectors (x: n, y: m floats) with the plain Scal
+// FP32 per-ray combine with rt_lc lanes per ray
+static void launch_combine32(plsqr_op *op, const float *x, float *y, const plsqr::Scal *sc) {
+    using namespace plsqr;
+#define COMB32(LC)                                                                                                 \
+    hipLaunchKernelGGL((k_rows_tiled_combine32<LC, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,  \
+                       op->tS, op->bounds, op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp, op->ci,  \
+                       op->rv32, x, y, sc, op->part)
+    switch (op->rt_lc) {
+        case 2: COMB32(2); break;
+        case 8: COMB32(8); break;
+        default: COMB32(4); break;
+    }
+#undef COMB32
+}
+
 static int aprod32_internal(plsqr_op *op, int mode, float *x, float *y) {
     using namespace plsqr;
     const Scal *sp = op->sc_plain;
     if (mode == 1) {
         if (op->rtiled) {
             launch_pass1<float>(op, op->rt_val32, x, sp, (float *)op->rt_segv);
-            hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m,
-                               op->tS, op->bounds, op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp,
-                               op->ci, op->rv32, x, y, sp, op->part);
+            launch_combine32(op, x, y, sp);
         } else {
             hipLaunchKernelGGL((k_rows32<16, BS>), dim3(op->G1), dim3(BS), 0, op->stream, op->m_k, op->m, op->rp,
                                op->ci, op->rv32, x, y, sp, op->part);
@@ -1430,9 +1443,7 @@ static int enqueue_iteration32(plsqr_op *op) {
     dim3 g1(op->G1), b(BS);
     if (op->rtiled) {
         launch_pass1<float>(op, op->rt_val32, op->v32, (const Scal *)op->sc, (float *)op->rt_segv);
-        hipLaunchKernelGGL((k_rows_tiled_combine32<4, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->tS, op->bounds,
-                           op->rt_rptr, op->rt_rseg, (const float *)op->rt_segv, op->rp, op->ci, op->rv32, op->v32,
-                           op->u32, (const Scal *)op->sc, op->part);
+        launch_combine32(op, op->v32, op->u32, (const Scal *)op->sc);
     } else switch (op->L1) {
         case 8: hipLaunchKernelGGL((k_rows32<8, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;
         case 32: hipLaunchKernelGGL((k_rows32<32, BS>), g1, b, 0, op->stream, op->m_k, op->m, op->rp, op->ci, op->rv32, op->v32, op->u32, (const Scal *)op->sc, op->part); break;

@@ -119,7 +119,26 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int e0 = beg > q0 ? beg : q0;
             const int e1 = end < q0 + cnt ? end : q0 + cnt;
-            for (int e = e0; e < e1; ++e) dot32_add(acc, bv[sh + e], xl[bx[sh + e]]);
+            // four, then two entries per step: the exact products formed before
+            // the adds (in entry order), so several LDS gathers are in flight
+            int e = e0;
+            for (; e + 3 < e1; e += 4) {
+                const double p0 = (double)bv[sh + e] * (double)xl[bx[sh + e]];
+                const double p1 = (double)bv[sh + e + 1] * (double)xl[bx[sh + e + 1]];
+                const double p2 = (double)bv[sh + e + 2] * (double)xl[bx[sh + e + 2]];
+                const double p3 = (double)bv[sh + e + 3] * (double)xl[bx[sh + e + 3]];
+                acc += p0;
+                acc += p1;
+                acc += p2;
+                acc += p3;
+            }
+            for (; e + 1 < e1; e += 2) {
+                const double p0 = (double)bv[sh + e] * (double)xl[bx[sh + e]];
+                const double p1 = (double)bv[sh + e + 1] * (double)xl[bx[sh + e + 1]];
+                acc += p0;
+                acc += p1;
+            }
+            if (e < e1) dot32_add(acc, bv[sh + e], xl[bx[sh + e]]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
