@@ -26,3 +26,22 @@ def test_trained_network_reduces_lsqr_iterations(P, gpu_available):
     for r in rows:
         assert r["mixed_warm"]["test2"] <= 1e-6 and r["mixed_zero"]["test2"] <= 1e-6
         assert r["fp64_warm"]["xerr"] < 1e-4 and r["mixed_warm"]["xerr"] < 1e-3
+
+
+def test_network_input_is_the_jacobi_step(P):
+    """Host side, no GPU: the network's input is A^T b divided by the column
+    sums of squares of [A_k; D] (the diagonal of A^T A), checked against
+    scipy on a small system; "atb" passes A^T b through; unknown kinds fail."""
+    import importlib
+    import scipy.sparse as sp
+    W = importlib.import_module(P.__name__ + ".warmstart")
+    s = P.generate(9, 8, 7, 500, 1, 5, 0.01, 0.05)
+    A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                   sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+    ssq = W.column_ssq(s)
+    np.testing.assert_allclose(ssq, np.asarray(A.multiply(A).sum(axis=0)).ravel(), rtol=1e-13)
+    atb = (A.T @ np.concatenate([s.b, np.zeros(s.m_d)]))[None, :]
+    np.testing.assert_array_equal(W.net_input(atb, ssq, "atb"), atb)
+    np.testing.assert_allclose(W.net_input(atb, ssq, "jacobi"), atb / ssq, rtol=1e-15)
+    with pytest.raises(ValueError):
+        W.net_input(atb, ssq, "other")
