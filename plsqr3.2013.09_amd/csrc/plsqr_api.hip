@@ -83,7 +83,7 @@ struct plsqr_op {
     float *fx = nullptr, *fy = nullptr;  // caller-order FP32 staging of the host FP32 aprod
     plsqr::Scal *sc_plain = nullptr;     // ib = 1, alpha = -1, vscale = 1: plain y += A x, x += A^T y
     double *dx = nullptr;
-    // column scaling of the mixed solve: d = 1 / ||A e_j|| (device column
+    // column scaling of the mixed solve: d = ||A e_j||^-power (device column
     // order), ||A||_F, and the scales the FP32 combine applies during a solve
     double *cscale = nullptr, *colss = nullptr;  // scales, column sums of squares
     double cs_power = 0.0;                        // exponent the scales were made for

@@ -586,21 +586,15 @@ int true_norms(plsqr_batch *b, const double *xv, std::vector<double> &sums) {
     return PLSQR_OK;
 }
 
-// Mixed precision per (system, right-hand side), as the single-system solve:
-// a continuous inner FP32 LSQR on r = [b;0] - A x for every live pair, an
-// FP64 check of xt = x + dx every refine_every launches (a pair stops there
-// once its FP64 test2 = ||A^T r|| / (anorm ||r||) <= tol, anorm the largest
-// inner estimate), and a restart on the refined residuals (x = xt) when
-// every pair still live has stopped its inner run without meeting tol.
 // Column sums of squares of every problem's [A_k; D] (once per handle: the
 // FP64 mode 2 on the squared values with u = 1), ||A_p||_F, and the scales
 // cs = colss^(-power/2) for a scaled mixed solve.
 int set_colscale(plsqr_batch *b, double power) {
     int rc;
     if (!(power > 0.0) || !std::isfinite(power)) return set_err(b, PLSQR_ERR_ARG, "precond_power must be > 0");
-    if (!b->colss) {
+    if (b->fro_p.size() != (size_t)b->P) {  // not formed yet (or an earlier attempt failed)
         double *sq_s = nullptr, *sq_d = nullptr;
-        if ((rc = balloc(b, &b->colss, (size_t)b->cols)) || (rc = balloc(b, &sq_s, (size_t)b->slots)) ||
+        if ((!b->colss && (rc = balloc(b, &b->colss, (size_t)b->cols))) || (rc = balloc(b, &sq_s, (size_t)b->slots)) ||
             (rc = balloc(b, &sq_d, (size_t)std::max(b->dnnz, 1LL))))
             return rc;
         hipLaunchKernelGGL(kb_square, dim3(grid_for(b->slots, 256, b->cu * 8)), dim3(256), 0, b->stream, b->slots,
@@ -645,6 +639,14 @@ int clear_colscale(plsqr_batch *b) {
     return PLSQR_OK;
 }
 
+// Mixed precision per (system, right-hand side), as the single-system solve:
+// a continuous inner FP32 LSQR on r = [b;0] - A x for every live pair, an
+// FP64 check of xt = x + dx every refine_every launches (a pair stops there
+// once its FP64 test2 = ||A^T r|| / (anorm ||r||) <= tol, anorm the largest
+// inner estimate), and a restart on the refined residuals (x = xt) when
+// every pair still live has stopped its inner run without meeting tol.
+// With PLSQR_PRECOND_COLSCALE every problem's inner LSQR runs on A_p D_p
+// and pair i's test2 takes anorm_refs[i] (or ||A_p||_F) as its anorm.
 int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, plsqr_solve_out *out, double *total_ms) {
     int rc;
     if ((rc = ensure_fp32(b))) return rc;
