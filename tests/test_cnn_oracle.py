@@ -1,4 +1,4 @@
-"""Pins the CPU oracle of the warm-start network before it judges the GPU."""
+"""Pins the CPU oracle of the warm-start network before it checks the GPU kernels."""
 import os
 import sys
 
