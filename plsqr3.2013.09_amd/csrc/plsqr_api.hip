@@ -97,6 +97,11 @@ struct plsqr_op {
     int *rperm = nullptr, *rinv = nullptr;  // new -> old, old -> new (kernel rows)
     double *hy2 = nullptr;                  // m, new-order staging
     double *bint = nullptr;                 // m_k, b in the new order
+    double *xt = nullptr;                   // n: the mixed solve's checked iterate x + dx
+    // pinned host staging owned by the handle (hipHostMalloc at create, freed
+    // at destroy): every read of device state or partial sums goes through it
+    plsqr::Scal *hscal = nullptr;           // copy of the device Scal
+    double *hpart = nullptr;                // part_len doubles
     std::vector<void *> allocs;
     size_t bytes = 0;
     // launch shapes
@@ -159,6 +164,82 @@ int env_int(const char *name, int dflt) {
     const char *e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
 }
+
+// Pinned host staging of the handle, allocated once at create.
+int alloc_host_staging(plsqr_op *op) {
+    hipError_t e = hipHostMalloc((void **)&op->hscal, sizeof(plsqr::Scal));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&op->hpart, sizeof(double) * std::max(op->part_len, 1));
+    if (e != hipSuccess)
+        return set_err(op, PLSQR_ERR_NOMEM, std::string("hipHostMalloc of the staging buffers: ") + hipGetErrorString(e));
+    return PLSQR_OK;
+}
+
+void free_host_staging(plsqr_op *op) {
+    if (op->hscal) (void)hipHostFree(op->hscal);
+    if (op->hpart) (void)hipHostFree(op->hpart);
+    op->hscal = nullptr;
+    op->hpart = nullptr;
+}
+
+// The device Scal copied to the host through the pinned staging (the stream
+// is synchronised before returning).
+int read_scal(plsqr_op *op, plsqr::Scal *h) {
+    HIP_TRY(op, hipMemcpyAsync(op->hscal, op->sc, sizeof(plsqr::Scal), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    std::memcpy(h, op->hscal, sizeof(plsqr::Scal));
+    return PLSQR_OK;
+}
+
+// Sum, in a fixed order on the host, of count <= part_len device partials.
+int host_sum(plsqr_op *op, const double *part, int count, double *out) {
+    if (count < 0 || count > op->part_len) return set_err(op, PLSQR_ERR_STATE, "partial count beyond the staging buffer");
+    HIP_TRY(op, hipMemcpyAsync(op->hpart, part, sizeof(double) * count, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    double s = 0.0;
+    for (int i = 0; i < count; ++i) s += op->hpart[i];
+    *out = s;
+    return PLSQR_OK;
+}
+
+// Owns what one solve creates (timing events, the captured iteration graph
+// and its executable) and releases it on every exit path, after the work
+// already on the stream has drained; also clears the column scales the FP32
+// combine reads during a scaled solve.
+struct SolveScope {
+    plsqr_op *op;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    explicit SolveScope(plsqr_op *o) : op(o) {}
+    SolveScope(const SolveScope &) = delete;
+    SolveScope &operator=(const SolveScope &) = delete;
+    int create_events() {
+        HIP_TRY(op, hipEventCreate(&e0));
+        HIP_TRY(op, hipEventCreate(&e1));
+        return PLSQR_OK;
+    }
+    // capture one iteration enqueued by f() and instantiate it
+    template <class F>
+    int capture(F &&f) {
+        HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = f();
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(op->stream, &g);
+        graph = g;
+        if (rc) return rc;
+        if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+        HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return PLSQR_OK;
+    }
+    ~SolveScope() {
+        if (op->stream) (void)hipStreamSynchronize(op->stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        op->cs_active = nullptr;
+    }
+};
 
 // -------------------------------------------------------------------------
 // kernel dispatch over the lanes-per-row template parameter
@@ -756,6 +837,7 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         for (void *p : op->allocs) (void)hipFree(p);
         op->allocs.clear();
         op->bytes = 0;
+        free_host_staging(op);
         if (op->own_stream && op->stream) (void)hipStreamDestroy(op->stream);
         op->stream = nullptr;
         op->own_stream = false;
@@ -867,7 +949,8 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     if ((rc = dalloc(op, &op->u, op->m)) || (rc = dalloc(op, &op->v, op->n)) || (rc = dalloc(op, &op->x, op->n)) ||
         (rc = dalloc(op, &op->w, op->n)) || (rc = dalloc(op, &op->bst, op->m_k)) ||
         (rc = dalloc(op, &op->P, (size_t)op->nsl * op->n)) || (rc = dalloc(op, &op->part, op->part_len)) ||
-        (rc = dalloc(op, &op->sc, 1)) || (rc = dalloc(op, &op->hx, op->n)) || (rc = dalloc(op, &op->hy, op->m)))
+        (rc = dalloc(op, &op->sc, 1)) || (rc = dalloc(op, &op->hx, op->n)) || (rc = dalloc(op, &op->hy, op->m)) ||
+        (rc = alloc_host_staging(op)))
         return fail(rc);
     e = hipMemsetAsync(op->u, 0, sizeof(double) * op->m, op->stream);
     if (e == hipSuccess) e = hipMemsetAsync(op->v, 0, sizeof(double) * op->n, op->stream);
@@ -881,6 +964,7 @@ void plsqr_op_destroy(plsqr_op *op) {
     if (!op) return;
     if (op->stream) (void)hipStreamSynchronize(op->stream);
     for (void *p : op->allocs) (void)hipFree(p);
+    free_host_staging(op);
     if (op->own_stream && op->stream) (void)hipStreamDestroy(op->stream);
     delete op;
 }
@@ -1327,61 +1411,35 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
                        (const Scal *)op->sc, op->v, w);
     LAUNCH_CHECK(op);
     // main loop, timed with events on the handle's stream
-    hipEvent_t e0, e1;
-    HIP_TRY(op, hipEventCreate(&e0));
-    HIP_TRY(op, hipEventCreate(&e1));
-    HIP_TRY(op, hipEventRecord(e0, op->stream));
+    SolveScope scope(op);
+    if ((rc = scope.create_events())) return rc;
+    HIP_TRY(op, hipEventRecord(scope.e0, op->stream));
     const bool use_graph = env_int("PLSQR_GRAPH", 1) != 0 && itn > 1;
     const int check_every = o->tol > 0.0 ? std::max(1, env_int("PLSQR_CHECK_EVERY", 8)) : 0;
-    int *done_h = nullptr;
-    if (check_every) HIP_TRY(op, hipHostMalloc((void **)&done_h, sizeof(int)));
-    if (use_graph) {
-        hipGraph_t graph;
-        hipGraphExec_t exec;
-        HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_iteration(op, x, w, se, trace_d);
-        hipGraph_t g2 = nullptr;
-        hipError_t ce = hipStreamEndCapture(op->stream, &g2);
-        if (rc) return rc;
-        if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
-        graph = g2;
-        HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        for (int it = 0; it < itn; ++it) {
-            HIP_TRY(op, hipGraphLaunch(exec, op->stream));
-            if (check_every && (it + 1) % check_every == 0) {
-                HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
-                HIP_TRY(op, hipStreamSynchronize(op->stream));
-                if (*done_h) break;
-            }
+    if (use_graph && (rc = scope.capture([&] { return enqueue_iteration(op, x, w, se, trace_d); }))) return rc;
+    for (int it = 0; it < itn; ++it) {
+        if (use_graph) {
+            HIP_TRY(op, hipGraphLaunch(scope.exec, op->stream));
+        } else if ((rc = enqueue_iteration(op, x, w, se, trace_d))) {
+            return rc;
         }
-        HIP_TRY(op, hipEventRecord(e1, op->stream));
-        HIP_TRY(op, hipStreamSynchronize(op->stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-    } else {
-        for (int it = 0; it < itn; ++it) {
-            if ((rc = enqueue_iteration(op, x, w, se, trace_d))) return rc;
-            if (check_every && (it + 1) % check_every == 0) {
-                HIP_TRY(op, hipMemcpyAsync(done_h, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
-                HIP_TRY(op, hipStreamSynchronize(op->stream));
-                if (*done_h) break;
-            }
+        if (check_every && (it + 1) % check_every == 0) {
+            // the stop flag through the handle's pinned staging
+            HIP_TRY(op, hipMemcpyAsync(&op->hscal->done, &op->sc->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+            HIP_TRY(op, hipStreamSynchronize(op->stream));
+            if (op->hscal->done) break;
         }
-        HIP_TRY(op, hipEventRecord(e1, op->stream));
-        HIP_TRY(op, hipStreamSynchronize(op->stream));
     }
-    if (done_h) (void)hipHostFree(done_h);
+    HIP_TRY(op, hipEventRecord(scope.e1, op->stream));
+    HIP_TRY(op, hipStreamSynchronize(op->stream));
     float ms = 0.f;
-    HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(op, hipEventElapsedTime(&ms, scope.e0, scope.e1));
     // the loop leaves the last 1/beta of u pending (lsqr.c:538); apply it so
     // that the resident u is the reference's u after the loop
     hipLaunchKernelGGL(k_scale_pending, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream,
                        (long long)op->m, (const Scal *)op->sc, op->u);
     LAUNCH_CHECK(op);
-    HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
-    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if ((rc = read_scal(op, &h))) return rc;
     if (se) {
         // se = rnorm / sqrt(m - n, or m when damped, or 1) * sqrt(var)  (lsqr.c:878-891)
         double t = 1.0;
@@ -1460,17 +1518,6 @@ static int enqueue_iteration32(plsqr_op *op) {
                        op->w, op->dx, op->part);
     hipLaunchKernelGGL(k_s_est, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G6, (double *)nullptr);
     LAUNCH_CHECK(op);
-    return PLSQR_OK;
-}
-
-// sum of a device partial array (fixed order on the host)
-static int host_sum(plsqr_op *op, const double *part, int count, double *out) {
-    std::vector<double> h((size_t)count);
-    HIP_TRY(op, hipMemcpyAsync(h.data(), part, sizeof(double) * count, hipMemcpyDeviceToHost, op->stream));
-    HIP_TRY(op, hipStreamSynchronize(op->stream));
-    double s = 0.0;
-    for (int i = 0; i < count; ++i) s += h[(size_t)i];
-    *out = s;
     return PLSQR_OK;
 }
 
@@ -1578,11 +1625,12 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     using namespace plsqr;
     int rc;
     if ((rc = ensure_fp32(op))) return rc;
-    if (!op->x0i && ((rc = dalloc(op, &op->x0i, op->n)) || (rc = dalloc(op, &op->r0, op->m)))) return rc;
-    double *xt = op->x0i == x_init ? op->r0 : op->x0i;  // n doubles not aliasing x_init
+    if (!op->xt && (rc = dalloc(op, &op->xt, op->n))) return rc;
+    double *const xt = op->xt;  // the checked iterate x + dx (n doubles, never x_init)
     const bool colscale = o->precond == PLSQR_PRECOND_COLSCALE;
     if (o->precond != PLSQR_PRECOND_NONE && !colscale) return set_err(op, PLSQR_ERR_ARG, "unknown preconditioner");
     if (colscale && (rc = ensure_colscale(op, o->precond_power == 0.0 ? 1.0 : o->precond_power))) return rc;
+    SolveScope scope(op);  // events, graph, exec; resets cs_active on every exit
     op->cs_active = colscale ? op->cscale : nullptr;
     const double anorm_fixed = colscale ? (o->anorm_ref > 0.0 ? o->anorm_ref : op->fro) : 0.0;
     const int itn = std::max(0, o->itn);
@@ -1593,15 +1641,10 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
     else
         hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->x);
     LAUNCH_CHECK(op);
-    hipEvent_t e0, e1;
-    HIP_TRY(op, hipEventCreate(&e0));
-    HIP_TRY(op, hipEventCreate(&e1));
-    HIP_TRY(op, hipEventRecord(e0, op->stream));
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    if ((rc = scope.create_events())) return rc;
+    HIP_TRY(op, hipEventRecord(scope.e0, op->stream));
     const int gs = std::min(op->part_len / 2, grid_for(op->m, BS, op->cu * 4));
-    Scal *hs = nullptr;  // pinned copy of the inner state for the polls
-    HIP_TRY(op, hipHostMalloc((void **)&hs, sizeof(Scal)));
+    const Scal *hs = op->hscal;  // pinned copy of the inner state for the polls
     const bool adaptive = o->refine_adaptive != 0 && o->tol > 0.0;
     double ratio = -1.0;  // FP64 test2 / inner estimate at the last check
     double anorm_max = 0.0, rn = 0.0, an = 0.0, test2 = INFINITY;
@@ -1647,14 +1690,7 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
                            op->w);
         LAUNCH_CHECK(op);
         ++starts;
-        if (!exec) {
-            HIP_TRY(op, hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_iteration32(op);
-            hipError_t ce = hipStreamEndCapture(op->stream, &graph);
-            if (rc) return rc;
-            if (ce != hipSuccess) return set_err(op, PLSQR_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
-            HIP_TRY(op, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        }
+        if (!scope.exec && (rc = scope.capture([&] { return enqueue_iteration32(op); }))) return rc;
         int inner_prev = 0;
         for (;;) {
             // a chunk of inner iterations, polling the inner state every 8.
@@ -1664,17 +1700,16 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
             const int k = std::min(adaptive && ratio > 0.0 ? 4 * period : period, itn - total);
             int launched = 0;
             for (int i = 0; i < k; ++i) {
-                HIP_TRY(op, hipGraphLaunch(exec, op->stream));
+                HIP_TRY(op, hipGraphLaunch(scope.exec, op->stream));
                 ++launched;
                 if ((i + 1) % 8 == 0 && i + 1 < k) {
-                    HIP_TRY(op, hipMemcpyAsync(hs, op->sc, sizeof(Scal), hipMemcpyDeviceToHost, op->stream));
+                    HIP_TRY(op, hipMemcpyAsync(op->hscal, op->sc, sizeof(Scal), hipMemcpyDeviceToHost, op->stream));
                     HIP_TRY(op, hipStreamSynchronize(op->stream));
                     if (hs->done) break;
                     if (adaptive && ratio > 0.0 && hs->test2 * ratio <= o->tol) break;
                 }
             }
-            HIP_TRY(op, hipMemcpyAsync(&h, op->sc, sizeof(h), hipMemcpyDeviceToHost, op->stream));
-            HIP_TRY(op, hipStreamSynchronize(op->stream));
+            if ((rc = read_scal(op, &h))) return rc;
             const int inner = std::max(h.itn, 0);
             total += inner - inner_prev;
             inner_prev = inner;
@@ -1700,16 +1735,10 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
             }
         }
     }
-    HIP_TRY(op, hipEventRecord(e1, op->stream));
+    HIP_TRY(op, hipEventRecord(scope.e1, op->stream));
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     float ms = 0.f;
-    HIP_TRY(op, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipHostFree(hs);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    op->cs_active = nullptr;
+    HIP_TRY(op, hipEventElapsedTime(&ms, scope.e0, scope.e1));
     if (out) {
         memset(out, 0, sizeof(*out));
         out->itn = total;
@@ -1912,39 +1941,31 @@ int plsqr_residual_norms(plsqr_op *op, const double *b_host, const double *x_hos
         if (rcr) return rcr;
     }
     LAUNCH_CHECK(op);
-    std::vector<double> hp((size_t)op->part_len);
-    auto host_sum = [&](int count) {
-        double s = 0.0;
-        for (int i = 0; i < count; ++i) s += hp[(size_t)i];
-        return s;
-    };
+    int rc;
+    double rn2 = 0.0, an2 = 0.0, af2 = 0.0;
     const int gs = std::min(op->part_len, grid_for(op->m, BS, op->cu * 4));
     hipLaunchKernelGGL((k_ssq<BS>), dim3(gs), dim3(BS), 0, op->stream, (long long)op->m, op->hy, op->part);
     LAUNCH_CHECK(op);
-    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * gs, hipMemcpyDeviceToHost, op->stream));
-    HIP_TRY(op, hipStreamSynchronize(op->stream));
-    const double rn = std::sqrt(host_sum(gs));
+    if ((rc = host_sum(op, op->part, gs, &rn2))) return rc;
+    const double rn = std::sqrt(rn2);
     // A^T r into hx
     hipLaunchKernelGGL(k_fill, dim3(grid_for(op->n, 256, op->cu * 4)), dim3(256), 0, op->stream, (long long)op->n, 0.0,
                        op->hx);
     LAUNCH_CHECK(op);
-    int rc = aprod2_dev(op, op->hx, op->hy);
-    if (rc) return rc;
+    if ((rc = aprod2_dev(op, op->hx, op->hy))) return rc;
     const int gn = std::min(op->part_len, grid_for(op->n, BS, op->cu * 4));
     hipLaunchKernelGGL((k_ssq<BS>), dim3(gn), dim3(BS), 0, op->stream, (long long)op->n, op->hx, op->part);
     LAUNCH_CHECK(op);
-    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * gn, hipMemcpyDeviceToHost, op->stream));
-    HIP_TRY(op, hipStreamSynchronize(op->stream));
-    const double an = std::sqrt(host_sum(gn));
+    if ((rc = host_sum(op, op->part, gn, &an2))) return rc;
+    const double an = std::sqrt(an2);
     const long long nnz = op->nnz_k + op->nnz_d;
     const int ga = std::min(op->part_len, grid_for(nnz, BS, op->cu * 4));
     hipLaunchKernelGGL((k_ssq<BS>), dim3(ga), dim3(BS), 0, op->stream, nnz, op->rv, op->part);
     LAUNCH_CHECK(op);
-    HIP_TRY(op, hipMemcpyAsync(hp.data(), op->part, sizeof(double) * ga, hipMemcpyDeviceToHost, op->stream));
-    HIP_TRY(op, hipStreamSynchronize(op->stream));
+    if ((rc = host_sum(op, op->part, ga, &af2))) return rc;
     if (rnorm) *rnorm = rn;
     if (arnorm) *arnorm = an;
-    if (anorm_f) *anorm_f = std::sqrt(host_sum(ga));
+    if (anorm_f) *anorm_f = std::sqrt(af2);
     return PLSQR_OK;
 }
 

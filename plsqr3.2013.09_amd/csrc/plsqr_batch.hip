@@ -93,6 +93,7 @@ struct plsqr_batch {
     int part_len = 0;
     Scal *sc = nullptr, *sc_plain = nullptr;
     int *live_d = nullptr, *live_h = nullptr;
+    void *hstage = nullptr;  // pinned: the states (S Scal) or the 2 S state sums on their way to the host
     // column scaling of the mixed solve: cs (cols) holds the scales during a
     // scaled solve and 1 otherwise (the FP32 combine always reads it);
     // colss the column sums of squares, fro_p each problem's ||A_p||_F,
@@ -511,6 +512,36 @@ int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, i
     return PLSQR_OK;
 }
 
+// every state (S Scal) to the host through the pinned staging
+int read_states(plsqr_batch *b, std::vector<Scal> &h) {
+    const size_t S = (size_t)b->P * b->K;
+    h.resize(S);
+    BT_TRY(b, hipMemcpyAsync(b->hstage, b->sc, sizeof(Scal) * S, hipMemcpyDeviceToHost, b->stream));
+    BT_TRY(b, hipStreamSynchronize(b->stream));
+    std::memcpy(h.data(), b->hstage, sizeof(Scal) * S);
+    return PLSQR_OK;
+}
+
+// Timing events of one solve, destroyed on every exit path once the stream
+// has drained.
+struct EventPair {
+    plsqr_batch *b;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    explicit EventPair(plsqr_batch *o) : b(o) {}
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    int create() {
+        BT_TRY(b, hipEventCreate(&e0));
+        BT_TRY(b, hipEventCreate(&e1));
+        return PLSQR_OK;
+    }
+    ~EventPair() {
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 float elapsed(hipEvent_t a, hipEvent_t c) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, c) != hipSuccess) return 0.f;
@@ -523,10 +554,9 @@ int solve_fp64(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, p
     const int S = b->P * b->K;
     std::vector<Scal> h((size_t)S, fresh_state(o, itn));
     if ((rc = upload(b, b->sc, h.data(), h.size()))) return rc;
-    hipEvent_t e0, e_start;
-    if ((rc = ensure_events(b, 2))) return rc;
-    BT_TRY(b, hipEventCreate(&e0));
-    BT_TRY(b, hipEventCreate(&e_start));
+    EventPair ev(b);
+    if ((rc = ensure_events(b, 2)) || (rc = ev.create())) return rc;
+    hipEvent_t e0 = ev.e0, e_start = ev.e1;
     BT_TRY(b, hipEventRecord(e0, b->stream));
     if ((rc = launch_load_b<double>(b, b_dev, b->u64)) || (rc = fill(b, b->v64, b->cols * b->K, 0.0)) ||
         (rc = fill(b, b->x, b->cols * b->K, 0.0)) || (rc = fill(b, b->w, b->cols * b->K, 0.0)) ||
@@ -539,8 +569,7 @@ int solve_fp64(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, p
         if ((rc = get_graph<double>(b, 0, b->rv, b->sval, b->u64, b->v64, b->x, &exec))) return rc;
         if ((rc = run_launches(b, exec, itn, 0, &launched))) return rc;
     }
-    BT_TRY(b, hipMemcpyAsync(h.data(), b->sc, sizeof(Scal) * h.size(), hipMemcpyDeviceToHost, b->stream));
-    BT_TRY(b, hipStreamSynchronize(b->stream));
+    if ((rc = read_states(b, h))) return rc;
     const float t_start = elapsed(e0, e_start);
     float t_all = t_start;
     for (int p = 0; p < b->P; ++p)
@@ -566,8 +595,6 @@ int solve_fp64(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, p
         }
     if (launched > 0) t_all = std::max(t_all, elapsed(e0, b->evs[(size_t)launched - 1]));
     if (total_ms) *total_ms = t_all;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e_start);
     return PLSQR_OK;
 }
 
@@ -581,8 +608,9 @@ int true_norms(plsqr_batch *b, const double *xv, std::vector<double> &sums) {
         (rc = launch_state_sums(b, b->r_cch, b->sums + S)))
         return rc;
     sums.resize((size_t)S * 2);
-    BT_TRY(b, hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, b->stream));
+    BT_TRY(b, hipMemcpyAsync(b->hstage, b->sums, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, b->stream));
     BT_TRY(b, hipStreamSynchronize(b->stream));
+    std::memcpy(sums.data(), b->hstage, sizeof(double) * 2 * S);
     return PLSQR_OK;
 }
 
@@ -670,8 +698,9 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
         fin_ev((size_t)S, -1);
     for (int p = 0; p < b->P; ++p)
         for (int k = b->Kr; k < b->K; ++k) odone[(size_t)p * b->K + k] = 1;
-    hipEvent_t e0;
-    BT_TRY(b, hipEventCreate(&e0));
+    EventPair ev(b);
+    if ((rc = ev.create())) return rc;
+    const hipEvent_t e0 = ev.e0;
     BT_TRY(b, hipEventRecord(e0, b->stream));
     if ((rc = launch_load_b<double>(b, b_dev, b->b0)) || (rc = fill(b, b->x, b->cols * b->K, 0.0)) ||
         (rc = true_norms(b, b->x, sums)))
@@ -727,7 +756,7 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             if ((rc = run_launches(b, exec, k, ev_next, &launched))) return rc;
             ev_next += (size_t)launched;
             total += launched;
-            BT_TRY(b, hipMemcpyAsync(h.data(), b->sc, sizeof(Scal) * h.size(), hipMemcpyDeviceToHost, b->stream));
+            if ((rc = read_states(b, h))) return rc;
             hipLaunchKernelGGL(kb_add3, dim3(grid_for(b->cols * b->K, 256, b->cu * 8)), dim3(256), 0, b->stream,
                                b->cols * b->K, (const double *)b->x, (const double *)b->dx, b->xt);
             BT_LAUNCH(b);
@@ -788,7 +817,6 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             r.elapsed_ms = fe >= 0 && (size_t)fe < check_ev.size() ? elapsed(e0, check_ev[(size_t)fe]) : t_all;
         }
     if (total_ms) *total_ms = t_all;
-    (void)hipEventDestroy(e0);
     return clear_colscale(b);
 }
 
@@ -1103,6 +1131,7 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->sc_plain, (size_t)S)) || (rc = balloc(b, &b->live_d, 1)))
         return rc;
     BT_TRY(b, hipHostMalloc((void **)&b->live_h, sizeof(int)));
+    BT_TRY(b, hipHostMalloc(&b->hstage, std::max(sizeof(Scal), 2 * sizeof(double)) * (size_t)S));
     // states of a plain operator application: y*1*1 + A x, v*1 + A^T u
     Scal z;
     memset(&z, 0, sizeof(z));
@@ -1188,6 +1217,7 @@ void plsqr_batch_destroy(plsqr_batch *b) {
     for (hipEvent_t e : b->evs) (void)hipEventDestroy(e);
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->live_h) (void)hipHostFree(b->live_h);
+    if (b->hstage) (void)hipHostFree(b->hstage);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

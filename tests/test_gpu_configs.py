@@ -1,0 +1,206 @@
+"""Every benchmark workload at its own size, on the GPU, through the C-ABI.
+
+  * 67^3 grid / 2,000,000 rays, FP64, 100 iterations: the exact-order
+    operator's x and trace bitwise the oracle's; the default operator's x
+    within 100x the oracle's own sensitivity at that iteration (x after a
+    fixed, mid-convergence iteration count moves with any rounding change;
+    the bar is measured, not assumed) and its rnorm trace close to the
+    oracle's;
+  * the same system in mixed precision on A D (column scaling, p = 1.5,
+    adaptive FP64 checks) and without scaling, each run to test2 <= 1e-6:
+    the true FP64 ||A^T r|| / (anorm ||r||) confirms the stop;
+  * 24 problems of 20^3..48^3 with 4 right-hand sides each, FP64 and mixed
+    on A_p D_p, every pair to test2 <= 1e-6 by its true FP64 residual; the
+    two smallest problems' pairs against the oracle run alone;
+  * the learned warm start with the committed 67^3 parameters on 4
+    held-out right-hand sides: fewer iterations than from zero, every x0
+    kept by the fallback test, every solve confirmed by its true test2.
+
+The oracle's 67^3 runs take about 30 s each on one core; they run on host
+threads (ctypes releases the GIL) while the GPU works.
+"""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+TOL = 1e-6
+
+
+def announce(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pool():
+    with ThreadPoolExecutor(3) as ex:
+        yield ex
+
+
+@pytest.fixture(scope="module")
+def big(P, O, gpu_available, pool):
+    announce("generating grid67_rays2m")
+    s = P.generate_config("grid67_rays2m")
+    announce(f"m_k={s.m_k} m_d={s.m_d} n={s.n} nnz_k={s.nnz_k}")
+    orc = O.Oracle(s)
+    b2 = s.b * (1 + 1e-15 * np.random.default_rng(1).standard_normal(s.b.size))
+    ref = pool.submit(orc.lsqr, np.array(s.b), 100)
+    pert = pool.submit(orc.lsqr, b2, 100, 0.0, 0.0, False, False)
+    return s, orc, ref, pert
+
+
+@pytest.fixture(scope="module")
+def op67(P, big):
+    s = big[0]
+    op = P.Operator(s)
+    yield op
+    op.close()
+
+
+def test_grid67_exact_order_100_iterations_bitwise(P, big):
+    s, _, ref, _ = big
+    announce("grid67 exact order, 100 iterations")
+    with P.Operator(s, P.OP_EXACT_ORDER) as op:
+        x, info, tr = op.lsqr(s.b, 100, trace=True)
+    xo, res, tro = ref.result()
+    assert np.array_equal(x, xo), f"max diff {np.abs(x - xo).max()}"
+    assert np.array_equal(tr, tro, equal_nan=True)
+    for key in ("anorm", "acond", "rnorm", "xnorm", "bnorm"):
+        assert info[key] == res[key], key
+    assert info["arnorm"] == res["arnorm"]  # test2, as lsqr.c:913 assigns it
+
+
+def test_grid67_default_100_iterations_within_sensitivity(P, big, op67):
+    s, _, ref, pert = big
+    announce("grid67 default operator, 100 iterations")
+    x, info, tr = op67.lsqr(s.b, 100, trace=True)
+    xo, res, tro = ref.result()
+    xp, _, _ = pert.result()
+    sens = rel(xp, xo)
+    d = rel(x, xo)
+    d_rn = float(np.max(np.abs(tr[:, 2] - tro[:, 2]) / tro[:, 2]))
+    announce(f"x vs oracle {d:.3e}; oracle moved by a 1e-15 change of b: {sens:.3e}; "
+             f"rnorm trace {d_rn:.3e}")
+    assert d <= max(1e-10, 100 * sens), (d, sens)
+    # the residual norm is well conditioned where x is not
+    assert d_rn < 1e-9, d_rn
+    np.testing.assert_allclose(info["anorm"], res["anorm"], rtol=1e-12)
+
+
+def test_grid67_mixed_solves_reach_tol(P, big, op67):
+    s = big[0]
+    announce("grid67 FP64 and mixed solves to test2 <= 1e-6")
+    x64, i64 = op67.lsqr(s.b, 5000, tol=TOL)
+    rn64, an64, fro = op67.residual_norms(s.b, x64)
+    # plain FP64 stops on its recurrence estimate; the true value agrees
+    t64 = an64 / (i64["anorm"] * rn64)
+    assert i64["arnorm"] <= TOL and t64 <= 1.05 * TOL, (i64["arnorm"], t64)
+    runs = {
+        "mixed": dict(precision=P.PREC_MIXED, refine_every=25),
+        "mixed_colscale": dict(precision=P.PREC_MIXED, refine_every=25, precond=P.PRECOND_COLSCALE,
+                               anorm_ref=i64["anorm"], refine_adaptive=True, precond_power=1.5),
+        # standalone: ||A||_F in the stop (no FP64 solve to borrow an anorm from)
+        "mixed_colscale_fro": dict(precision=P.PREC_MIXED, refine_every=25, precond=P.PRECOND_COLSCALE,
+                                   refine_adaptive=True, precond_power=1.5),
+    }
+    for name, kw in runs.items():
+        xm, im = op67.lsqr(s.b, 5000, tol=TOL, **kw)
+        rn, an, _ = op67.residual_norms(s.b, xm)
+        an_used = im["anorm"]
+        announce(f"{name}: {im['itn']} iterations ({im['refinements']} FP64 checks) against {i64['itn']}; "
+                 f"true test2 {an / (an_used * rn):.3e} (anorm {an_used:.4e}), with the FP64 solve's anorm "
+                 f"{an / (i64['anorm'] * rn):.3e}; x vs FP64 {rel(xm, x64):.2e}; ||r|| vs FP64 "
+                 f"{(rn - rn64) / rn64:.2e}")
+        assert im["arnorm"] <= TOL
+        assert an / (an_used * rn) <= TOL * 1.001
+        if name != "mixed_colscale_fro":
+            np.testing.assert_allclose(an_used, i64["anorm"] if "colscale" in name else an_used)
+        else:
+            np.testing.assert_allclose(an_used, fro, rtol=1e-12)
+        # both stop at test2 = 1e-6, where x is still 0.3-0.75% from the converged solution
+        assert rel(xm, x64) < 2e-2, rel(xm, x64)
+        assert abs(rn - rn64) / rn64 < 1e-4
+        if "colscale" in name:
+            assert im["itn"] < i64["itn"]
+
+
+def test_batch24_every_pair_reaches_tol(P, O, gpu_available, pool):
+    announce("generating batch24_grid20to48_rhs4")
+    specs = P.batch_specs("batch24_grid20to48_rhs4")
+    systems, B, _ = P.generate_batch(specs=specs)
+    K = 4
+    nnz = sum(s.nnz_k for s in systems)
+    announce(f"{len(systems)} problems, nnz_k {nnz}")
+    small = [int(i) for i in np.argsort([s.n for s in systems])[:2]]
+    with P.Batch(systems, K, flags=P.BATCH_TUNE_MIXED) as bt:
+        X64, I64, t64 = bt.solve(B, 5000, tol=TOL)
+        bt.set_anorm_refs([[i["anorm"] for i in row] for row in I64])
+        XM, IM, tm = bt.solve(B, 5000, tol=TOL, precision=P.PREC_MIXED, refine_every=25,
+                              precond=P.PRECOND_COLSCALE, precond_power=1.5)
+        announce(f"batch FP64 {t64:.0f} ms, mixed on A_p D_p {tm:.0f} ms")
+        worst = {}
+        for name, X, I in (("fp64", X64, I64), ("mixed_colscale", XM, IM)):
+            _, Y = bt.aprod(1, X, [np.zeros((K, m)) for m in bt.m])
+            R = [np.concatenate([B[p], np.zeros((K, s.m_d))], axis=1) - Y[p] for p, s in enumerate(systems)]
+            Z, _ = bt.aprod(2, [np.zeros((K, n)) for n in bt.n], R)
+            t2 = np.array([[np.linalg.norm(Z[p][k]) / (I64[p][k]["anorm"] * np.linalg.norm(R[p][k]))
+                            for k in range(K)] for p in range(len(systems))])
+            worst[name] = float(t2.max())
+            assert all(I[p][k]["arnorm"] <= TOL for p in range(len(systems)) for k in range(K))
+            assert t2.max() <= (1.05 if name == "fp64" else 1.001) * TOL, (name, t2.max())
+        announce(f"largest true test2: {worst}")
+    # the two smallest problems: each pair of the FP64 batch against the oracle
+    # stopped at the pair's own iteration count
+    from test_batch import oracle_sensitivity
+    jobs = []
+    for p in small:
+        orc = O.Oracle(systems[p])
+        for k in range(K):
+            itn = I64[p][k]["itn"]
+            jobs.append((p, k, pool.submit(orc.lsqr, B[p][k], itn, 0.0, 0.0, False, False),
+                         pool.submit(oracle_sensitivity, orc, B[p][k], itn)))
+    for p, k, fx, fs in jobs:
+        xo = fx.result()[0]
+        bar = max(1e-10, 100 * fs.result())
+        assert rel(X64[p][k], xo) < bar, (p, k, rel(X64[p][k], xo), bar)
+
+
+def test_learned_warm_start_grid67(P, big, op67):
+    """The committed parameters (trained on the 67^3 / 2M-ray geometry with
+    A^T b / diag(A^T A) as input) on 4 right-hand sides whose seeds lie
+    outside the training set."""
+    import importlib
+    W = importlib.import_module(P.__name__ + ".warmstart")
+    s = big[0]
+    grid = (67, 67, 67)
+    params = np.load(os.path.join(ROOT, "plsqr3.2013.09_amd", "weights", "warmstart_grid67_rays2m.npy"),
+                     allow_pickle=False)
+    announce("learned warm start: pairs for 4 held-out seeds")
+    with P.Batch([s], 4, grids=[grid], flags=P.BATCH_NO_TUNE) as bt:
+        B, ATB = W.make_pairs(bt, s, grid, [900001, 900002, 900003, 900004], P.CONFIGS["grid67_rays2m"]["sigma"])
+    with P.WarmStartNet(grid, max_batch=4) as net:
+        net.set_params(params)
+        X0 = net.predict(W.net_input(ATB, W.column_ssq(s), "jacobi"))
+    kw = dict(precision=P.PREC_MIXED, refine_every=25, precond=P.PRECOND_COLSCALE, refine_adaptive=True,
+              precond_power=1.5)
+    its_zero, its_warm = [], []
+    for j in range(4):
+        x64, i64 = op67.lsqr(B[j], 5000, tol=TOL)
+        xz, iz = op67.lsqr_x0(B[j], None, 5000, tol=TOL, anorm_ref=i64["anorm"], **kw)
+        xw, iw = op67.lsqr_x0(B[j], X0[j], 5000, tol=TOL, anorm_ref=i64["anorm"], **kw)
+        assert iw["used_x0"] and not iz["used_x0"]
+        for x, inf in ((xz, iz), (xw, iw)):
+            rn, an, _ = op67.residual_norms(B[j], x)
+            assert inf["arnorm"] <= TOL and an / (i64["anorm"] * rn) <= TOL * 1.001
+            assert rel(x, x64) < 2e-2
+        announce(f"seed {j}: x0 error {rel(X0[j], x64):.3f}; iterations FP64 {i64['itn']}, mixed on A D from "
+                 f"zero {iz['itn']}, from x0 {iw['itn']}")
+        its_zero.append(iz["itn"])
+        its_warm.append(iw["itn"])
+        assert rel(X0[j], x64) < 0.5
+    assert np.mean(its_warm) < 0.9 * np.mean(its_zero), (its_warm, its_zero)

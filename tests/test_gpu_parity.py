@@ -464,3 +464,42 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
     print(f"iterations: FP64 {i64['itn']}, mixed+colscale {is_['itn']} (||A||_F), {ir['itn']} "
           f"(FP64 anorm), adaptive checks {ia['itn']} ({ia['refinements']} checks); mixed adaptive "
           f"{im['itn']} ({im['refinements']} checks)", flush=True)
+
+
+def test_mixed_warm_start_with_fewer_rows_than_columns(P, O, gpu_available):
+    """A system without damping rows and with fewer kernel rows than columns
+    (m_k + m_d < n): the mixed solve's checked iterate is an n-vector and must
+    not be written into a buffer sized by the rows.  Warm starts from x0 = 0
+    (which always passes the fallback test) and from zero, then the operator
+    is checked again against the oracle, so a write past a buffer would show
+    in either the solution or the operator's data.  The system has full row
+    rank (120 of 300 columns), so LSQR converges to the minimum-norm solution
+    (the oracle: 2e-15 from numpy's lstsq after 100 iterations) while test2
+    stays near 0.1 (r -> 0); the solves run a fixed inner iteration count."""
+    announce("mixed warm start, m_d = 0, m_k < n")
+    g = P.generate(10, 6, 5, 120, 0, 41, 0.01, 0.1)
+    assert g.m_k < g.n
+    zi = np.zeros(g.n + 1, np.int32)
+    s = P.System.from_arrays(g.m_k, 0, g.n, g.k_col_ptr, g.k_row_idx, g.k_val, np.zeros(1, np.int32),
+                             np.zeros(0, np.int32), np.zeros(0), zi, np.zeros(0, np.int32), np.zeros(0), g.b)
+    orc = O.Oracle(s)
+    x_ref, res, _ = orc.lsqr(s.b, 150, trace=False)
+    with P.Operator(s, grid=(10, 6, 5)) as op:
+        x64, i64 = op.lsqr(s.b, 150)
+        assert rel(x64, x_ref) < 1e-10
+        for x0 in (np.zeros(s.n), None):
+            xm, im = op.lsqr_x0(s.b, x0, 150, tol=0.0, precision=P.PREC_MIXED, refine_every=25)
+            assert im["used_x0"] == (x0 is not None)
+            assert 0 < im["itn"] <= 150 and im["refinements"] >= 1
+            # FP32 Golub-Kahan vectors: the minimum-norm solution to FP32 accuracy
+            assert rel(xm, x_ref) < 1e-4, rel(xm, x_ref)
+        rng = np.random.default_rng(3)
+        x = rng.standard_normal(s.n)
+        y = rng.standard_normal(s.m_k)
+        for mode in (1, 2):
+            xg, yg = x.copy(), y.copy()
+            op.aprod(mode, xg, yg)
+            xo, yo = x.copy(), y.copy()
+            orc.aprod(mode, xo, yo)
+            got, want = (yg, yo) if mode == 1 else (xg, xo)
+            assert rel(got, want) < 1e-13
