@@ -172,6 +172,51 @@ int plsqr_ker2plsqr3(const char *dir, const char *list_file, int n_col, const ch
 int plsqr_write_x(const char *path, const double *x, int n);
 int plsqr_read_x(const char *path, double *x, int n);
 
+/* The reference's ASCII-to-binary kernel step (PLSQR3_tools/kernel_format/
+ * ker_ascii2bin.c:18-94).  list_path names the row files: first token their
+ * count, then one name per kernel row (:41-42,58).  Each in_dir/<name> holds
+ * entries "col ix iy iz value" (:76-77) and becomes out_dir/<name>.bin, one
+ * 16-byte record {int32 ix, int32 col, float64 value} per entry (:81-83),
+ * the row-file format plsqr_ker2plsqr3 reads.  out_list (may be NULL) is
+ * written into out_dir with one "<name>.bin nnz" line per row: the list
+ * plsqr_ker2plsqr3 takes (the example dataset ships it by hand,
+ * data/ker_bin.list).  An entry that does not parse as four integers and a
+ * number -> PLSQR_ERR_FORMAT (the tool itself would loop on it). */
+int plsqr_ker_ascii2bin(const char *list_path, const char *in_dir, const char *out_dir, const char *out_list);
+
+/* Damping matrix of the reference tool PLSQR3_tools/damping_format/
+ * damping_binary.py (its argv[4..14], :36-47).  n_param parameter blocks
+ * follow one another (block p owns columns p*nx*ny*nz .. (p+1)*nx*ny*nz-1,
+ * :93-95); in each block every cell, x fastest (:89-92), gets
+ *   identity                           w[0]                       (:93-113)
+ *   xx if ix < nx-2                    (1,-2,1)*w[1] at ix..ix+2  (:115-178)
+ *   yy if iy < ny-2, zz if iz < nz-2   the same along y (w[2]) and z (w[3])
+ *   with cross != 0 also
+ *   xy if ix < nx-2 and iy < ny-2      (0.5,-0.5,-0.5,0.5)*w[4] at (ix,iy),
+ *                                      (ix+2,iy), (ix,iy+2), (ix+2,iy+2)
+ *   xz, yz                             the same in those planes (w[5], w[6])
+ *                                                                 (:291-479)
+ * The reference tool always writes the cross rows; the synthetic workloads
+ * have none.  All values are FP64 in both layouts (the tool narrows its
+ * column copy to FP32, damping_binary.py:70; identical whenever FP32 holds
+ * the weights exactly, as for 1.0). */
+typedef struct plsqr_damping_spec {
+    int nx, ny, nz;   /* grid of one parameter block */
+    int n_param;      /* parameter blocks (argv[7])  */
+    double w[7];      /* identity, xx, yy, zz, xy, xz, yz weights (argv[8..14]) */
+    int cross;        /* 1: with the xy, xz, yz rows */
+} plsqr_damping_spec;
+
+/* Fills a library-owned plsqr_system with the damping part only (m_k = 0,
+ * kernel arrays and b NULL; n = nx*ny*nz*n_param). */
+int plsqr_gen_damping(const plsqr_damping_spec *spec, plsqr_system *out);
+
+/* The four damping files of the tool (row data/info, column data/info;
+ * 1-based rows and columns, damping-local rows), as plsqr_write_reference_files
+ * writes them.  Any name may be NULL to skip that file. */
+int plsqr_write_damping_files(const plsqr_damping_spec *spec, const char *dir, const char *damp_f,
+                              const char *damp_i, const char *damp_f_bycol, const char *damp_i_bycol);
+
 /* ---------------------------------------------------------------------- */
 /* Device operator                                                         */
 /* ---------------------------------------------------------------------- */

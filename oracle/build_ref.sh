@@ -41,3 +41,14 @@ for f in gptl threadutil GPTLutil; do
 done
 "$MPICC" -o "$OUT/plsqr3_ref_fp64" "$TMP"/*.o -lm
 echo "built $OUT/plsqr3_ref_fp64"
+# the kernel-format tools (PLSQR3_tools/kernel_format/makefile builds them the
+# same way: each .c with FileIO.c where it is used); they write the golden
+# digests of the example dataset's conversion chain
+TOOLS=${PLSQR_REF_TOOLS:-$(dirname "$REF")/PLSQR3_tools/kernel_format}
+if [ -f "$TOOLS/ker_ascii2bin.c" ]; then
+    "$MPICC" $FLAGS -o "$OUT/ker_ascii2bin" "$TOOLS/ker_ascii2bin.c" 2>/dev/null
+    for t in Ker2PLSQR3 Ker2PLSQR3_preprocess; do
+        "$MPICC" $FLAGS -I"$REF" -o "$OUT/$t" "$TOOLS/$t.c" "$REF/FileIO.c" -lm 2>/dev/null
+    done
+    echo "built $OUT/ker_ascii2bin $OUT/Ker2PLSQR3 $OUT/Ker2PLSQR3_preprocess"
+fi

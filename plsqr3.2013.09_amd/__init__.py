@@ -21,7 +21,8 @@ from ._lib import (  # noqa: F401
     OP_WITH_FP32, PREC_FP64, PREC_MIXED, PRECOND_COLSCALE, PRECOND_NONE, SIGNATURES, TRACE_FIELDS,
     TRACE_WIDTH, PlsqrError,
     SolveOptsC, SolveOutC, SystemC, System, batch_specs, dptr, fptr, gen_field, gen_rhs, generate,
-    kernel_rows_to_csc, ker2plsqr3,
+    kernel_rows_to_csc, ker2plsqr3, ker_ascii2bin, damping_spec, gen_damping, write_damping_files,
+    DampingSpecC,
     generate_batch, generate_config, load, read_x, write_x, _check,
 )
 

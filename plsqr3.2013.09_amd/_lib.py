@@ -64,6 +64,11 @@ class GenParamsC(C.Structure):
     ]
 
 
+class DampingSpecC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_param", C.c_int),
+                ("w", C.c_double * 7), ("cross", C.c_int)]
+
+
 class GridC(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
 
@@ -141,6 +146,9 @@ SIGNATURES = {
     "plsqr_kernel_rows_to_csc": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p,
                                            c_double_p]),
     "plsqr_ker2plsqr3": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_char_p]),
+    "plsqr_ker_ascii2bin": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "plsqr_gen_damping": (C.c_int, [C.POINTER(DampingSpecC), C.POINTER(SystemC)]),
+    "plsqr_write_damping_files": (C.c_int, [C.POINTER(DampingSpecC), C.c_char_p] + [C.c_char_p] * 4),
     "plsqr_lsqr_solve_se": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(SolveOptsC), c_double_p, c_double_p,
                                       C.POINTER(SolveOutC)]),
     "plsqr_aprod_device_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -454,6 +462,41 @@ def ker2plsqr3(directory, list_file, n_col, out_bin, out_info):
     to the column-sorted kernel files directory/out_bin and out_info."""
     _check(load().plsqr_ker2plsqr3(directory.encode(), list_file.encode(), int(n_col), out_bin.encode(),
                                    out_info.encode()))
+
+
+def ker_ascii2bin(list_path, in_dir, out_dir, out_list=None):
+    """ASCII kernel rows ("col ix iy iz value") to the 16-byte row files."""
+    _check(load().plsqr_ker_ascii2bin(list_path.encode(), in_dir.encode(), out_dir.encode(),
+                                      out_list.encode() if out_list else None))
+
+
+def damping_spec(nx, ny, nz, n_param=1, weights=(1.0,) * 7, cross=True):
+    """Damping of damping_binary.py: weights = identity, xx, yy, zz, xy, xz, yz."""
+    d = DampingSpecC()
+    d.nx, d.ny, d.nz, d.n_param, d.cross = int(nx), int(ny), int(nz), int(n_param), int(bool(cross))
+    w = [float(v) for v in weights]
+    assert len(w) == 7
+    for k in range(7):
+        d.w[k] = w[k]
+    return d
+
+
+def gen_damping(nx, ny, nz, n_param=1, weights=(1.0,) * 7, cross=True):
+    """A System holding only the damping part (m_k = 0)."""
+    s = SystemC()
+    _check(load().plsqr_gen_damping(C.byref(damping_spec(nx, ny, nz, n_param, weights, cross)), C.byref(s)))
+    return System(s)
+
+
+def write_damping_files(directory, nx, ny, nz, n_param=1, weights=(1.0,) * 7, cross=True, prefix="damp"):
+    """The four damping files (names as the reference tool writes them for
+    the prefix); returns the names."""
+    names = dict(damp_f=prefix + "_row_data.bin", damp_i=prefix + "_row_info.bin",
+                 damp_f_bycol=prefix + "_col_data.bin", damp_i_bycol=prefix + "_col_info.bin")
+    _check(load().plsqr_write_damping_files(C.byref(damping_spec(nx, ny, nz, n_param, weights, cross)),
+                                            directory.encode(), *[names[k].encode() for k in (
+                                                "damp_f", "damp_i", "damp_f_bycol", "damp_i_bycol")]))
+    return names
 
 
 def read_x(path, n):

@@ -220,66 +220,106 @@ void kernel_rhs(const int m_k, const long long *row_ptr, const int *col, const d
     });
 }
 
-// Damping matrix in the reference tool's row order (damping_binary.py:84-289,
-// np = 1, no cross rows).  Columns inside a row ascend.
-void build_damping(const Grid &g, double lambda, double wxx, double wyy, double wzz,
-                   plsqr_system *s) {
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+// Damping matrix in the reference tool's row order
+// (damping_binary.py:84-479): parameter blocks one after the other (block p
+// holds columns p*ncell.., :93-95), inside a block the cells x fastest
+// (:89-92), and per cell the identity row, then the second differences
+// along x, y and z (1,-2,1) and, with cross rows, the mixed differences in
+// the xy, xz and yz planes (0.5,-0.5,-0.5,0.5 at the corners of a 3 x 3
+// square).  w = {identity, xx, yy, zz, xy, xz, yz}.  Columns inside a row
+// ascend.  The column layout holds the same FP64 values (the tool narrows
+// its column copy to FP32, :70, which only differs for weights FP32 cannot
+// represent).
+int build_damping(int nx, int ny, int nz, int n_param, const double w[7], bool cross, plsqr_system *s) {
     const long long ncell = (long long)nx * ny * nz;
-    const long long rxx = (long long)std::max(nx - 2, 0) * ny * nz;
-    const long long ryy = (long long)nx * std::max(ny - 2, 0) * nz;
-    const long long rzz = (long long)nx * ny * std::max(nz - 2, 0);
-    const long long m_d = ncell + rxx + ryy + rzz;
-    const long long nnz_d = ncell + 3 * (rxx + ryy + rzz);
+    const long long ex = std::max(nx - 2, 0), ey = std::max(ny - 2, 0), ez = std::max(nz - 2, 0);
+    const long long rxx = ex * ny * nz, ryy = nx * ey * nz, rzz = (long long)nx * ny * ez;
+    const long long rxy = cross ? ex * ey * nz : 0, rxz = cross ? ex * ny * ez : 0, ryz = cross ? nx * ey * ez : 0;
+    const long long m_d = (ncell + rxx + ryy + rzz + rxy + rxz + ryz) * n_param;  // :50
+    const long long nnz_d = (ncell + 3 * (rxx + ryy + rzz) + 4 * (rxy + rxz + ryz)) * n_param;  // :61
+    const long long n = ncell * n_param;
+    if (m_d >= (1LL << 31) || nnz_d >= (1LL << 31) || n >= (1LL << 31))
+        return fail(PLSQR_ERR_ARG, "damping matrix too large for int32 indices");
     s->m_d = (int)m_d;
     s->nnz_d = nnz_d;
     s->d_row_ptr = (int *)malloc(sizeof(int) * (m_d + 1));
-    s->d_col_idx = (int *)malloc(sizeof(int) * nnz_d);
-    s->d_val = (double *)malloc(sizeof(double) * nnz_d);
+    s->d_col_idx = (int *)malloc(sizeof(int) * std::max(nnz_d, 1LL));
+    s->d_val = (double *)malloc(sizeof(double) * std::max(nnz_d, 1LL));
+    s->dc_col_ptr = (int *)calloc((size_t)n + 1, sizeof(int));
+    s->dc_row_idx = (int *)malloc(sizeof(int) * std::max(nnz_d, 1LL));
+    s->dc_val = (double *)malloc(sizeof(double) * std::max(nnz_d, 1LL));
+    if (!s->d_row_ptr || !s->d_col_idx || !s->d_val || !s->dc_col_ptr || !s->dc_row_idx || !s->dc_val)
+        return fail(PLSQR_ERR_NOMEM, "host allocation of the damping matrix failed");
     long long row = 0, e = 0;
     s->d_row_ptr[0] = 0;
-    auto idmap = [&](int ix, int iy, int iz) { return iz * ny * nx + iy * nx + ix; };
-    for (int iz = 0; iz < nz; ++iz)
-        for (int iy = 0; iy < ny; ++iy)
-            for (int ix = 0; ix < nx; ++ix) {
-                s->d_col_idx[e] = idmap(ix, iy, iz);
-                s->d_val[e++] = 1.0 * lambda;
-                s->d_row_ptr[++row] = (int)e;
-                if (ix < nx - 2) {
-                    s->d_col_idx[e] = idmap(ix, iy, iz);     s->d_val[e++] = 1.0 * wxx;
-                    s->d_col_idx[e] = idmap(ix + 1, iy, iz); s->d_val[e++] = -2.0 * wxx;
-                    s->d_col_idx[e] = idmap(ix + 2, iy, iz); s->d_val[e++] = 1.0 * wxx;
-                    s->d_row_ptr[++row] = (int)e;
+    for (int p = 0; p < n_param; ++p) {
+        const long long off = (long long)p * ncell;
+        auto col = [&](int ix, int iy, int iz) { return (int)(off + (long long)iz * ny * nx + (long long)iy * nx + ix); };
+        auto put = [&](int c, double v) {
+            s->d_col_idx[e] = c;
+            s->d_val[e++] = v;
+        };
+        auto end_row = [&] { s->d_row_ptr[++row] = (int)e; };
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) {
+                    put(col(ix, iy, iz), 1.0 * w[0]);
+                    end_row();
+                    if (ix < nx - 2) {
+                        put(col(ix, iy, iz), 1.0 * w[1]);
+                        put(col(ix + 1, iy, iz), -2.0 * w[1]);
+                        put(col(ix + 2, iy, iz), 1.0 * w[1]);
+                        end_row();
+                    }
+                    if (iy < ny - 2) {
+                        put(col(ix, iy, iz), 1.0 * w[2]);
+                        put(col(ix, iy + 1, iz), -2.0 * w[2]);
+                        put(col(ix, iy + 2, iz), 1.0 * w[2]);
+                        end_row();
+                    }
+                    if (iz < nz - 2) {
+                        put(col(ix, iy, iz), 1.0 * w[3]);
+                        put(col(ix, iy, iz + 1), -2.0 * w[3]);
+                        put(col(ix, iy, iz + 2), 1.0 * w[3]);
+                        end_row();
+                    }
+                    if (!cross) continue;
+                    if (ix < nx - 2 && iy < ny - 2) {
+                        put(col(ix, iy, iz), 0.5 * w[4]);
+                        put(col(ix + 2, iy, iz), -0.5 * w[4]);
+                        put(col(ix, iy + 2, iz), -0.5 * w[4]);
+                        put(col(ix + 2, iy + 2, iz), 0.5 * w[4]);
+                        end_row();
+                    }
+                    if (ix < nx - 2 && iz < nz - 2) {
+                        put(col(ix, iy, iz), 0.5 * w[5]);
+                        put(col(ix + 2, iy, iz), -0.5 * w[5]);
+                        put(col(ix, iy, iz + 2), -0.5 * w[5]);
+                        put(col(ix + 2, iy, iz + 2), 0.5 * w[5]);
+                        end_row();
+                    }
+                    if (iy < ny - 2 && iz < nz - 2) {
+                        put(col(ix, iy, iz), 0.5 * w[6]);
+                        put(col(ix, iy + 2, iz), -0.5 * w[6]);
+                        put(col(ix, iy, iz + 2), -0.5 * w[6]);
+                        put(col(ix, iy + 2, iz + 2), 0.5 * w[6]);
+                        end_row();
+                    }
                 }
-                if (iy < ny - 2) {
-                    s->d_col_idx[e] = idmap(ix, iy, iz);     s->d_val[e++] = 1.0 * wyy;
-                    s->d_col_idx[e] = idmap(ix, iy + 1, iz); s->d_val[e++] = -2.0 * wyy;
-                    s->d_col_idx[e] = idmap(ix, iy + 2, iz); s->d_val[e++] = 1.0 * wyy;
-                    s->d_row_ptr[++row] = (int)e;
-                }
-                if (iz < nz - 2) {
-                    s->d_col_idx[e] = idmap(ix, iy, iz);     s->d_val[e++] = 1.0 * wzz;
-                    s->d_col_idx[e] = idmap(ix, iy, iz + 1); s->d_val[e++] = -2.0 * wzz;
-                    s->d_col_idx[e] = idmap(ix, iy, iz + 2); s->d_val[e++] = 1.0 * wzz;
-                    s->d_row_ptr[++row] = (int)e;
-                }
-            }
+    }
     // column layout: stable by column so rows ascend inside a column
-    // (damping_binary.py:523 sorts the row-ordered triplets by column).
-    const int n = (int)ncell;
-    s->dc_col_ptr = (int *)calloc((size_t)n + 1, sizeof(int));
-    s->dc_row_idx = (int *)malloc(sizeof(int) * nnz_d);
-    s->dc_val = (double *)malloc(sizeof(double) * nnz_d);
+    // (damping_binary.py:523 sorts the row-ordered triplets by column)
     for (long long k = 0; k < nnz_d; ++k) s->dc_col_ptr[s->d_col_idx[k] + 1]++;
-    for (int c = 0; c < n; ++c) s->dc_col_ptr[c + 1] += s->dc_col_ptr[c];
+    for (long long c = 0; c < n; ++c) s->dc_col_ptr[c + 1] += s->dc_col_ptr[c];
     std::vector<int> fill(s->dc_col_ptr, s->dc_col_ptr + n);
     for (long long r = 0; r < m_d; ++r)
         for (int k = s->d_row_ptr[r]; k < s->d_row_ptr[r + 1]; ++k) {
             const int c = s->d_col_idx[k];
-            const int pos = fill[c]++;
+            const int pos = fill[(size_t)c]++;
             s->dc_row_idx[pos] = (int)r;
             s->dc_val[pos] = s->d_val[k];
         }
+    return PLSQR_OK;
 }
 
 // CSR (row_ptr as long long) -> CSC with rows ascending inside each column.
@@ -488,8 +528,10 @@ int plsqr_gen_system(const plsqr_gen_params *p, plsqr_system *out) {
     csr_to_csc(m_k, n, row_ptr.data(), col.data(), val.data(), out->k_col_ptr, out->k_row_idx, out->k_val);
     make_blobs(g, p->seed, p->n_blobs, p->blob_width, out->x_true);
     kernel_rhs(m_k, row_ptr.data(), col.data(), val.data(), out->x_true, p->seed, p->sigma, out->b);
-    build_damping(g, p->lambda, p->w_xx, p->w_yy, p->w_zz, out);
-    return PLSQR_OK;
+    const double w[7] = {p->lambda, p->w_xx, p->w_yy, p->w_zz, 0.0, 0.0, 0.0};
+    const int rc = build_damping(p->nx, p->ny, p->nz, 1, w, false, out);
+    if (rc) plsqr_system_free(out);
+    return rc;
 }
 
 int plsqr_gen_field(const plsqr_gen_params *p, double *x_true) {
@@ -753,6 +795,88 @@ int plsqr_read_reference_files(const char *dir, const char *ker_f, const char *k
         if (!out->b) return bail(fail(PLSQR_ERR_NOMEM, "host allocation failed"));
         for (int i = 0; i < row_k; ++i)
             if (fscanf(f, "%le", &out->b[i]) != 1) return bail(fail(PLSQR_ERR_FORMAT, "b file holds fewer than row_k values"));
+    }
+    return PLSQR_OK;
+}
+
+int plsqr_gen_damping(const plsqr_damping_spec *spec, plsqr_system *out) {
+    if (!spec || !out) return fail(PLSQR_ERR_ARG, "null argument");
+    memset(out, 0, sizeof(*out));
+    if (spec->nx < 1 || spec->ny < 1 || spec->nz < 1 || spec->n_param < 1)
+        return fail(PLSQR_ERR_ARG, "grid edges and n_param must be >= 1");
+    for (int k = 0; k < 7; ++k)
+        if (!std::isfinite(spec->w[k])) return fail(PLSQR_ERR_NONFINITE, "damping weight is NaN or Inf");
+    const long long n = (long long)spec->nx * spec->ny * spec->nz * spec->n_param;
+    if (n >= (1LL << 31)) return fail(PLSQR_ERR_ARG, "too many columns for int32 indices");
+    out->owned = 1;
+    out->n = (int)n;
+    const int rc = build_damping(spec->nx, spec->ny, spec->nz, spec->n_param, spec->w, spec->cross != 0, out);
+    if (rc) plsqr_system_free(out);
+    return rc;
+}
+
+int plsqr_write_damping_files(const plsqr_damping_spec *spec, const char *dir, const char *damp_f,
+                              const char *damp_i, const char *damp_f_bycol, const char *damp_i_bycol) {
+    plsqr_system d;
+    int rc = plsqr_gen_damping(spec, &d);
+    if (rc) return rc;
+    rc = plsqr_write_reference_files(&d, dir, nullptr, nullptr, damp_f, damp_i, damp_f_bycol, damp_i_bycol, nullptr);
+    plsqr_system_free(&d);
+    return rc;
+}
+
+int plsqr_ker_ascii2bin(const char *list_path, const char *in_dir, const char *out_dir, const char *out_list) {
+    if (!list_path || !in_dir || !out_dir) return fail(PLSQR_ERR_ARG, "list_path, in_dir and out_dir are required");
+    FILE *lf = fopen(list_path, "r");
+    if (!lf) return fail(PLSQR_ERR_IO, std::string("cannot open ") + list_path + ": " + strerror(errno));
+    std::vector<std::string> names;
+    {
+        FileCloser fc{lf};
+        int count = 0;  // the list: the number of rows, then one file name per row (ker_ascii2bin.c:41,58)
+        if (fscanf(lf, "%d", &count) != 1 || count < 0)
+            return fail(PLSQR_ERR_FORMAT, std::string(list_path) + ": the first token must be the row count");
+        char name[4096];
+        for (int i = 0; i < count; ++i) {
+            if (fscanf(lf, "%4095s", name) != 1)
+                return fail(PLSQR_ERR_FORMAT, std::string(list_path) + ": fewer file names than the row count");
+            names.emplace_back(name);
+        }
+    }
+    std::vector<long long> nnz(names.size(), 0);
+    for (size_t i = 0; i < names.size(); ++i) {
+        const std::string in = join(in_dir, names[i].c_str()), out = join(out_dir, (names[i] + ".bin").c_str());
+        FILE *fi = fopen(in.c_str(), "r");
+        if (!fi) return fail(PLSQR_ERR_IO, "cannot open " + in + ": " + strerror(errno));
+        FileCloser ci{fi};
+        FILE *fo = fopen(out.c_str(), "wb");
+        if (!fo) return fail(PLSQR_ERR_IO, "cannot open " + out + ": " + strerror(errno));
+        FileCloser co{fo};
+        // "col ix iy iz value" per entry (ker_ascii2bin.c:76-77) -> {ix, col, value} (:81-83)
+        int col, ix, iy, iz, k;
+        double v;
+        std::vector<Record> buf;
+        buf.reserve(1 << 14);
+        auto flush = [&]() -> bool {
+            const bool ok = buf.empty() || fwrite(buf.data(), sizeof(Record), buf.size(), fo) == buf.size();
+            buf.clear();
+            return ok;
+        };
+        while ((k = fscanf(fi, "%d %d %d %d %lf", &col, &ix, &iy, &iz, &v)) == 5) {
+            buf.push_back(Record{ix, col, v});
+            ++nnz[i];
+            if (buf.size() == (1 << 14) && !flush()) return fail(PLSQR_ERR_IO, "short write on " + out);
+        }
+        if (k != EOF)
+            return fail(PLSQR_ERR_FORMAT, in + ": entry " + std::to_string(nnz[i] + 1) +
+                                              " is not \"col ix iy iz value\"");
+        if (!flush()) return fail(PLSQR_ERR_IO, "short write on " + out);
+    }
+    if (out_list) {
+        const std::string path = join(out_dir, out_list);
+        FILE *f = fopen(path.c_str(), "w");
+        if (!f) return fail(PLSQR_ERR_IO, "cannot open " + path + ": " + strerror(errno));
+        FileCloser fc{f};
+        for (size_t i = 0; i < names.size(); ++i) fprintf(f, "%s.bin %lld\n", names[i].c_str(), nnz[i]);
     }
     return PLSQR_OK;
 }

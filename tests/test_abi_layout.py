@@ -18,6 +18,7 @@ STRUCTS = {
     "plsqr_solve_opts": "SolveOptsC",
     "plsqr_solve_out": "SolveOutC",
     "plsqr_batch_tuning": "BatchTuningC",
+    "plsqr_damping_spec": "DampingSpecC",
 }
 # ctypes field names that differ from the C member names
 RENAME = {("plsqr_gen_params", "lambda_"): "lambda"}
