@@ -147,46 +147,96 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "plsqr3_ref_fp64")
 MPIEXEC = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
 
 
-def cpu_baseline_reference(P, s, iters_to_tol, sample_iters):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """Cores this process may run on (the box's share), capped at 16 ranks."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def run_reference(s, names, work, sample_iters, nproc):
+    """PLSQR3 itself on nproc MPI ranks (its even column partition,
+    plsqr3.c:227-250); returns its "Parallel LSQR time" (the loop alone,
+    lsqr.c:476-842) and the wall time of the run."""
+    import subprocess
+    cmd = [MPIEXEC, "-np", str(nproc), REF_BIN, "-dir", work]
+    for key, flag in (("ker_f", "-ker_f"), ("ker_i", "-ker_i"), ("damp_f", "-damp_f"), ("damp_i", "-damp_i"),
+                      ("damp_f_bycol", "-damp_f_bycol"), ("damp_i_bycol", "-damp_i_bycol"), ("b_k", "-b_k")):
+        cmd += [flag, names[key]]
+    cmd += ["-row_k", str(s.m_k), "-row_d", str(s.m_d), "-col", str(s.n), "-itn", str(sample_iters)]
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600, env=env)
+    wall = time.perf_counter() - t0
+    t = re.findall(r"Parallel LSQR time=(\S+) s", r.stdout)
+    if r.returncode != 0 or not t:
+        raise RuntimeError(f"reference run on {nproc} ranks failed ({r.returncode}): {r.stderr[-300:]}")
+    return float(t[-1]), wall
+
+
+def cpu_baseline_reference(P, s, iters_to_tol, sample_iters, sample_iters_all=None):
     """The reference itself (oracle/_ref: the PLSQR3 sources compiled here at
-    -O3, FP64 configuration) on one MPI rank: the system written in its file
-    formats, a run of sample_iters iterations, and its own "Parallel LSQR
-    time" (which excludes loading), scaled to the GPU solve's iteration
-    count.  None when the binary or mpiexec is missing or the run fails."""
+    -O3, FP64 configuration): the system written in its file formats, a run
+    of sample_iters iterations on one MPI rank and one on every host core
+    (up to 16 ranks), each scaled from its own "Parallel LSQR time" (which
+    excludes loading) to the GPU solve's iteration count.  The all-core run
+    is the baseline; the one-rank run is reported inside it.  None when the
+    binary or mpiexec is missing or the one-rank run fails."""
     import shutil
     import subprocess
     import tempfile
     if not (os.path.exists(REF_BIN) and os.path.exists(MPIEXEC)):
         return None
     work = tempfile.mkdtemp(prefix="plsqr_ref_", dir=os.environ.get("TMPDIR", "/tmp"))
+    ncore = host_cores()
+    sample_all = sample_iters_all or sample_iters
     try:
         t0 = time.perf_counter()
         names = s.write_reference_files(work)
         write_s = time.perf_counter() - t0
-        cmd = [MPIEXEC, "-np", "1", REF_BIN, "-dir", work]
-        for key, flag in (("ker_f", "-ker_f"), ("ker_i", "-ker_i"), ("damp_f", "-damp_f"), ("damp_i", "-damp_i"),
-                          ("damp_f_bycol", "-damp_f_bycol"), ("damp_i_bycol", "-damp_i_bycol"), ("b_k", "-b_k")):
-            cmd += [flag, names[key]]
-        cmd += ["-row_k", str(s.m_k), "-row_d", str(s.m_d), "-col", str(s.n), "-itn", str(sample_iters)]
-        t0 = time.perf_counter()
-        r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=600)
-        wall = time.perf_counter() - t0
-        t = re.findall(r"Parallel LSQR time=(\S+) s", r.stdout)
-        if r.returncode != 0 or not t:
-            print(f"[bench] reference run failed ({r.returncode}); using the port", file=sys.stderr, flush=True)
-            return None
-        lsqr_s = float(t[-1])
-    except (OSError, subprocess.SubprocessError) as e:
+        lsqr_1, wall_1 = run_reference(s, names, work, sample_iters, 1)
+        all_line = None
+        if ncore > 1:
+            try:
+                lsqr_p, wall_p = run_reference(s, names, work, sample_all, ncore)
+                per_p = lsqr_p / (sample_all + 0.5)
+                all_line = (lsqr_p, wall_p, per_p)
+            except (OSError, RuntimeError, subprocess.SubprocessError) as e:
+                print(f"[bench] reference run on {ncore} ranks failed ({e})", file=sys.stderr, flush=True)
+    except (OSError, RuntimeError, subprocess.SubprocessError) as e:
         print(f"[bench] reference run failed ({e}); using the port", file=sys.stderr, flush=True)
         return None
     finally:
         shutil.rmtree(work, ignore_errors=True)
-    per_it = lsqr_s / (sample_iters + 0.5)
-    return {"value": per_it * (iters_to_tol + 0.5), "unit": "s", "cores": 1, "kind": "reference",
-            "sample": f"PLSQR3 itself (compiled from its sources at -O3, FP64, mpiexec -np 1) on the same system "
-                      f"from its files: {sample_iters} iterations, Parallel LSQR time {lsqr_s:.1f} s "
-                      f"({per_it:.3f} s/iteration; files written in {write_s:.0f} s, run {wall:.0f} s incl. "
-                      f"loading), scaled to {iters_to_tol} iterations"}
+    per_1 = lsqr_1 / (sample_iters + 0.5)
+    single = {"value": per_1 * (iters_to_tol + 0.5), "unit": "s", "cores": 1,
+              "sample": f"mpiexec -np 1, {sample_iters} iterations, Parallel LSQR time {lsqr_1:.1f} s "
+                        f"({per_1:.4f} s/iteration; run {wall_1:.0f} s incl. loading)"}
+    common = (f"PLSQR3 itself (compiled from its sources at -O3, FP64) on the same system from its files "
+              f"(written in {write_s:.0f} s), its own Parallel LSQR time scaled from the sample to "
+              f"{iters_to_tol} iterations (the plain FP64 solve's count); host CPU: {cpu_model()}")
+    if all_line is None:
+        return dict(single, kind="reference", sample=common + "; " + single["sample"], cpu_model=cpu_model(),
+                    single_core=single)
+    lsqr_p, wall_p, per_p = all_line
+    return {"value": per_p * (iters_to_tol + 0.5), "unit": "s", "cores": ncore, "kind": "reference",
+            "sample": common + f"; mpiexec -np {ncore} (its even column partition), {sample_all} iterations, "
+                               f"Parallel LSQR time {lsqr_p:.1f} s ({per_p:.4f} s/iteration; run {wall_p:.0f} s "
+                               f"incl. loading)",
+            "cpu_model": cpu_model(), "single_core": single}
 
 
 def cpu_baseline(P, s, iters_to_tol, sample_iters):
@@ -343,6 +393,240 @@ def bench_learned(P, hip, a):
     print(json.dumps(out), flush=True)
 
 
+def converged_solution(op, s, tol=1e-12, max_iters=20000):
+    """Plain FP64 LSQR run far past the benchmark's tolerance (untimed): the
+    reference solution the per-solve errors are measured against.  Returns
+    x*, ||r*|| (true FP64 residual) and the solve's info."""
+    xs, info = op.lsqr(s.b, max_iters, tol=tol)
+    rn, an, fro = op.residual_norms(s.b, xs)
+    return xs, rn, info, an / (info["anorm"] * rn) if rn > 0 else 0.0
+
+
+def solution_errors(op, b, x, xstar, rstar, anorm):
+    """x and residual error of a solve against the converged FP64 solution,
+    and its true FP64 test2 = ||A^T r|| / (anorm ||r||) (lsqr.c:742-744)."""
+    rn, an, _ = op.residual_norms(b, x)
+    return {"x_rel_err_vs_converged": float(np.linalg.norm(x - xstar) / np.linalg.norm(xstar)),
+            "r_rel_excess_vs_converged": float((rn - rstar) / rstar) if rstar > 0 else None,
+            "true_test2": float(an / (anorm * rn)) if rn > 0 and anorm > 0 else 0.0}
+
+
+def bench_learned_line(P, hip, op, s, a, log):
+    """The learned warm start on held-out right-hand sides of the 67^3 system
+    with the committed parameters: per right-hand side, plain FP64 LSQR from
+    zero (the reference's method; its anorm makes the other stops as strict),
+    mixed precision on A D from zero, and the learned path timed end to end
+    from b resident in HBM: A^T b on the device, the Jacobi-scaled network
+    input, the network's x0, r0 = [b;0] - A x0 with the fallback test, and
+    mixed precision on A D from x0."""
+    import importlib
+    W = importlib.import_module(P.__name__ + ".warmstart")
+    cfg = P.CONFIGS[a.workload]
+    grid = (cfg["nx"], cfg["ny"], cfg["nz"])
+    wpath = os.path.join(ROOT, "plsqr3.2013.09_amd", "weights", "warmstart_grid67_rays2m.npy")
+    if not os.path.exists(wpath) or a.workload != "grid67_rays2m":
+        return None
+    t0 = time.perf_counter()
+    seeds = [a.learned_seed0 + i for i in range(a.learned_pairs)]
+    with P.Batch([s], 4, grids=[grid], flags=P.BATCH_NO_TUNE) as bt:
+        B, _ = W.make_pairs(bt, s, grid, seeds, cfg["sigma"])
+    colss = W.column_ssq(s)
+    inv = np.where(colss > 0, 1.0 / np.where(colss > 0, colss, 1.0), 1.0)
+    net = P.WarmStartNet(grid, max_batch=1)
+    net.set_params(np.load(wpath, allow_pickle=False))
+    m, n = s.m_k + s.m_d, s.n
+    b_dev, y_dev, atb_dev = hip.malloc(8 * s.m_k), hip.malloc(8 * m), hip.malloc(8 * n)
+    x0_dev, x_dev = hip.malloc(8 * n), hip.malloc(8 * n)
+    setup_s = time.perf_counter() - t0
+    log(f"[bench] learned: {len(seeds)} held-out right-hand sides ready ({setup_s:.1f} s)")
+    kw = dict(precision=P.PREC_MIXED, refine_every=a.refine_every, precond=P.PRECOND_COLSCALE,
+              refine_adaptive=True, precond_power=a.precond_power)
+    zeros_n = np.zeros(n)
+    rows = []
+    try:
+        for j, b in enumerate(B):
+            y = np.zeros(m)
+            y[:s.m_k] = b
+            hip.h2d(b_dev, np.ascontiguousarray(b))
+            hip.h2d(y_dev, y)
+            hip.sync()
+            # plain FP64 from zero
+            t1 = time.perf_counter()
+            i64 = op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol)
+            hip.sync()
+            t64 = time.perf_counter() - t1
+            x64 = np.empty(n)
+            hip.d2h(x64, x_dev)
+            # mixed precision on A D from zero
+            t1 = time.perf_counter()
+            iz = op.lsqr_x0_device(b_dev, 0, x_dev, a.max_iters, tol=a.tol, anorm_ref=i64["anorm"], **kw)
+            hip.sync()
+            tz = time.perf_counter() - t1
+            xz = np.empty(n)
+            hip.d2h(xz, x_dev)
+            # learned: everything from b on the device
+            t1 = time.perf_counter()
+            hip.h2d(atb_dev, zeros_n)
+            op.aprod_device(2, atb_dev, y_dev)
+            atb = np.empty(n)
+            hip.d2h(atb, atb_dev)
+            x0 = net.predict((atb * inv)[None, :])[0]
+            hip.h2d(x0_dev, x0)
+            iw = op.lsqr_x0_device(b_dev, x0_dev, x_dev, a.max_iters, tol=a.tol, anorm_ref=i64["anorm"], **kw)
+            hip.sync()
+            tw = time.perf_counter() - t1
+            xw = np.empty(n)
+            hip.d2h(xw, x_dev)
+            r = {"fp64_zero": {"s": t64, "itn": i64["itn"], "test2": i64["arnorm"]},
+                 "mixed_colscale_zero": {"s": tz, "itn": iz["itn"], "test2": iz["arnorm"],
+                                         "x_rel_diff_vs_fp64": float(np.linalg.norm(xz - x64) / np.linalg.norm(x64))},
+                 "learned": {"s": tw, "itn": iw["itn"], "test2": iw["arnorm"], "used_x0": iw["used_x0"],
+                             "x0_rel_diff_vs_fp64": float(np.linalg.norm(x0 - x64) / np.linalg.norm(x64)),
+                             "x_rel_diff_vs_fp64": float(np.linalg.norm(xw - x64) / np.linalg.norm(x64))}}
+            if j < a.learned_converged:
+                # the converged FP64 solution of this b (untimed)
+                xs, inf_s = op.lsqr(b, 20000, tol=1e-12)
+                rs, _, _ = op.residual_norms(b, xs)
+                for key, xx in (("fp64_zero", x64), ("mixed_colscale_zero", xz), ("learned", xw)):
+                    r[key].update(solution_errors(op, b, xx, xs, rs, i64["anorm"]))
+            rows.append(r)
+            if (j + 1) % 8 == 0 or j + 1 == len(B):
+                log(f"[bench] learned: {j + 1} of {len(B)} right-hand sides")
+    finally:
+        net.close()
+        for p in (b_dev, y_dev, atb_dev, x0_dev, x_dev):
+            hip.free(p)
+
+    def stat(key, field):
+        v = np.array([r[key][field] for r in rows], float)
+        return {"mean": float(v.mean()), "median": float(np.median(v)), "max": float(v.max())}
+
+    out = {"method": "A^T b / diag(A^T A) -> 3-D CNN (committed parameters) -> x0, fallback to x0 = 0 if "
+                     "||[b;0] - A x0|| > ||[b;0]||, then mixed precision on A D (column scaling) with adaptive FP64 "
+                     "checks; timed from b resident in HBM, including A^T b, the network and r0",
+           "right_hand_sides": len(rows), "seeds": [seeds[0], seeds[-1]], "tol": a.tol}
+    for key in ("fp64_zero", "mixed_colscale_zero", "learned"):
+        out[key] = {"s": stat(key, "s"), "iterations": stat(key, "itn"), "test2_max": stat(key, "test2")["max"]}
+        if key != "fp64_zero":
+            out[key]["x_rel_diff_vs_fp64"] = stat(key, "x_rel_diff_vs_fp64")
+            out[key]["speedup_vs_fp64"] = round(out["fp64_zero"]["s"]["mean"] / out[key]["s"]["mean"], 3)
+        conv = [r[key] for r in rows if "x_rel_err_vs_converged" in r[key]]
+        if conv:
+            out[key]["vs_converged"] = {f: [c[f] for c in conv] for f in ("x_rel_err_vs_converged",
+                                                                           "r_rel_excess_vs_converged",
+                                                                           "true_test2")}
+    out["learned"]["used_x0"] = int(sum(r["learned"]["used_x0"] for r in rows))
+    out["learned"]["x0_rel_diff_vs_fp64"] = stat("learned", "x0_rel_diff_vs_fp64")
+    out["learned"]["tol_reached"] = bool(all(r["learned"]["test2"] <= a.tol for r in rows))
+    out["setup_s"] = round(setup_s, 2)
+    return out
+
+
+def bench_batch_line(P, hip, a, log):
+    """The batched workload (24 problems of 20^3..48^3 cells, 4 right-hand
+    sides each) beside the single-system line: one FP64 batch solve and one
+    mixed-precision batch solve on A_p D_p, every pair to test2 <= tol."""
+    name = "batch24_grid20to48_rhs4"
+    specs = P.batch_specs(name)
+    t0 = time.perf_counter()
+    systems, B, _ = P.generate_batch(specs=specs)
+    gen_s = time.perf_counter() - t0
+    K = len(specs[0]["rhs"])
+    t0 = time.perf_counter()
+    bt = P.Batch(systems, K, flags=P.BATCH_TUNE_MIXED)
+    setup_s = time.perf_counter() - t0
+    log(f"[bench] batch: generated ({gen_s:.1f} s), on device with autotuning ({setup_s:.1f} s)")
+    try:
+        b = bt.pack_b(B)
+        b_dev = hip.malloc(8 * b.size)
+        x_dev = hip.malloc(8 * sum(x.n for x in systems) * K)
+        hip.h2d(b_dev, b)
+        res = {}
+        try:
+            for label, kw in (("fp64", dict()),
+                              ("mixed_colscale", dict(precision=P.PREC_MIXED, refine_every=a.refine_every,
+                                                      precond=P.PRECOND_COLSCALE, precond_power=a.precond_power))):
+                if label == "mixed_colscale":
+                    bt.set_anorm_refs([[i["anorm"] for i in row] for row in res["fp64"]["infos"]])
+                bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol, **kw)  # warm-up
+                hip.sync()
+                t1 = time.perf_counter()
+                infos, tot = bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol, **kw)
+                hip.sync()
+                wall = time.perf_counter() - t1
+                per = np.array([max(i["elapsed_ms"] for i in row) for row in infos])
+                its = np.array([[i["itn"] for i in row] for row in infos])
+                res[label] = {"infos": infos, "s_per_batch": wall, "device_ms": tot,
+                              "solves_per_s": len(systems) * K / wall,
+                              "iterations": {"min": int(its.min()), "max": int(its.max())},
+                              "per_problem_ms": {"p50": float(np.percentile(per, 50)),
+                                                 "p99": float(np.percentile(per, 99))},
+                              "tol_reached": bool(all(i["arnorm"] <= a.tol for row in infos for i in row))}
+        finally:
+            hip.free(b_dev)
+            hip.free(x_dev)
+    finally:
+        bt.close()
+    out = {"workload": name, "problems": len(systems), "rhs_per_problem": K,
+           "nnz_k": int(sum(x.nnz_k for x in systems)), "generate_s": round(gen_s, 1),
+           "create_and_autotune_s": round(setup_s, 1)}
+    for label, r in res.items():
+        r.pop("infos")
+        out[label] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    if "fp64" in out and "mixed_colscale" in out:
+        out["mixed_colscale"]["speedup_vs_fp64"] = round(res["fp64"]["s_per_batch"] /
+                                                         res["mixed_colscale"]["s_per_batch"], 3)
+    return out
+
+
+def row_block_system(P, s, k, nblk):
+    """Row block k of nblk of the system as one GPU of a row-sharded solve
+    would hold it: a contiguous block of kernel rows (balanced by nonzeros)
+    and the matching share of the damping rows, all n columns (x replicated)."""
+    import scipy.sparse as sp
+    Ak = sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)).tocsr()
+    cnt = np.cumsum(np.diff(Ak.indptr))
+    lo = 0 if k == 0 else int(np.searchsorted(cnt, s.nnz_k * k / nblk)) + 1
+    hi = int(np.searchsorted(cnt, s.nnz_k * (k + 1) / nblk)) + 1 if k + 1 < nblk else s.m_k
+    D = sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))
+    dlo, dhi = s.m_d * k // nblk, s.m_d * (k + 1) // nblk
+    Kb = Ak[lo:hi].tocsc()
+    Kb.sort_indices()
+    Db = D[dlo:dhi]
+    Dc = Db.tocsc()
+    Dc.sort_indices()
+    return P.System.from_arrays(hi - lo, dhi - dlo, s.n, Kb.indptr, Kb.indices, Kb.data, Db.indptr, Db.indices,
+                                Db.data, Dc.indptr, Dc.indices, Dc.data, np.asarray(s.b)[lo:hi], grid=s.grid)
+
+
+def bench_row_block(P, hip, s, a, log):
+    """One GPU's share of the row-sharded design at 8 GPUs: the operator of
+    one nonzero-balanced block of kernel rows (2M / 8 rays) and 1/8 of the
+    damping rows, all columns, timed per aprod mode with HIP events."""
+    t0 = time.perf_counter()
+    sb = row_block_system(P, s, 0, 8)
+    with P.Operator(sb) as ob:
+        setup_s = time.perf_counter() - t0
+        m = sb.m_k + sb.m_d
+        x_dev, y_dev = hip.malloc(8 * sb.n), hip.malloc(8 * m)
+        try:
+            hip.h2d(x_dev, np.random.default_rng(0).standard_normal(sb.n))
+            hip.h2d(y_dev, np.random.default_rng(1).standard_normal(m) * 1e-3)
+            ms1 = time_aprod(hip, ob, 1, x_dev, y_dev, a.aprod_reps)
+            ms2 = time_aprod(hip, ob, 2, x_dev, y_dev, a.aprod_reps)
+        finally:
+            hip.free(x_dev)
+            hip.free(y_dev)
+    b1, b2 = aprod_bytes(sb.m_k, sb.m_d, sb.n, sb.nnz_k, sb.nnz_d)
+    log(f"[bench] row block 1/8: {sb.m_k} rays, mode 1 {ms1:.3f} ms, mode 2 {ms2:.3f} ms")
+    return {"block": "rows 1 of 8 (nonzero-balanced kernel rows, 1/8 of the damping rows, all columns)",
+            "m_k": sb.m_k, "m_d": sb.m_d, "n": sb.n, "nnz_k": int(sb.nnz_k), "nnz_d": int(sb.nnz_d),
+            "mode1_ms": round(ms1, 4), "mode2_ms": round(ms2, 4), "mode1_bytes": int(b1), "mode2_bytes": int(b2),
+            "mode1_gbs": round(b1 / ms1 / 1e6, 1), "mode2_gbs": round(b2 / ms2 / 1e6, 1),
+            "collective_not_timed": "one all-reduce of n doubles (A^T u partials) per iteration",
+            "setup_s": round(setup_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +655,8 @@ def main():
                          "or the reference when it is there")
     ap.add_argument("--ref-iters", type=int, default=40,
                     help="iterations of the reference run timed for the CPU baseline")
+    ap.add_argument("--ref-iters-all", type=int, default=100,
+                    help="iterations of the reference run on every host core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="FP64 line only: skip the mixed-precision column-scaled solve timed beside it")
@@ -388,10 +674,25 @@ def main():
                     help="test2 the FP64 training targets x* are solved to")
     ap.add_argument("--epochs", type=float, default=20)
     ap.add_argument("--save-params", default=None, help="write the trained parameters (.npy)")
+    ap.add_argument("--learned-pairs", type=int, default=32,
+                    help="default line: held-out right-hand sides of the learned warm-start sub-object")
+    ap.add_argument("--learned-seed0", type=int, default=600000,
+                    help="first seed of those right-hand sides (training used 500000-500511)")
+    ap.add_argument("--learned-converged", type=int, default=2,
+                    help="of them, how many also get a converged FP64 solution for the error figures")
+    ap.add_argument("--converged-tol", type=float, default=1e-12,
+                    help="test2 of the untimed FP64 solution the per-solve errors are measured against")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-row-block", action="store_true")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="skip the optional sub-objects whose estimated time would pass this many seconds")
     ap.add_argument("--net-input", default="jacobi", choices=["atb", "jacobi"],
                     help="network input: A^T b divided by the diagonal of A^T A (x0 error 0.28 at 67^3), "
                          "or A^T b itself (0.43)")
     a = ap.parse_args()
+    t_start = time.perf_counter()
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
               "(the row-sharded multi-GPU design is described but not run)", file=sys.stderr)
@@ -516,6 +817,7 @@ def main():
         xm = np.empty(s.n)
         hip.d2h(xm, x_dev)
         mi = minfos[-1]
+        x_mixed = xm
         mixed_line = {"method": f"mixed precision on A D, D = diag(||A e_j||^-{a.precond_power:g}) "
                                 f"(column scaling), adaptive FP64 checks",
                       "s_per_solve": round(mwall, 6), "iterations_to_tol": mi["itn"],
@@ -525,6 +827,43 @@ def main():
                       "x_rel_diff_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))}
         print(f"[bench] mixed + column scaling: {1000 * mwall:.2f} ms per solve, {mi['itn']} iterations, "
               f"{mixed_line['speedup_vs_fp64']}x the FP64 solve", file=sys.stderr, flush=True)
+
+    # errors of the timed solves against a converged FP64 solution (untimed),
+    # and the sub-objects of the other workloads
+    extra = {}
+    if prec == P.PREC_FP64 and not a.exact_order and not a.aprod_only:
+        t1 = time.perf_counter()
+        xstar, rstar, istar, t2star = converged_solution(op, s, a.converged_tol)
+        log(f"[bench] converged FP64 solution: {istar['itn']} iterations, test2 {istar['arnorm']:.2e} "
+            f"(true {t2star:.2e}), {time.perf_counter() - t1:.1f} s")
+        extra["converged_fp64"] = {"iterations": istar["itn"], "test2": istar["arnorm"], "true_test2": t2star,
+                                   "rnorm": rstar}
+        x64_last = np.empty(s.n)
+        if mixed_line is not None:
+            x64_last = x64
+        else:
+            op.lsqr_device(b_dev, x_dev, a.max_iters, tol=a.tol)
+            hip.d2h(x64_last, x_dev)
+        extra["fp64_errors"] = solution_errors(op, s.b, x64_last, xstar, rstar, infos[-1]["anorm"])
+        if mixed_line is not None:
+            mixed_line.update(solution_errors(op, s.b, x_mixed, xstar, rstar, infos[-1]["anorm"]))
+        budget_left = lambda: a.budget_s - (time.perf_counter() - t_start)  # noqa: E731
+        for name, enabled, need_s, fn in (
+                ("learned", not a.no_learned, 3.5 * a.learned_pairs + 30,
+                 lambda: bench_learned_line(P, hip, op, s, a, log)),
+                ("row_block", not a.no_row_block, 30, lambda: bench_row_block(P, hip, s, a, log)),
+                ("batch", not a.no_batch, 150, lambda: bench_batch_line(P, hip, a, log))):
+            if not enabled:
+                continue
+            if budget_left() < need_s:
+                extra[name] = {"skipped": f"about {need_s:.0f} s needed, {budget_left():.0f} s of the budget left"}
+                continue
+            t1 = time.perf_counter()
+            try:
+                extra[name] = fn()
+            except Exception as e:  # a sub-object never takes the main line down with it
+                extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] {name}: {time.perf_counter() - t1:.1f} s")
 
     # aprod kernels, timed with events on the handle's stream: on vectors in
     # the handle's order (the SpMV kernels alone, as in the solve loop), and
@@ -572,7 +911,7 @@ def main():
     iters_ref = ref64["itn"] if ref64 is not None else iters
     if not a.no_cpu:
         if a.cpu_kind in ("auto", "reference") and not a.exact_order:
-            cpu = cpu_baseline_reference(P, s, iters_ref, a.ref_iters)
+            cpu = cpu_baseline_reference(P, s, iters_ref, a.ref_iters, a.ref_iters_all)
         if cpu is None and a.cpu_kind != "reference":
             cpu = cpu_baseline(P, s, iters_ref, a.cpu_iters)
     out = {
@@ -594,6 +933,7 @@ def main():
             "speedup_device": round(ref64["elapsed_ms"] / max(infos[-1]["elapsed_ms"], 1e-9), 3),
             "x_rel_diff": ref64["xdiff"]},
         "mixed_colscale": mixed_line,
+        **extra,
         "aprod_gbs": {"mode1": round(gbs1, 1), "mode2": round(gbs2, 1), "both": round(gbs, 1)},
         "aprod_caller_order_ms": {"mode1": round(cm1, 4), "mode2": round(cm2, 4)},
         "roofline": roofline,

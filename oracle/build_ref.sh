@@ -41,6 +41,24 @@ for f in gptl threadutil GPTLutil; do
 done
 "$MPICC" -o "$OUT/plsqr3_ref_fp64" "$TMP"/*.o -lm
 echo "built $OUT/plsqr3_ref_fp64"
+# The drop-in: the same reference objects with source/aprod.c replaced by the
+# binding a maintainer would add (plsqr3.2013.09_amd/integration/aprod_plsqr3.c),
+# linked against libplsqr_hip.so (built first by plsqr3.2013.09_amd/build.py).
+PKG="$(dirname "$HERE")/plsqr3.2013.09_amd"
+if [ -f "$PKG/libplsqr_hip.so" ]; then
+    "$MPICC" $FLAGS -I"$TMP" -I"$REF" -I"$REF/tool" -I"$(dirname "$HERE")/include" -I- \
+        -c "$PKG/integration/aprod_plsqr3.c" -o "$TMP/aprod_plsqr3.o"
+    # linked by gcc itself so that the system libstdc++ (which the HIP runtime
+    # needs) comes before the older one next to MPICH in the run-time search
+    MPILIB=$(dirname "$MPICC")/../lib
+    gcc -o "$OUT/plsqr3_hip_dropin" $(ls "$TMP"/*.o | grep -v '/aprod\.o$') \
+        -L"$PKG" -lplsqr_hip -Wl,-rpath,'$ORIGIN/../../plsqr3.2013.09_amd' \
+        -Wl,-rpath,/usr/lib/x86_64-linux-gnu -Wl,-rpath,/opt/rocm/lib -Wl,-rpath-link,/opt/rocm/lib \
+        -L"$MPILIB" -lmpi -Wl,-rpath,"$MPILIB" -lm
+    echo "built $OUT/plsqr3_hip_dropin"
+else
+    echo "libplsqr_hip.so not built yet; drop-in binary skipped" >&2
+fi
 # the kernel-format tools (PLSQR3_tools/kernel_format/makefile builds them the
 # same way: each .c with FileIO.c where it is used); they write the golden
 # digests of the example dataset's conversion chain

@@ -1,10 +1,9 @@
-"""Build recipe for libplsqr_hip.so (gfx950) and the host CLI.
+"""Build recipe for libplsqr_hip.so (gfx950).
 
-    python plsqr3.2013.09_amd/build.py          # library + CLI
+    python plsqr3.2013.09_amd/build.py
 Compiles in-tree so the built files travel with the source tree.
 """
 import os
-import shutil
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -14,7 +13,6 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libplsqr_hip.so")
-CLI = os.path.join(HERE, "plsqr3_hip")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PLSQR_ARCH", "gfx950")
@@ -56,19 +54,8 @@ def build_lib(out=LIB):
     return out
 
 
-def build_cli(out=CLI):
-    src = os.path.join(CSRC, "plsqr3_hip_main.c")
-    if not os.path.exists(src):
-        return None
-    cc = shutil.which("gcc") or "cc"
-    run([cc, "-O2", "-std=c99", "-Wall", "-I", INCLUDE, src, "-o", out,
-         "-L", HERE, "-lplsqr_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
-    return out
-
-
 def main():
     build_lib()
-    build_cli()
 
 
 if __name__ == "__main__":
