@@ -469,6 +469,17 @@ int  plsqr_cnn_conv_forward(int nx, int ny, int nz, int batch, const float *in, 
                             const float *bias, int relu, float *out);
 int  plsqr_cnn_conv_backward(int nx, int ny, int nz, int batch, const float *in, const float *w,
                              const float *dout, float *din, float *dw, float *dbias);
+/* plsqr_cnn_conv_forward on device pointers that allocates nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  in_dev and out_dev
+ * hold batch*nx*ny*nz*32 floats, w_dev 27*32*32, bias_dev 32 (or NULL).
+ * Enqueued on stream (hipStream_t as void*, NULL = the default stream); no
+ * synchronisation.  Its scratch (the bf16 operands) is one fixed array per
+ * device inside the library, so batch*nx*ny*nz may not exceed 131072
+ * (PLSQR_ERR_ARG), and calls that overlap on one device race. */
+int  plsqr_cnn_conv_forward_noalloc(int nx, int ny, int nz, int batch, const float *in_dev,
+                                    const float *w_dev, const float *bias_dev, int relu, float *out_dev,
+                                    void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batched solves: many independent systems, up to 4 right-hand sides each */

@@ -187,6 +187,8 @@ SIGNATURES = {
                                          c_float_p, C.c_int, c_float_p]),
     "plsqr_cnn_conv_backward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p,
                                           c_float_p, c_float_p, c_float_p, c_float_p]),
+    "plsqr_cnn_conv_forward_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "plsqr_batch_create": (C.c_int, [C.POINTER(C.POINTER(SystemC)), C.POINTER(GridC), C.c_int,
                                      C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "plsqr_batch_destroy": (None, [C.c_void_p]),

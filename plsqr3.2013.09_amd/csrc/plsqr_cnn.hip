@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -269,6 +270,44 @@ int load_samples(plsqr_cnn *h, int batch, const double *atb, const double *xs, s
     return PLSQR_OK;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of plsqr_cnn_conv_forward for callers that cannot
+// allocate device memory (HIP graph capture, serving loops).  Its scratch --
+// the bf16 copy of the input and the two bf16 weight layouts -- lives in one
+// file-scope device array, fixed in size; each device holds its own copy,
+// found once per device with hipGetSymbolAddress.  Every element the kernels
+// read is written earlier in the same call, so no clearing is needed; calls
+// that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr long long NOALLOC_MAX_ELEMS = 1LL << 22;  // batch * voxels * 32 channels of input
+constexpr size_t NOALLOC_BYTES = sizeof(uint16_t) * ((size_t)NOALLOC_MAX_ELEMS + 2 * (size_t)WSZ);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's address of `scratch` (looked up on the first call on
+// that device, with that device current).
+int scratch_ptr(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_err(nullptr, PLSQR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return set_err(nullptr, PLSQR_ERR_STATE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(nullptr, PLSQR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PLSQR_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -534,6 +573,35 @@ int plsqr_cnn_conv_forward(int nx, int ny, int nz, int batch, const float *in, c
         if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(float) * total * C, hipMemcpyDeviceToHost);
     }
     cleanup();
+    if (e != hipSuccess) return set_err(nullptr, PLSQR_ERR_HIP, std::string("conv forward: ") + hipGetErrorString(e));
+    return PLSQR_OK;
+}
+
+// plsqr_cnn_conv_forward on device pointers with the scratch above: no allocation.
+int plsqr_cnn_conv_forward_noalloc(int nx, int ny, int nz, int batch, const float *in_dev, const float *w_dev,
+                                   const float *bias_dev, int relu, float *out_dev, void *stream) {
+    g_cnn_err.clear();
+    if (nx < 1 || ny < 1 || nz < 1 || batch < 1 || !in_dev || !w_dev || !out_dev)
+        return set_err(nullptr, PLSQR_ERR_ARG, "bad arguments");
+    const long long total = (long long)batch * nx * ny * nz;
+    if (total > NOALLOC_MAX_ELEMS / C)
+        return set_err(nullptr, PLSQR_ERR_ARG,
+                       "batch * nx * ny * nz = " + std::to_string(total) + " exceeds the allocation-free limit of " +
+                           std::to_string(NOALLOC_MAX_ELEMS / C) + " voxels");
+    void *base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc) return rc;
+    uint16_t *bin = (uint16_t *)base;
+    uint16_t *wf = bin + NOALLOC_MAX_ELEMS, *wb = wf + WSZ;
+    const hipStream_t st = (hipStream_t)stream;
+    plsqr_cnn tmp;  // grid only: no buffers
+    tmp.nx = nx;
+    tmp.ny = ny;
+    tmp.nz = nz;
+    hipLaunchKernelGGL(kc_to_bf16, dim3(grid_for(total * C, 256, 4096)), dim3(256), 0, st, total * C, in_dev, bin);
+    hipLaunchKernelGGL(kc_prep_weights, dim3((WSZ + 255) / 256), dim3(256), 0, st, w_dev, wf, wb);
+    conv_fwd(&tmp, total, bin, wf, bias_dev, nullptr, relu, nullptr, out_dev, st);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_err(nullptr, PLSQR_ERR_HIP, std::string("conv forward: ") + hipGetErrorString(e));
     return PLSQR_OK;
 }
