@@ -288,7 +288,8 @@ def bench_batch(P, hip, a):
         bt.set_anorm_refs([[i["anorm"] for i in row] for row in ref64[0]])
         pc = P.PRECOND_COLSCALE
         print(f"[bench] plain FP64 batch: {ref64[1]:.1f} ms", file=sys.stderr, flush=True)
-    kw = dict(tol=a.tol, precision=prec, refine_every=a.refine_every, precond=pc, precond_power=a.precond_power)
+    kw = dict(tol=a.tol, precision=prec, refine_every=a.refine_every, precond=pc, precond_power=a.precond_power,
+              refine_adaptive=a.refine_adaptive)
     for _ in range(a.warmup):
         bt.solve_device(b_dev, x_dev, a.max_iters, **kw)
     hip.sync()
@@ -468,6 +469,7 @@ def bench_learned_line(P, hip, op, s, a, log):
             t1 = time.perf_counter()
             hip.h2d(atb_dev, zeros_n)
             op.aprod_device(2, atb_dev, y_dev)
+            hip.sync()  # the operator's stream does not order with the copy below
             atb = np.empty(n)
             hip.d2h(atb, atb_dev)
             x0 = net.predict((atb * inv)[None, :])[0]
@@ -545,7 +547,8 @@ def bench_batch_line(P, hip, a, log):
         try:
             for label, kw in (("fp64", dict()),
                               ("mixed_colscale", dict(precision=P.PREC_MIXED, refine_every=a.refine_every,
-                                                      precond=P.PRECOND_COLSCALE, precond_power=a.precond_power))):
+                                                      precond=P.PRECOND_COLSCALE, precond_power=a.precond_power,
+                                                      refine_adaptive=True))):
                 if label == "mixed_colscale":
                     bt.set_anorm_refs([[i["anorm"] for i in row] for row in res["fp64"]["infos"]])
                 bt.solve_device(b_dev, x_dev, a.max_iters, tol=a.tol, **kw)  # warm-up
@@ -682,6 +685,8 @@ def main():
                     help="of them, how many also get a converged FP64 solution for the error figures")
     ap.add_argument("--converged-tol", type=float, default=1e-12,
                     help="test2 of the untimed FP64 solution the per-solve errors are measured against")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="skip the converged FP64 solution and the error figures measured against it")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-row-block", action="store_true")
@@ -827,11 +832,32 @@ def main():
                       "x_rel_diff_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))}
         print(f"[bench] mixed + column scaling: {1000 * mwall:.2f} ms per solve, {mi['itn']} iterations, "
               f"{mixed_line['speedup_vs_fp64']}x the FP64 solve", file=sys.stderr, flush=True)
+        # the same scaled solve with the FP64 operator: what the scaling gains
+        # without FP32 arithmetic (3 timed solves)
+        kw64 = dict(kw, precision=P.PREC_FP64)
+        op.lsqr_device(b_dev, x_dev, a.max_iters, **kw64)
+        hip.sync()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            fi = op.lsqr_device(b_dev, x_dev, a.max_iters, **kw64)
+        hip.sync()
+        fwall = (time.perf_counter() - t1) / 3
+        x_fp64cs = np.empty(s.n)
+        hip.d2h(x_fp64cs, x_dev)
+        mixed_line["fp64_same_scaling"] = {
+            "s_per_solve": round(fwall, 6), "iterations_to_tol": fi["itn"], "fp64_checks": fi["refinements"],
+            "tol_reached": bool(fi["arnorm"] <= a.tol),
+            "ms_per_iteration": round(fi["elapsed_ms"] / max(fi["itn"], 1), 4),
+            "speedup_vs_fp64": round((wall / a.steps) / fwall, 3),
+            "x_rel_diff_vs_fp64": float(np.linalg.norm(x_fp64cs - x64) / np.linalg.norm(x64)),
+            "note": "column scaling alone; the mixed line's gain over this is FP32 arithmetic"}
+        print(f"[bench] FP64 + column scaling: {1000 * fwall:.2f} ms per solve, {fi['itn']} iterations",
+              file=sys.stderr, flush=True)
 
     # errors of the timed solves against a converged FP64 solution (untimed),
     # and the sub-objects of the other workloads
     extra = {}
-    if prec == P.PREC_FP64 and not a.exact_order and not a.aprod_only:
+    if prec == P.PREC_FP64 and not a.exact_order and not a.aprod_only and not a.no_converged:
         t1 = time.perf_counter()
         xstar, rstar, istar, t2star = converged_solution(op, s, a.converged_tol)
         log(f"[bench] converged FP64 solution: {istar['itn']} iterations, test2 {istar['arnorm']:.2e} "

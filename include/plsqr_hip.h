@@ -323,8 +323,12 @@ enum {
                               FP64 scalars and an FP64 residual refinement          */
 };
 
-/* Preconditioning of the mixed-precision solve (no counterpart in the
- * reference, whose lsqr() always runs on A itself). */
+/* Preconditioning (no counterpart in the reference, whose lsqr() always runs
+ * on A itself).  A column-scaled solve runs its inner LSQR on A D in the
+ * solve's precision (FP32 operator for PLSQR_PREC_MIXED, FP64 operator for
+ * PLSQR_PREC_FP64) and checks test2 on A in FP64 every refine_every inner
+ * iterations (refine_adaptive as for the mixed solve); no trace, no
+ * standard errors. */
 enum {
     PLSQR_PRECOND_NONE = 0,      /* LSQR on A                                          */
     PLSQR_PRECOND_COLSCALE = 1   /* LSQR on A D with D = diag(||A e_j||^-p), p =
@@ -344,8 +348,8 @@ typedef struct plsqr_solve_opts {
     double atol, btol, conlim; /* istop tests (lsqr.c:759-775); reference values
                               1e-10, 1e-10, 100 when all three are 0            */
     int    itnlim;         /* istop=5 threshold; 1000 (lsqr.c:50) when 0          */
-    int    precond;        /* mixed only: PLSQR_PRECOND_*                         */
-    double anorm_ref;      /* mixed with PLSQR_PRECOND_COLSCALE: the ||A|| of the
+    int    precond;        /* PLSQR_PRECOND_*                                     */
+    double anorm_ref;      /* with PLSQR_PRECOND_COLSCALE: the ||A|| of the
                               FP64 test2 = ||A^T r|| / (anorm_ref ||r||); 0 = ||A||_F.
                               Passing the anorm a plain FP64 solve reported makes
                               the stop exactly as strict as that solve's         */
@@ -552,7 +556,10 @@ int plsqr_batch_aprod(plsqr_batch *h, int mode, int precision, double *x, double
  * MIXED accepts precond = PLSQR_PRECOND_COLSCALE (precond_power as for
  * plsqr_lsqr_solve): each problem's inner LSQR runs on A_p D_p and pair i's
  * FP64 test2 takes the anorm set by plsqr_batch_set_anorm_refs, or
- * ||A_p||_F; refine_adaptive and anorm_ref are not used here. */
+ * ||A_p||_F (anorm_ref is not used here).  refine_adaptive = 1 (MIXED,
+ * tol > 0): after the first check, the next one comes as soon as any live
+ * pair's inner test2 estimate times its FP64 / inner ratio at the last check
+ * reaches tol (polled every 8 iterations), at most 4 * refine_every apart. */
 int plsqr_batch_solve(plsqr_batch *h, const double *b_host, const plsqr_solve_opts *opts,
                       double *x_host, plsqr_solve_out *out, double *total_ms);
 /* The anorm of the FP64 test2 of every (problem, right-hand side) pair of a

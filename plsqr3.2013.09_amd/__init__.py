@@ -338,23 +338,25 @@ class Batch:
         self._check(self.lib.plsqr_batch_set_anorm_refs(self.h, dptr(r)))
 
     def solve(self, B, itn, tol=0.0, precision=PREC_FP64, refine_every=25, damp=0.0, precond=PRECOND_NONE,
-              precond_power=0.0):
+              precond_power=0.0, refine_adaptive=False):
         """B: list of (K, m_k) arrays.  Returns (X list of (K, n), infos [p][k] dicts, total_ms)."""
         b = self._pack(B, self.m_k)
         x = np.empty(sum(self.n) * self.K, np.float64)
         out = (SolveOutC * (self.P * self.K))()
         tot = C.c_double()
-        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power)
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power,
+                               refine_adaptive=refine_adaptive)
         self._check(self.lib.plsqr_batch_solve(self.h, dptr(b), C.byref(o), dptr(x), out, C.byref(tot)))
         infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]
         return self._unpack(x, self.n), infos, tot.value
 
     def solve_device(self, b_ptr, x_ptr, itn, tol=0.0, precision=PREC_FP64, refine_every=25,
-                     damp=0.0, precond=PRECOND_NONE, precond_power=0.0):
+                     damp=0.0, precond=PRECOND_NONE, precond_power=0.0, refine_adaptive=False):
         """b_ptr, x_ptr: device arrays in the library's batch layout.  Returns (infos, total_ms)."""
         out = (SolveOutC * (self.P * self.K))()
         tot = C.c_double()
-        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power)
+        o = Operator.make_opts(itn, tol, precision, refine_every, damp, precond, precond_power=precond_power,
+                               refine_adaptive=refine_adaptive)
         self._check(self.lib.plsqr_batch_solve_device(self.h, C.c_void_p(b_ptr), C.byref(o),
                                                       C.c_void_p(x_ptr), out, C.byref(tot)))
         infos = [[out[p * self.K + k].as_dict() for k in range(self.K)] for p in range(self.P)]

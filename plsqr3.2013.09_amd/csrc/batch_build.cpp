@@ -213,7 +213,9 @@ int build_batch_plan(const plsqr_system *const *systems, const plsqr_grid *grids
         std::vector<int> slices = global_sort ? std::vector<int>() : cut_slices(s);
         L.S = S;
         RaySorted rays;
-        sort_rays(s, perm, tiles, slices, (std::getenv("PLSQR_RAYKEY") && std::atoi(std::getenv("PLSQR_RAYKEY")) == 0) ? nullptr : g, &rays);
+        // damping rows stay in place: the batch keeps its own damping row chunks
+        sort_rays(s, perm, tiles, slices, (std::getenv("PLSQR_RAYKEY") && std::atoi(std::getenv("PLSQR_RAYKEY")) == 0) ? nullptr : g, &rays,
+                  false);
         L.rperm = rays.rperm;
         const plsqr_system *ss = &rays.sys;
         if (global_sort) slices = cut_slices(ss);

@@ -43,6 +43,12 @@ struct plsqr_op {
     // damping CSC by column
     int *dcp = nullptr, *dri = nullptr;
     double *dcv = nullptr;
+    // the same columns interleaved by groups of 64 (tiled operator): column c's
+    // k-th entry at dpos[c] + 64 k, dlen[c] entries; NULL when not built
+    int *dpos = nullptr, *dri_il = nullptr;
+    double *dcv_il = nullptr;
+    uint8_t *dlen = nullptr;
+    long long d_il = 0;  // slots of the interleaved arrays
     // vectors
     double *u = nullptr, *v = nullptr, *x = nullptr, *w = nullptr, *se = nullptr;
     double *bst = nullptr;  // b staging (m_k)
@@ -94,7 +100,7 @@ struct plsqr_op {
     // ray order: kernel rows renumbered so that rays crossing the same tiles
     // are adjacent; every row-space vector on the device is in the new order
     bool rowperm = false;
-    int *rperm = nullptr, *rinv = nullptr;  // new -> old, old -> new (kernel rows)
+    int *rperm = nullptr, *rinv = nullptr;  // new -> old, old -> new (m rows: kernel, then damping)
     double *hy2 = nullptr;                  // m, new-order staging
     double *bint = nullptr;                 // m_k, b in the new order
     double *xt = nullptr;                   // n: the mixed solve's checked iterate x + dx
@@ -164,6 +170,12 @@ int env_int(const char *name, int dflt) {
     const char *e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
 }
+
+// damping columns as the combine kernels read them: interleaved when built,
+// else the CSC (dlen NULL)
+inline const int *dcol_start(const plsqr_op *op) { return op->dlen ? op->dpos : op->dcp; }
+inline const int *dcol_row(const plsqr_op *op) { return op->dlen ? op->dri_il : op->dri; }
+inline const double *dcol_val(const plsqr_op *op) { return op->dlen ? op->dcv_il : op->dcv; }
 
 // Pinned host staging of the handle, allocated once at create.
 int alloc_host_staging(plsqr_op *op) {
@@ -325,7 +337,10 @@ int launch_residual(plsqr_op *op, const double *x, const double *b, double *r) {
 
 
 
-int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part) {
+// cs: column scales of an LSQR on A D (the FP64 inner iteration of a scaled
+// solve), NULL for A itself
+int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double *part,
+                const double *cs = nullptr) {
     using namespace plsqr;
     if (op->tiled) {
 #define COLS(TW, B, CAP, NT)                                                                                          \
@@ -344,7 +359,8 @@ int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double
 #undef COLS
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
-                           op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
+                           op->P, dcol_start(op), dcol_row(op), dcol_val(op), u, sc, v, part, cs,
+                           (const uint8_t *)op->dlen);
         LAUNCH_CHECK(op);
         return PLSQR_OK;
     }
@@ -360,7 +376,7 @@ int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double
     }
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL((k_cols_combine<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->nsl,
-                       op->P, op->dcp, op->dri, op->dcv, u, sc, v, part);
+                       op->P, op->dcp, op->dri, op->dcv, u, sc, v, part, cs);
     LAUNCH_CHECK(op);
     return PLSQR_OK;
 }
@@ -405,18 +421,16 @@ int to_external(plsqr_op *op, const double *in, double *ext) {
 }
 
 // row-space vectors (m = kernel rows then damping rows) between the caller's
-// order and the device order; only the kernel rows move
+// order and the device order (kernel rows among themselves, damping rows
+// among themselves)
 int rows_to_internal(plsqr_op *op, const double *ext, double *in) {
     if (!op->rowperm) {
         if (in != ext) HIP_TRY(op, hipMemcpyAsync(in, ext, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
         return PLSQR_OK;
     }
-    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m_k, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m,
                        op->rperm, ext, in);
     LAUNCH_CHECK(op);
-    if (op->m_d > 0)
-        HIP_TRY(op, hipMemcpyAsync(in + op->m_k, ext + op->m_k, sizeof(double) * op->m_d, hipMemcpyDeviceToDevice,
-                                   op->stream));
     return PLSQR_OK;
 }
 
@@ -425,12 +439,9 @@ int rows_to_external(plsqr_op *op, const double *in, double *ext) {
         if (in != ext) HIP_TRY(op, hipMemcpyAsync(ext, in, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
         return PLSQR_OK;
     }
-    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m_k, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m_k,
+    hipLaunchKernelGGL(plsqr::k_gather, dim3(grid_for(op->m, 256, op->cu * 4)), dim3(256), 0, op->stream, op->m,
                        op->rinv, in, ext);
     LAUNCH_CHECK(op);
-    if (op->m_d > 0)
-        HIP_TRY(op, hipMemcpyAsync(ext + op->m_k, in + op->m_k, sizeof(double) * op->m_d, hipMemcpyDeviceToDevice,
-                                   op->stream));
     return PLSQR_OK;
 }
 
@@ -644,6 +655,43 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         if (nd > 0) {
             HIP_TRY(op, hipMemcpy(op->dri, ri.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
             HIP_TRY(op, hipMemcpy(op->dcv, cv.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+        }
+        // interleaved copy for the combine kernels: the columns in groups of 64,
+        // each group as rows of 64 slots (slot k of column c at dpos[c] + 64 k)
+        // as deep as its longest column; PLSQR_DAMP_IL=0 keeps the CSC
+        int maxlen = 0;
+        for (int j = 0; j < op->n; ++j) maxlen = std::max(maxlen, cp[(size_t)j + 1] - cp[(size_t)j]);
+        if (nd > 0 && maxlen <= 255 && env_int("PLSQR_DAMP_IL", 1) != 0) {
+            const int ng = (op->n + 63) / 64;
+            std::vector<int> pos((size_t)op->n);
+            std::vector<uint8_t> len((size_t)op->n);
+            long long slots = 0;
+            for (int g = 0; g < ng; ++g) {
+                int w = 0;
+                for (int j = g * 64; j < std::min(op->n, g * 64 + 64); ++j) {
+                    len[(size_t)j] = (uint8_t)(cp[(size_t)j + 1] - cp[(size_t)j]);
+                    w = std::max(w, (int)len[(size_t)j]);
+                    pos[(size_t)j] = (int)(slots + (j - g * 64));
+                }
+                slots += (long long)w * 64;
+            }
+            if (slots < INT_MAX) {
+                std::vector<int> ril((size_t)slots, 0);
+                std::vector<double> vil((size_t)slots, 0.0);
+                for (int j = 0; j < op->n; ++j)
+                    for (int k = 0; k < len[(size_t)j]; ++k) {
+                        ril[(size_t)pos[(size_t)j] + (size_t)k * 64] = ri[(size_t)cp[(size_t)j] + k];
+                        vil[(size_t)pos[(size_t)j] + (size_t)k * 64] = cv[(size_t)cp[(size_t)j] + k];
+                    }
+                if ((rc = dalloc(op, &op->dpos, op->n)) || (rc = dalloc(op, &op->dlen, op->n)) ||
+                    (rc = dalloc(op, &op->dri_il, slots)) || (rc = dalloc(op, &op->dcv_il, slots)))
+                    return rc;
+                op->d_il = slots;
+                HIP_TRY(op, hipMemcpy(op->dpos, pos.data(), sizeof(int) * op->n, hipMemcpyHostToDevice));
+                HIP_TRY(op, hipMemcpy(op->dlen, len.data(), op->n, hipMemcpyHostToDevice));
+                HIP_TRY(op, hipMemcpy(op->dri_il, ril.data(), sizeof(int) * slots, hipMemcpyHostToDevice));
+                HIP_TRY(op, hipMemcpy(op->dcv_il, vil.data(), sizeof(double) * slots, hipMemcpyHostToDevice));
+            }
         }
     } else {
         HIP_TRY(op, hipMemcpyAsync(op->dcp, s->dc_col_ptr, sizeof(int) * (op->n + 1), hipMemcpyHostToDevice, op->stream));
@@ -900,7 +948,11 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         }
         if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
             R.reset(new plsqr::RowTiles());
-            if (!plsqr::build_row_tiles(*L, sys->m_k, R.get(), &why)) {
+            // segments in ray order inside each item; PLSQR_SEGWIN=w sorts them by
+            // length inside windows of w so a wavefront's lanes finish together
+            // (mode 1 0.59 -> 0.69 ms at 67^3/2M rays with w = 256..1024: the
+            // per-ray sums then gather partials from more cache lines)
+            if (!plsqr::build_row_tiles(*L, sys->m_k, R.get(), &why, env_int("PLSQR_SEGWIN", 1))) {
                 R.reset();  // mode 1 stays on the CSR kernel
             } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::row_tiles_self_check(sys, *L, *R, &why)) {
                 return fail(set_err(op, PLSQR_ERR_STATE, why));
@@ -922,11 +974,11 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     L.reset();
     if (rays) {
         op->rowperm = true;
-        if ((rc = dalloc(op, &op->rperm, op->m_k)) || (rc = dalloc(op, &op->rinv, op->m_k)) ||
+        if ((rc = dalloc(op, &op->rperm, op->m)) || (rc = dalloc(op, &op->rinv, op->m)) ||
             (rc = dalloc(op, &op->hy2, op->m)) || (rc = dalloc(op, &op->bint, op->m_k)))
             return fail(rc);
-        e = hipMemcpy(op->rperm, rays->rperm.data(), sizeof(int) * op->m_k, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(op->rinv, rays->rinv.data(), sizeof(int) * op->m_k, hipMemcpyHostToDevice);
+        e = hipMemcpy(op->rperm, rays->rperm.data(), sizeof(int) * op->m, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(op->rinv, rays->rinv.data(), sizeof(int) * op->m, hipMemcpyHostToDevice);
         if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e)));
         rays.reset();
     }
@@ -1094,7 +1146,8 @@ static int aprod32_internal(plsqr_op *op, int mode, float *x, float *y) {
     } else {
         launch_cols_tiled32(op, y, sp);
         hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
-                           op->P, op->dcp, op->dri, op->dcv, y, sp, x, op->part);
+                           op->P, dcol_start(op), dcol_row(op), dcol_val(op), y, sp, x, op->part,
+                           (const double *)nullptr, (const uint8_t *)op->dlen);
     }
     LAUNCH_CHECK(op);
     return PLSQR_OK;
@@ -1124,17 +1177,14 @@ int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
     if (mode != 1 && mode != 2) return set_err(op, PLSQR_ERR_ARG, "mode must be 1 or 2");
     int rc;
     if ((rc = ensure_aprod32(op))) return rc;
-    const int gn = grid_for(op->n, 256, op->cu * 4), gm = grid_for(op->m_k, 256, op->cu * 4);
+    const int gn = grid_for(op->n, 256, op->cu * 4), gm = grid_for(op->m, 256, op->cu * 4);
     // caller order -> device order
     if (op->permuted)
         hipLaunchKernelGGL(k_gather32, dim3(gn), dim3(256), 0, op->stream, op->n, op->inv, x_dev, op->v32);
     else
         HIP_TRY(op, hipMemcpyAsync(op->v32, x_dev, sizeof(float) * op->n, hipMemcpyDeviceToDevice, op->stream));
     if (op->rowperm) {
-        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m_k, op->rperm, y_dev, op->u32);
-        if (op->m_d > 0)
-            HIP_TRY(op, hipMemcpyAsync(op->u32 + op->m_k, y_dev + op->m_k, sizeof(float) * op->m_d,
-                                       hipMemcpyDeviceToDevice, op->stream));
+        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m, op->rperm, y_dev, op->u32);
     } else {
         HIP_TRY(op, hipMemcpyAsync(op->u32, y_dev, sizeof(float) * op->m, hipMemcpyDeviceToDevice, op->stream));
     }
@@ -1147,10 +1197,7 @@ int plsqr_aprod_device_f32(plsqr_op *op, int mode, float *x_dev, float *y_dev) {
         else
             HIP_TRY(op, hipMemcpyAsync(x_dev, op->v32, sizeof(float) * op->n, hipMemcpyDeviceToDevice, op->stream));
     } else if (op->rowperm) {
-        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m_k, op->rinv, op->u32, y_dev);
-        if (op->m_d > 0)
-            HIP_TRY(op, hipMemcpyAsync(y_dev + op->m_k, op->u32 + op->m_k, sizeof(float) * op->m_d,
-                                       hipMemcpyDeviceToDevice, op->stream));
+        hipLaunchKernelGGL(k_gather32, dim3(gm), dim3(256), 0, op->stream, op->m, op->rinv, op->u32, y_dev);
     } else {
         HIP_TRY(op, hipMemcpyAsync(y_dev, op->u32, sizeof(float) * op->m, hipMemcpyDeviceToDevice, op->stream));
     }
@@ -1292,7 +1339,9 @@ void plsqr_aprod(int mode, int m, int n, double x[], double y[], void *UsrWrk) {
 // ---------------------------------------------------------------------------
 // device-resident LSQR
 // ---------------------------------------------------------------------------
-static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, double *trace_d) {
+// cs: column scales (an FP64 LSQR on A D), or NULL
+static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, double *trace_d,
+                             const double *cs = nullptr) {
     using namespace plsqr;
     int rc;
     if (op->exact) {
@@ -1324,7 +1373,7 @@ static int enqueue_iteration(plsqr_op *op, double *x, double *w, double *se, dou
     if ((rc = launch_rows(op, op->v, op->u, op->sc, op->part))) return rc;
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
     LAUNCH_CHECK(op);
-    if ((rc = launch_cols(op, op->u, op->v, op->sc, op->part))) return rc;
+    if ((rc = launch_cols(op, op->u, op->v, op->sc, op->part, cs))) return rc;
     hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4);
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL((k_update<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v, w,
@@ -1345,7 +1394,7 @@ static int solve_fp64(plsqr_op *op, const double *b_dev, const plsqr_solve_opts 
     using namespace plsqr;
     int rc;
     if (o->precond != PLSQR_PRECOND_NONE)
-        return set_err(op, PLSQR_ERR_ARG, "column scaling is offered on the mixed-precision solve");
+        return set_err(op, PLSQR_ERR_ARG, "the column-scaled solve has no trace or standard errors");
     const int itn = std::max(0, o->itn);
     const bool want_trace = trace_host != nullptr;
     if (want_trace && (rc = ensure_trace(op, itn + 1))) return rc;
@@ -1511,7 +1560,8 @@ static int enqueue_iteration32(plsqr_op *op) {
     hipLaunchKernelGGL(k_s_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G1);
     launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
     hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS, op->P,
-                       op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part, op->cs_active);
+                       dcol_start(op), dcol_row(op), dcol_val(op), op->u32, (const Scal *)op->sc, op->v32, op->part,
+                       op->cs_active, (const uint8_t *)op->dlen);
     LAUNCH_CHECK(op);
     hipLaunchKernelGGL(k_s_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4);
     hipLaunchKernelGGL((k_update32<BS>), dim3(op->G6), dim3(BS), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
@@ -1577,11 +1627,16 @@ static int ensure_colscale(plsqr_op *op, double power) {
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, 1.0, ones);
     hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->colss);
     LAUNCH_CHECK(op);
+    // the squared values stand in for the values for one mode 2; the damping
+    // columns are read from the CSC for it
+    uint8_t *dl = op->dlen;
+    op->dlen = nullptr;
     std::swap(op->t_sval, sq_t);
     std::swap(op->dcv, sq_d);
     rc = launch_cols(op, ones, op->colss, nullptr, nullptr);
     std::swap(op->t_sval, sq_t);
     std::swap(op->dcv, sq_d);
+    op->dlen = dl;
     HIP_TRY(op, hipStreamSynchronize(op->stream));
     dfree(op, sq_t, op->t_nslot);
     dfree(op, sq_d, op->nnz_d);
@@ -1624,7 +1679,13 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
                        const double *x_init = nullptr) {
     using namespace plsqr;
     int rc;
-    if ((rc = ensure_fp32(op))) return rc;
+    // PLSQR_PREC_FP64 here: the same solve with an FP64 inner LSQR (the
+    // FP64 operator on A D), which separates what the scaling gains from
+    // what FP32 arithmetic gains
+    const bool inner64 = o->precision == PLSQR_PREC_FP64;
+    if (!inner64 && (rc = ensure_fp32(op))) return rc;
+    if (inner64 && !op->tiled) return set_err(op, PLSQR_ERR_STATE, "column scaling needs the tiled operator");
+    if (!op->dx && (rc = dalloc(op, &op->dx, op->n))) return rc;
     if (!op->xt && (rc = dalloc(op, &op->xt, op->n))) return rc;
     double *const xt = op->xt;  // the checked iterate x + dx (n doubles, never x_init)
     const bool colscale = o->precond == PLSQR_PRECOND_COLSCALE;
@@ -1672,25 +1733,42 @@ static int solve_mixed(plsqr_op *op, const double *b_dev, const plsqr_solve_opts
         z.itn_max = itn - total;
         z.cs2 = -1.0;
         HIP_TRY(op, hipMemcpyAsync(op->sc, &z, sizeof(z), hipMemcpyHostToDevice, op->stream));
-        hipLaunchKernelGGL(k_to_f32, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, op->hy, op->u32);
-        hipLaunchKernelGGL(k_fill32, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, op->v32);
+        if (inner64) {
+            HIP_TRY(op, hipMemcpyAsync(op->u, op->hy, sizeof(double) * op->m, hipMemcpyDeviceToDevice, op->stream));
+            hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->v);
+        } else {
+            hipLaunchKernelGGL(k_to_f32, dim3(gz), dim3(256), 0, op->stream, (long long)op->m, op->hy, op->u32);
+            hipLaunchKernelGGL(k_fill32, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, op->v32);
+        }
         hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->dx);
         hipLaunchKernelGGL(k_fill, dim3(gz), dim3(256), 0, op->stream, (long long)op->n, 0.0, op->w);
         hipLaunchKernelGGL((k_ssq2<BS>), dim3(gs), dim3(BS), 0, op->stream, op->m_k, op->m, op->hy, op->part);
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_beta, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, gs);
-        launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
-        hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
-                           op->P, op->dcp, op->dri, op->dcv, op->u32, (const Scal *)op->sc, op->v32, op->part,
-                           op->cs_active);
+        if (inner64) {
+            if ((rc = launch_cols(op, op->u, op->v, op->sc, op->part, op->cs_active))) return rc;
+        } else {
+            launch_cols_tiled32(op, op->u32, (const Scal *)op->sc);
+            hipLaunchKernelGGL((k_cols_combine32<BS>), dim3(op->G4), dim3(BS), 0, op->stream, op->n, op->m_k, op->tS,
+                               op->P, dcol_start(op), dcol_row(op), dcol_val(op), op->u32, (const Scal *)op->sc,
+                               op->v32, op->part, op->cs_active, (const uint8_t *)op->dlen);
+        }
         LAUNCH_CHECK(op);
         hipLaunchKernelGGL(k_s_init_alpha, dim3(1), dim3(256), 0, op->stream, op->sc, op->part, op->G4,
                            (double *)nullptr);
-        hipLaunchKernelGGL(k_init_vw32, dim3(gz), dim3(256), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
-                           op->w);
+        if (inner64)
+            hipLaunchKernelGGL(k_init_vw, dim3(gz), dim3(256), 0, op->stream, op->n, (const Scal *)op->sc, op->v,
+                               op->w);
+        else
+            hipLaunchKernelGGL(k_init_vw32, dim3(gz), dim3(256), 0, op->stream, op->n, (const Scal *)op->sc, op->v32,
+                               op->w);
         LAUNCH_CHECK(op);
         ++starts;
-        if (!scope.exec && (rc = scope.capture([&] { return enqueue_iteration32(op); }))) return rc;
+        if (!scope.exec && (rc = scope.capture([&] {
+                return inner64 ? enqueue_iteration(op, op->dx, op->w, nullptr, nullptr, op->cs_active)
+                               : enqueue_iteration32(op);
+            })))
+            return rc;
         int inner_prev = 0;
         for (;;) {
             // a chunk of inner iterations, polling the inner state every 8.
@@ -1765,7 +1843,8 @@ int plsqr_lsqr_solve_device(plsqr_op *op, const double *b_dev, const plsqr_solve
     default_opts(opts, &o);
     int rc;
     if ((rc = b_internal(op, b_dev, &b_dev))) return rc;
-    if (o.precision == PLSQR_PREC_MIXED) {
+    if (o.precision == PLSQR_PREC_MIXED || (o.precision == PLSQR_PREC_FP64 && o.precond != PLSQR_PRECOND_NONE)) {
+        if (trace) return set_err(op, PLSQR_ERR_ARG, "no trace for the mixed or column-scaled solves");
         rc = solve_mixed(op, b_dev, &o, out);
     } else if (o.precision == PLSQR_PREC_FP64) {
         // the loop works on the handle's x and w (device column order)
@@ -1875,7 +1954,7 @@ int plsqr_lsqr_solve_x0_device(plsqr_op *op, const double *b_dev, const double *
         use = std::isfinite(rn2) && rn2 <= bn2;
     }
     if (used_x0) *used_x0 = use;
-    if (o.precision == PLSQR_PREC_MIXED) {
+    if (o.precision == PLSQR_PREC_MIXED || (o.precision == PLSQR_PREC_FP64 && o.precond != PLSQR_PRECOND_NONE)) {
         rc = solve_mixed(op, b_dev, &o, out, use ? op->x0i : nullptr);
     } else if (o.precision == PLSQR_PREC_FP64) {
         rc = solve_fp64(op, b_dev, &o, op->x, op->w, out, nullptr, use ? op->r0 : nullptr);

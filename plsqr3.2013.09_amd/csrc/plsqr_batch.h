@@ -774,6 +774,28 @@ __global__ void kb_count_live(int n_states, const Scal *__restrict__ sc, int *ou
     if (threadIdx.x == 0) *out = cnt;
 }
 
+// out[0] = states still live, out[1] = live states whose inner test2
+// estimate times ratio[i] (the FP64 / inner ratio measured at the last FP64
+// check, <= 0 when unknown) has reached tol: the adaptive check schedule of
+// the mixed solve checks as soon as one of them may have converged.
+__global__ void kb_count_pred(int n_states, const Scal *__restrict__ sc, const double *__restrict__ ratio, double tol,
+                              int *out) {
+    __shared__ int cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int live = 0, pred = 0;
+    for (int i = threadIdx.x; i < n_states; i += blockDim.x) {
+        if (sc[i].done) continue;
+        ++live;
+        const double r = ratio[i];
+        if (r > 0.0 && sc[i].test2 * r <= tol) ++pred;
+    }
+    atomicAdd(&cnt[0], live);
+    atomicAdd(&cnt[1], pred);
+    __syncthreads();
+    if (threadIdx.x < 2) out[threadIdx.x] = cnt[threadIdx.x];
+}
+
 // ---------------------------------------------------------------------------
 // Layout conversion between the caller's arrays (problem after problem, and
 // inside a problem right-hand side after right-hand side: b of length m_k, x

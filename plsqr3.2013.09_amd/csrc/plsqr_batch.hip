@@ -92,7 +92,8 @@ struct plsqr_batch {
     long long stage_len = 0;
     int part_len = 0;
     Scal *sc = nullptr, *sc_plain = nullptr;
-    int *live_d = nullptr, *live_h = nullptr;
+    int *live_d = nullptr, *live_h = nullptr;  // live states, and those predicted to have converged
+    double *ratio_d = nullptr;                  // S: FP64 / inner test2 at the last check (adaptive checks)
     void *hstage = nullptr;  // pinned: the states (S Scal) or the 2 S state sums on their way to the host
     // column scaling of the mixed solve: cs (cols) holds the scales during a
     // scaled solve and 1 otherwise (the FP32 combine always reads it);
@@ -492,7 +493,11 @@ void default_opts(const plsqr_solve_opts *o, plsqr_solve_opts *d) {
 
 // runs `count` graph launches, polling the live count every 8; returns the
 // number launched.  Records b->evs[first + i] after launch i.
-int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, int *launched) {
+// ratio_d (may be NULL): per state the FP64 / inner test2 ratio of the last
+// check; with it the chunk also ends once a live state's predicted test2
+// reaches tol.
+int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, int *launched,
+                 const double *ratio_d = nullptr, double tol = 0.0) {
     int rc;
     if ((rc = ensure_events(b, first + (size_t)count + 1))) return rc;
     *launched = 0;
@@ -501,12 +506,17 @@ int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, i
         BT_TRY(b, hipEventRecord(b->evs[first + (size_t)i], b->stream));
         ++*launched;
         if ((i + 1) % 8 == 0 && i + 1 < count) {
-            hipLaunchKernelGGL(kb_count_live, dim3(1), dim3(256), 0, b->stream, b->P * b->K, (const Scal *)b->sc,
-                               b->live_d);
+            if (ratio_d)
+                hipLaunchKernelGGL(kb_count_pred, dim3(1), dim3(256), 0, b->stream, b->P * b->K, (const Scal *)b->sc,
+                                   ratio_d, tol, b->live_d);
+            else
+                hipLaunchKernelGGL(kb_count_live, dim3(1), dim3(256), 0, b->stream, b->P * b->K, (const Scal *)b->sc,
+                                   b->live_d);
             BT_LAUNCH(b);
-            BT_TRY(b, hipMemcpyAsync(b->live_h, b->live_d, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+            BT_TRY(b, hipMemcpyAsync(b->live_h, b->live_d, sizeof(int) * 2, hipMemcpyDeviceToHost, b->stream));
             BT_TRY(b, hipStreamSynchronize(b->stream));
-            if (*b->live_h == 0) break;
+            if (b->live_h[0] == 0) break;
+            if (ratio_d && b->live_h[1] > 0) break;
         }
     }
     return PLSQR_OK;
@@ -691,6 +701,12 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
     const int itn = std::max(0, o->itn);
     const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int S = b->P * b->K;
+    // adaptive checks: after the first, a chunk runs up to 4 periods and ends
+    // once a live pair's inner test2 estimate times its last FP64 / inner
+    // ratio reaches tol
+    const bool adaptive = o->refine_adaptive != 0 && o->tol > 0.0;
+    bool have_ratio = false;
+    std::vector<double> ratio((size_t)S, -1.0);
     std::vector<Scal> h((size_t)S);
     std::vector<double> anorm_max((size_t)S, 0.0), rn((size_t)S, 0.0), an((size_t)S, 0.0), t2((size_t)S, INFINITY);
     std::vector<double> rn0((size_t)S, 0.0), sums;
@@ -751,9 +767,10 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
         if (!exec && (rc = get_graph<float>(b, 1, b->rv32, b->sval32, b->u32, b->v32, b->dx, &exec))) return rc;
         bool restart = false;
         while (!restart) {
-            const int k = std::min(period, itn - total);
+            const bool pred = adaptive && have_ratio;
+            const int k = std::min(pred ? 4 * period : period, itn - total);
             int launched = 0;
-            if ((rc = run_launches(b, exec, k, ev_next, &launched))) return rc;
+            if ((rc = run_launches(b, exec, k, ev_next, &launched, pred ? b->ratio_d : nullptr, o->tol))) return rc;
             ev_next += (size_t)launched;
             total += launched;
             if ((rc = read_states(b, h))) return rc;
@@ -776,6 +793,7 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
                 t2[(size_t)st] = (rn[(size_t)st] > 0.0 && anorm_t > 0.0)
                                      ? an[(size_t)st] / (anorm_t * rn[(size_t)st])
                                      : 0.0;
+                if (adaptive && q.test2 > 0.0 && std::isfinite(q.test2)) ratio[(size_t)st] = t2[(size_t)st] / q.test2;
                 const bool stuck = q.done && q.itn == 0;  // the inner run could not start
                 if (t2[(size_t)st] <= o->tol || total >= itn || stuck) {
                     odone[(size_t)st] = 1;
@@ -786,6 +804,10 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
                     ++live;
                     if (!q.done) all_inner_done = false;
                 }
+            }
+            if (adaptive) {
+                if ((rc = upload(b, b->ratio_d, ratio.data(), ratio.size()))) return rc;
+                have_ratio = true;
             }
             if (live == 0 || total >= itn) {
                 restart = true;
@@ -1128,9 +1150,11 @@ int build_batch(plsqr_batch *b, plsqr::BatchPlan &plan) {
         (rc = balloc(b, &b->w, (size_t)b->cols * K)) || (rc = balloc(b, &b->dx, (size_t)b->cols * K)) ||
         (rc = balloc(b, &b->part, (size_t)b->part_len)) || (rc = balloc(b, &b->sums, (size_t)S * 2)) ||
         (rc = balloc(b, &b->stage, (size_t)b->stage_len)) || (rc = balloc(b, &b->sc, (size_t)S)) ||
-        (rc = balloc(b, &b->sc_plain, (size_t)S)) || (rc = balloc(b, &b->live_d, 1)))
+        (rc = balloc(b, &b->sc_plain, (size_t)S)) || (rc = balloc(b, &b->live_d, 2)) ||
+        (rc = balloc(b, &b->ratio_d, (size_t)S)))
         return rc;
-    BT_TRY(b, hipHostMalloc((void **)&b->live_h, sizeof(int)));
+    BT_TRY(b, hipHostMalloc((void **)&b->live_h, sizeof(int) * 2));
+    b->live_h[0] = b->live_h[1] = 0;
     BT_TRY(b, hipHostMalloc(&b->hstage, std::max(sizeof(Scal), 2 * sizeof(double)) * (size_t)S));
     // states of a plain operator application: y*1*1 + A x, v*1 + A^T u
     Scal z;

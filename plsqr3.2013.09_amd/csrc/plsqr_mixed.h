@@ -305,7 +305,8 @@ __global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, 
                                                        const double *__restrict__ dav, const float *__restrict__ u,
                                                        const Scal *__restrict__ sc, float *__restrict__ v,
                                                        double *__restrict__ part,
-                                                       const double *__restrict__ cs = nullptr) {
+                                                       const double *__restrict__ cs = nullptr,
+                                                       const uint8_t *__restrict__ dlen = nullptr) {
     __shared__ double sh[BS / kWave];
     if (sc->done || !sc->beta_pos) return;
     const double ib = sc->ib;
@@ -316,7 +317,13 @@ __global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, 
         double tk = P[c];
         for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
         double td = 0.0;
-        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * ((double)ud[dri[k]] * ib);
+        if (dlen) {  // damping columns interleaved by groups of 64 (k_cols_combine)
+            const int len = dlen[c];
+            int q = dcp[c];
+            for (int k = 0; k < len; ++k, q += kWave) td += dav[q] * ((double)ud[dri[q]] * ib);
+        } else {
+            for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * ((double)ud[dri[k]] * ib);
+        }
         if (cs) {
             const double d = cs[c];
             const float vn = v[c] * vs + (float)(d * d * (tk + td));

@@ -504,6 +504,27 @@ __global__ void k_remap(long long len, const int *__restrict__ map, int *__restr
 // slice order and td the damping column sum in row order
 // (cpuSpMVTranspose_CSC_v2, CPUSpMV.c:281-291); x += (tk + td) groups as
 // aprod_impl.c:487-489.  part[b] = block b's sum of squares of the new v.
+// With column scales cs (LSQR on A D, D = diag(cs)): the solver keeps
+// v^ = D v, so v^ <- v^*vscale + cs^2 (tk + td) and part holds ||v^ / cs||^2.
+// The damping columns come as a CSC (dlen NULL: column c's entries at
+// dcp[c] .. dcp[c+1]-1) or interleaved by groups of 64 columns (dlen[c]
+// entries at dcp[c], dcp[c] + 64, ...: lane c % 64 of a wavefront reads
+// the address next to lane c % 64 - 1's), entries in the same order either
+// way.
+__device__ __forceinline__ double damp_col_sum(int c, const int *__restrict__ dcp, const uint8_t *__restrict__ dlen,
+                                               const int *__restrict__ dri, const double *__restrict__ dav,
+                                               const double *__restrict__ ud, double ib) {
+    double td = 0.0;
+    if (dlen) {
+        const int len = dlen[c];
+        int q = dcp[c];
+        for (int k = 0; k < len; ++k, q += kWave) td += dav[q] * (ud[dri[q]] * ib);
+    } else {
+        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * (ud[dri[k]] * ib);
+    }
+    return td;
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_cols_combine(int n, int m_k, int nsl,
                                                      const double *__restrict__ P,
@@ -513,7 +534,9 @@ __global__ __launch_bounds__(BS) void k_cols_combine(int n, int m_k, int nsl,
                                                      const double *__restrict__ u,
                                                      const Scal *__restrict__ sc,
                                                      double *__restrict__ v,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part,
+                                                     const double *__restrict__ cs = nullptr,
+                                                     const uint8_t *__restrict__ dlen = nullptr) {
     __shared__ double sh[BS / kWave];
     double ib = 1.0, vs = 1.0;
     const bool scaled = sc != nullptr;
@@ -527,13 +550,20 @@ __global__ __launch_bounds__(BS) void k_cols_combine(int n, int m_k, int nsl,
     for (int c = blockIdx.x * BS + threadIdx.x; c < n; c += gridDim.x * BS) {
         double tk = P[c];
         for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
-        double td = 0.0;
-        for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * (ud[dri[k]] * ib);
+        const double td = damp_col_sum(c, dcp, dlen, dri, dav, ud, ib);
         double vn = v[c];
         if (scaled) vn = vn * vs;
-        vn = vn + (tk + td);
-        v[c] = vn;
-        ssq += vn * vn;
+        if (cs) {
+            const double d = cs[c];
+            vn = vn + d * d * (tk + td);
+            v[c] = vn;
+            const double q = vn / d;
+            ssq += q * q;
+        } else {
+            vn = vn + (tk + td);
+            v[c] = vn;
+            ssq += vn * vn;
+        }
     }
     if (part) {
         const double a = block_sum<BS>(ssq, sh);

@@ -274,7 +274,7 @@ void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_star
     }
 }
 
-bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err) {
+bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err, int window) {
     const int TW = L.TW, G = RowTiles::G;
     const long long items = (long long)L.T * L.S;
     R->grp_base.assign((size_t)items + 1, 0);
@@ -301,9 +301,16 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     R->n_ent = item_ent[(size_t)items];
     R->ent_vox.assign((size_t)R->n_ent, 0);
     R->ent_val.assign((size_t)R->n_ent, 0.0);
+    // Position of every segment in its item's mode-1 order: the item's
+    // segments (ray order) cut into windows of `window`, each window sorted by
+    // descending length (stable), so that the 64 segments of a wavefront's
+    // group have similar lengths and its lanes finish together.  pos[k] is
+    // the mode-1 position of global segment k; the partial of the segment at
+    // position j of item it is written to seg_start[it] + j.
+    std::vector<int> pos((size_t)L.n_seg, 0);
     std::atomic<bool> ok{true};
     par_for(items, [&](long long ib, long long ie) {
-        std::vector<int> len;
+        std::vector<int> len, ord;
         std::vector<long long> off;
         for (long long it = ib; it < ie; ++it) {
             const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
@@ -311,23 +318,32 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
             const long long base = L.slot_base[(size_t)it];
             for (int v = 0; v < TW; ++v)
                 for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) ++len[L.sseg[(size_t)(base + (long long)j * TW + v)]];
-            off.assign((size_t)ns + 1, item_ent[(size_t)it]);
-            for (int i = 0; i < ns; ++i) {
-                if (len[(size_t)i] > 255) ok.store(false);
-                off[(size_t)i + 1] = off[(size_t)i] + len[(size_t)i];
+            ord.resize((size_t)ns);
+            for (int i = 0; i < ns; ++i) ord[(size_t)i] = i;
+            if (window > 1)
+                for (int w0 = 0; w0 < ns; w0 += window)
+                    std::stable_sort(ord.begin() + w0, ord.begin() + std::min(ns, w0 + window),
+                                     [&](int a, int b) { return len[(size_t)a] > len[(size_t)b]; });
+            int *pit = pos.data() + L.seg_start[(size_t)it];
+            for (int j = 0; j < ns; ++j) pit[ord[(size_t)j]] = j;
+            off.assign((size_t)ns + 1, item_ent[(size_t)it]);  // by position
+            for (int j = 0; j < ns; ++j) {
+                const int ln = len[(size_t)ord[(size_t)j]];
+                if (ln > 255) ok.store(false);
+                off[(size_t)j + 1] = off[(size_t)j] + ln;
             }
             const int g0 = R->grp_base[(size_t)it];
             for (int g = 0; g * G < ns; ++g) {
                 R->ent_base[(size_t)g0 + g] = off[(size_t)g * G];
                 for (int l = 0; l < G && g * G + l < ns; ++l)
-                    R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)(g * G + l)], 255);
+                    R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)ord[(size_t)(g * G + l)]], 255);
             }
             // voxels ascending: each segment's list comes out in voxel order
             for (int v = 0; v < TW; ++v)
                 for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) {
                     const long long slot = base + (long long)j * TW + v;
                     const int sg = L.sseg[(size_t)slot];
-                    const long long o = off[(size_t)sg]++;
+                    const long long o = off[(size_t)pit[sg]]++;
                     R->ent_vox[(size_t)o] = (uint8_t)v;
                     R->ent_val[(size_t)o] = L.sval[(size_t)slot];
                 }
@@ -343,7 +359,11 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     for (int r = 0; r < m_k; ++r) R->ray_ptr[(size_t)r + 1] += R->ray_ptr[(size_t)r];
     R->ray_seg.assign((size_t)L.n_seg, 0);
     std::vector<int> next(R->ray_ptr.begin(), R->ray_ptr.end() - 1);
-    for (long long k = 0; k < L.n_seg; ++k) R->ray_seg[(size_t)next[(size_t)L.seg_ray[(size_t)k]]++] = (int)k;
+    // per ray, its segments in tile order (items ascending), each named by its
+    // mode-1 partial's index
+    for (long long it = 0; it < items; ++it)
+        for (long long k = L.seg_start[(size_t)it]; k < L.seg_start[(size_t)it + 1]; ++k)
+            R->ray_seg[(size_t)next[(size_t)L.seg_ray[(size_t)k]]++] = (int)(L.seg_start[(size_t)it] + pos[(size_t)k]);
     return true;
 }
 
@@ -477,7 +497,7 @@ std::vector<uint64_t> end_keys(const plsqr_system *s, int nx, int ny, int nz) {
 }  // namespace
 
 void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
-               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *o) {
+               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *o, bool sort_damping) {
     const int n = s->n, mk = s->m_k;
     o->rperm.resize((size_t)mk);
     for (int i = 0; i < mk; ++i) o->rperm[(size_t)i] = i;
@@ -529,6 +549,59 @@ void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::v
     o->sys.owned = 0;
     o->sys.k_row_idx = o->ri.data();
     o->sys.k_val = o->kv.data();
+    if (!sort_damping) return;
+    // damping rows: stable sort by the new column of each row's first entry
+    const int md = s->m_d;
+    std::vector<int> dperm((size_t)md);
+    for (int i = 0; i < md; ++i) dperm[(size_t)i] = i;
+    if (md > 0 && (int)perm.size() == n) {
+        std::vector<int> key((size_t)md, INT_MAX);
+        for (int r = 0; r < md; ++r)
+            if (s->d_row_ptr[r] < s->d_row_ptr[r + 1]) key[(size_t)r] = perm[(size_t)s->d_col_idx[s->d_row_ptr[r]]];
+        std::stable_sort(dperm.begin(), dperm.end(), [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+    }
+    std::vector<int> dinv((size_t)md);
+    for (int i = 0; i < md; ++i) dinv[(size_t)dperm[(size_t)i]] = i;
+    const long long nd = s->nnz_d;
+    o->drp.assign((size_t)md + 1, 0);
+    o->dci.resize((size_t)std::max(nd, 1LL));
+    o->dv.resize((size_t)std::max(nd, 1LL));
+    for (int i = 0; i < md; ++i) {
+        const int r = dperm[(size_t)i];
+        const int b = s->d_row_ptr[r], e = s->d_row_ptr[r + 1];
+        o->drp[(size_t)i + 1] = o->drp[(size_t)i] + (e - b);
+        std::copy(s->d_col_idx + b, s->d_col_idx + e, o->dci.begin() + o->drp[(size_t)i]);
+        std::copy(s->d_val + b, s->d_val + e, o->dv.begin() + o->drp[(size_t)i]);
+    }
+    // damping CSC: the same columns, rows renamed and ascending again
+    o->dcr.resize((size_t)std::max(nd, 1LL));
+    o->dcv.resize((size_t)std::max(nd, 1LL));
+    par_for(n, [&](long long cb, long long ce) {
+        std::vector<std::pair<int, double>> tmp;
+        for (long long c = cb; c < ce; ++c) {
+            const int b = s->dc_col_ptr[c], e = s->dc_col_ptr[c + 1];
+            tmp.clear();
+            for (int k = b; k < e; ++k) tmp.emplace_back(dinv[(size_t)s->dc_row_idx[k]], s->dc_val[k]);
+            std::sort(tmp.begin(), tmp.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b2) {
+                return a.first < b2.first;
+            });
+            for (int k = b; k < e; ++k) {
+                o->dcr[(size_t)k] = tmp[(size_t)(k - b)].first;
+                o->dcv[(size_t)k] = tmp[(size_t)(k - b)].second;
+            }
+        }
+    });
+    o->sys.d_row_ptr = o->drp.data();
+    o->sys.d_col_idx = o->dci.data();
+    o->sys.d_val = o->dv.data();
+    o->sys.dc_row_idx = o->dcr.data();
+    o->sys.dc_val = o->dcv.data();
+    o->rperm.resize((size_t)mk + md);
+    o->rinv.resize((size_t)mk + md);
+    for (int i = 0; i < md; ++i) {
+        o->rperm[(size_t)mk + i] = mk + dperm[(size_t)i];
+        o->rinv[(size_t)mk + i] = mk + dinv[(size_t)i];
+    }
 }
 
 

@@ -93,9 +93,12 @@ struct RowTiles {
 };
 
 // Builds the mode-1 layout from a mode-2 layout (same items and segments).
+// Inside an item the segments are taken in windows of `window` (ray order
+// between windows), sorted by descending length inside each window so that a
+// wavefront's 64 segments have similar lengths (window <= 1: ray order).
 // Returns false with *err set when a segment holds more than 255 entries or
 // the group count overflows.
-bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err);
+bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err, int window = 1);
 
 // Host evaluation of y = A_k x (kernel rows, new column numbering) through
 // the mode-1 layout, adding every ray's partials in tile order; the check
@@ -119,14 +122,24 @@ void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_star
 // ray directions.  sys gets the renumbered kernel CSC (rows ascending in
 // every column); the damping arrays and b are the caller's.  rperm[new] =
 // old kernel row, rinv[old] = new.
+// The system with its kernel rows sorted (rperm/rinv over the first m_k
+// entries) and its damping rows sorted by the new position of the cell that
+// owns them (the row's first column), so that the damping rows a brick of
+// columns touches are neighbours in u (entries m_k.. of rperm/rinv, offset by
+// m_k).  rperm[new row] = old row, rinv[old row] = new row, over all m rows.
 struct RaySorted {
     plsqr_system sys;
     std::vector<int> ri, rperm, rinv;
     std::vector<double> kv;
+    std::vector<int> drp, dci, dcr;    // damping CSR (row_ptr, col_idx) and CSC row_idx in the new row order
+    std::vector<double> dv, dcv;       // their values
 };
 
+// sort_damping = false leaves the damping rows in place (rperm/rinv then
+// cover the m_k kernel rows only).
 void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::vector<int> &tile_start,
-               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *out);
+               const std::vector<int> &slice_bounds, const plsqr_grid *grid, RaySorted *out,
+               bool sort_damping = true);
 
 // Brick permutation for an nx*ny*nz grid (x fastest): bricks of bx*by*bz
 // voxels, voxels inside a brick x fastest.  Brick order: 1 = z fastest

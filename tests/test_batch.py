@@ -327,3 +327,33 @@ def test_batch_mixed_column_scaling(P, O, gpu_available):
                     assert e_s < 3 * e_64 + 1e-7, (p, e_s, e_64)
         print(f"problem {p}: iterations FP64 {[inf64[p][k]['itn'] for k in range(2)]}, scaled mixed "
               f"{[infs[p][k]['itn'] for k in range(2)]}", flush=True)
+
+
+@pytest.mark.gpu
+def test_batch_mixed_adaptive_checks(P, O, gpu_available):
+    """Adaptive FP64 checks in the batched mixed solve: every pair still stops
+    on its true FP64 test2 <= tol (checked here with scipy), with no more
+    checks than the fixed schedule and no more iterations."""
+    announce("batch mixed, adaptive checks")
+    import scipy.sparse as sp
+    systems, B = make_batch_inputs(P, SPECS[1:], 2)
+    with P.Batch(systems, 2, flags=P.BATCH_TUNE_MIXED) as bt:
+        X64, inf64, _ = bt.solve(B, 3000, tol=1e-6)
+        bt.set_anorm_refs([[inf64[p][k]["anorm"] for k in range(2)] for p in range(len(systems))])
+        kw = dict(tol=1e-6, precision=P.PREC_MIXED, refine_every=25, precond=P.PRECOND_COLSCALE,
+                  precond_power=1.5)
+        Xf, inff, tf = bt.solve(B, 3000, **kw)
+        Xa, infa, ta = bt.solve(B, 3000, refine_adaptive=True, **kw)
+    for p, s in enumerate(systems):
+        A = sp.vstack([sp.csc_matrix((s.k_val, s.k_row_idx, s.k_col_ptr), shape=(s.m_k, s.n)),
+                       sp.csr_matrix((s.d_val, s.d_col_idx, s.d_row_ptr), shape=(s.m_d, s.n))]).tocsr()
+        for k in range(2):
+            r = np.concatenate([B[p][k], np.zeros(s.m_d)]) - A @ Xa[p][k]
+            assert infa[p][k]["arnorm"] <= 1e-6
+            assert np.linalg.norm(A.T @ r) / (inf64[p][k]["anorm"] * np.linalg.norm(r)) <= 1e-6 * 1.001
+            assert infa[p][k]["refinements"] <= inff[p][k]["refinements"] + 1, (p, k)
+    it_f = max(i["itn"] for row in inff for i in row)
+    it_a = max(i["itn"] for row in infa for i in row)
+    print(f"slowest pair: {it_f} iterations with checks every 25, {it_a} adaptive; batch {tf:.1f} / {ta:.1f} ms",
+          flush=True)
+    assert it_a <= it_f

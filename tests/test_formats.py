@@ -176,12 +176,16 @@ def test_operator_rejects_bad_indices_before_touching_the_gpu(P):
     assert e.value.code in (2, 7)
 
 
-def test_tiled_layout_self_check_on_host(P, monkeypatch):
+@pytest.mark.parametrize("segwin", ["1", "64", "512"])
+def test_tiled_layout_self_check_on_host(P, monkeypatch, segwin):
     """The bricked mode-2 layout is built on the host before any device call;
     with PLSQR_TILE_CHECK=1 its per-slice column sums are compared with the
-    CSC's there.  Without a GPU the creation then stops at hipGetDevice, so a
+    CSC's there, and the mode-1 layout's ray sums (segments taken in ray
+    order, or sorted by length inside windows of 64 or 512) with the CSC's
+    row sums.  Without a GPU the creation then stops at hipGetDevice, so a
     HIP error (not a self-check error) means the layout passed."""
     monkeypatch.setenv("PLSQR_TILE_CHECK", "1")
+    monkeypatch.setenv("PLSQR_SEGWIN", segwin)
     for args in ((16, 16, 16, 20000, 0, 2013, 0.01, 0.1), (9, 7, 20, 3000, 1, 4, 0.02, 0.05)):
         s = P.generate(*args)
         for grid in ("auto", None):
@@ -262,3 +266,60 @@ def test_ker2plsqr3_files_load_as_the_reference_system(P, gpu_available, tmp_pat
     assert np.array_equal(t.k_val, s.k_val)
     assert (tmp_path / "matrix_bycol.mm2.bin").read_bytes() == (tmp_path / names["ker_f"]).read_bytes()
     assert (tmp_path / "matrix_bycol.mm2.info").read_text() == (tmp_path / names["ker_i"]).read_text()
+
+
+def test_damping_cross_rows_and_parameter_blocks(P):
+    """damping_binary.py's full stencil: per cell identity, xx, yy, zz, then
+    xy, xz, yz (0.5, -0.5, -0.5, 0.5 at the corners of a 3 x 3 square,
+    :291-479), for every parameter block (block p on columns p*V.., :93-95,
+    rows after block p-1's); the row counts of :50 and nonzeros of :61; the
+    column layout the transpose of the row layout."""
+    import scipy.sparse as sp
+    nx, ny, nz, npar = 5, 4, 3, 2
+    w = (0.3, 1.1, 1.2, 1.3, 2.1, 2.2, 2.3)
+    d = P.gen_damping(nx, ny, nz, n_param=npar, weights=w, cross=True)
+    V = nx * ny * nz
+    per = (V + (nx - 2) * ny * nz + nx * (ny - 2) * nz + nx * ny * (nz - 2) + (nx - 2) * (ny - 2) * nz
+           + (nx - 2) * ny * (nz - 2) + nx * (ny - 2) * (nz - 2))
+    nnz = (V + 3 * ((nx - 2) * ny * nz + nx * (ny - 2) * nz + nx * ny * (nz - 2))
+           + 4 * ((nx - 2) * (ny - 2) * nz + (nx - 2) * ny * (nz - 2) + nx * (ny - 2) * (nz - 2)))
+    assert (d.m_d, d.nnz_d, d.n) == (npar * per, npar * nnz, npar * V)
+    rp, ci, v = d.d_row_ptr, d.d_col_idx, d.d_val
+    for p in range(npar):
+        r = p * per  # cell (0,0,0) of block p: identity, xx, yy, zz, xy, xz, yz
+        o = p * V
+        rows = [(list(ci[rp[r + k]:rp[r + k + 1]] - o), list(v[rp[r + k]:rp[r + k + 1]])) for k in range(7)]
+        assert rows[0] == ([0], [0.3])
+        assert rows[1] == ([0, 1, 2], [1.1, -2.2, 1.1])
+        assert rows[2] == ([0, nx, 2 * nx], [1.2, -2.4, 1.2])
+        assert rows[3] == ([0, nx * ny, 2 * nx * ny], [1.3, -2.6, 1.3])
+        assert rows[4] == ([0, 2, 2 * nx, 2 * nx + 2], [0.5 * 2.1, -0.5 * 2.1, -0.5 * 2.1, 0.5 * 2.1])
+        assert rows[5] == ([0, 2, 2 * nx * ny, 2 * nx * ny + 2], [0.5 * 2.2, -0.5 * 2.2, -0.5 * 2.2, 0.5 * 2.2])
+        assert rows[6] == ([0, 2 * nx, 2 * nx * ny, 2 * nx * ny + 2 * nx],
+                           [0.5 * 2.3, -0.5 * 2.3, -0.5 * 2.3, 0.5 * 2.3])
+        # the last cell of the block has only its identity row
+        assert list(ci[rp[r + per - 1]:rp[r + per]]) == [o + V - 1]
+    R = sp.csr_matrix((v, ci, rp), shape=(d.m_d, d.n))
+    Cc = sp.csc_matrix((d.dc_val, d.dc_row_idx, d.dc_col_ptr), shape=(d.m_d, d.n))
+    assert (R != Cc).nnz == 0
+    for c in range(d.n):
+        rows_c = d.dc_row_idx[d.dc_col_ptr[c]:d.dc_col_ptr[c + 1]]
+        assert np.all(np.diff(rows_c) > 0)
+    # without cross rows and with one block: the generator's own damping
+    g = P.generate(nx, ny, nz, 4, 0, 1, 0.0, 0.3, w=1.1)
+    e = P.gen_damping(nx, ny, nz, n_param=1, weights=(0.3, 1.1, 1.1, 1.1, 0, 0, 0), cross=False)
+    for k in ("d_row_ptr", "d_col_idx", "d_val", "dc_col_ptr", "dc_row_idx", "dc_val"):
+        assert np.array_equal(getattr(e, k), getattr(g, k)), k
+
+
+def test_damping_files_round_trip(P, tmp_path):
+    """The four files plsqr_write_damping_files writes load back through the
+    reference loader (with a kernel) as the same matrix."""
+    s = P.generate(5, 4, 3, 30, 0, 3, 0.01, 0.1)
+    dn = P.write_damping_files(str(tmp_path), 5, 4, 3, n_param=1, weights=(0.1, 1, 1, 1, 0.5, 0.5, 0.5))
+    names = s.write_reference_files(str(tmp_path), prefix="k_")
+    d = P.gen_damping(5, 4, 3, n_param=1, weights=(0.1, 1, 1, 1, 0.5, 0.5, 0.5))
+    r = P.System.read_reference_files(str(tmp_path), dict(names, **dn), s.m_k, d.m_d, s.n)
+    for k in ("d_row_ptr", "d_col_idx", "d_val", "dc_col_ptr", "dc_row_idx", "dc_val"):
+        assert np.array_equal(getattr(r, k), getattr(d, k)), k
+    assert np.array_equal(r.k_val, s.k_val) and np.array_equal(r.b, s.b)

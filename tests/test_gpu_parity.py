@@ -439,10 +439,15 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
                            precond=P.PRECOND_COLSCALE, anorm_ref=i64["anorm"], refine_adaptive=True,
                            precond_power=1.5)
         xs_k2, _ = op.lsqr(s.b, k, tol=0.0, precision=P.PREC_MIXED, refine_every=k, precond=P.PRECOND_COLSCALE)
+        # the same scaled solve with an FP64 inner LSQR: textbook LSQR on A D in
+        # float64 to rounding for the first iterations, then the checked stop
+        x64s_k, _ = op.lsqr(s.b, k, tol=0.0, refine_every=k, precond=P.PRECOND_COLSCALE, precond_power=1.5)
+        x64s, i64s = op.lsqr(s.b, 2000, tol=1e-6, refine_every=25, precond=P.PRECOND_COLSCALE,
+                             anorm_ref=i64["anorm"], refine_adaptive=True, precond_power=1.5)
         with pytest.raises(P.PlsqrError):
             op.lsqr(s.b, 10, precision=P.PREC_MIXED, precond=P.PRECOND_COLSCALE, precond_power=-1.0)
         with pytest.raises(P.PlsqrError):
-            op.lsqr(s.b, 10, precond=P.PRECOND_COLSCALE)  # FP64 has no preconditioner
+            op.lsqr(s.b, 10, precond=P.PRECOND_COLSCALE, trace=True)  # no trace on the checked solve
     e_same, e_other = rel(xs_k, x_np_scaled), rel(xs_k, x_np_plain)
     print(f"{k} inner iterations: vs LSQR on A D {e_same:.2e}, vs LSQR on A {e_other:.2e}; "
           f"unscaled vs LSQR on A {rel(xp_k, x_np_plain):.2e}", flush=True)
@@ -455,8 +460,12 @@ def test_mixed_column_scaling(P, O, MG, gpu_available):
     assert e15 < 1e-3 and e15 < 0.05 * e15_other, (e15, e15_other)
     assert np.array_equal(xs, xs2)
     np.testing.assert_allclose(is_["anorm"], fro, rtol=1e-12)
+    e64 = rel(x64s_k, lsqr_scaled_np(A, bb, k, d15))
+    print(f"FP64 inner LSQR on A D^1.5: {e64:.2e} from textbook LSQR after {k} iterations; "
+          f"{i64s['itn']} iterations to tol against {i15['itn']} mixed", flush=True)
+    assert e64 < 1e-10, e64
     for x, info, an in ((xs, is_, fro), (xr, ir, i64["anorm"]), (xa, ia, i64["anorm"]), (xm, im, im["anorm"]),
-                        (x15, i15, i64["anorm"])):
+                        (x15, i15, i64["anorm"]), (x64s, i64s, i64["anorm"])):
         r = bb - A @ x
         assert info["arnorm"] <= 1e-6
         assert np.linalg.norm(A.T @ r) / (an * np.linalg.norm(r)) <= 1e-6 * 1.001
@@ -503,3 +512,37 @@ def test_mixed_warm_start_with_fewer_rows_than_columns(P, O, gpu_available):
             orc.aprod(mode, xo, yo)
             got, want = (yg, yo) if mode == 1 else (xg, xo)
             assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("knob,values", [("PLSQR_SEGWIN", ("1", "64", "512")), ("PLSQR_DAMP_IL", ("0", "1"))])
+def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, knob, values):
+    """Layout choices that keep every sum's order: the mode-1 layout may take
+    an item's segments sorted by length inside windows (PLSQR_SEGWIN; every
+    segment still summed in voxel order and every ray's partials in tile
+    order), and the damping columns of mode 2 may be stored interleaved by
+    groups of 64 (PLSQR_DAMP_IL; each column's entries in the same order).
+    aprod of both modes, FP64 and FP32, and a whole solve are bitwise the same
+    either way."""
+    announce(f"layout knob {knob}")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    res = {}
+    for val in values:
+        monkeypatch.setenv(knob, val)
+        with P.Operator(s) as op:
+            out = []
+            for mode in (1, 2):
+                xg, yg = x.copy(), y.copy()
+                op.aprod(mode, xg, yg)
+                out.append(yg if mode == 1 else xg)
+                x32, y32 = x.astype(np.float32), y.astype(np.float32)
+                op.aprod_f32(mode, x32, y32)
+                out.append(y32 if mode == 1 else x32)
+            out.append(op.lsqr(s.b, 50)[0])
+            out.append(op.lsqr(s.b, 400, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)[0])
+        res[val] = out
+    for val in values[1:]:
+        for a, b in zip(res[val], res[values[0]]):
+            assert np.array_equal(a, b), (knob, val)
