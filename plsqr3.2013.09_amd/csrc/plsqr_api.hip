@@ -72,6 +72,12 @@ struct plsqr_op {
     uint16_t *t_vlen = nullptr, *t_sseg = nullptr, *t_vcut = nullptr;
     double *t_sval = nullptr;
     long long t_nslot = 0, t_nseg = 0;
+    // slot form of the tiled mode-2 layout (SlotRows): 0 by voxel, 1 by
+    // position in packed rows, 2 by position in TW-wide rows
+    int t_form = 0;
+    uint8_t *t_pvox = nullptr;
+    long long *t_rbase = nullptr;
+    int *t_roff = nullptr;
     bool nt_loads = false;
     int tw = 128;  // voxels per tile (brick) of the tiled layouts: 128 or 256
     bool rows_nt = false;  // streaming loads for the mode-1 matrix data
@@ -343,18 +349,29 @@ int launch_cols(plsqr_op *op, const double *u, double *v, const Scal *sc, double
                 const double *cs = nullptr) {
     using namespace plsqr;
     if (op->tiled) {
-#define COLS(TW, B, CAP, NT)                                                                                          \
-    hipLaunchKernelGGL((k_cols_tiled<TW, B, CAP, NT>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n, op->tS,   \
-                       op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval, op->t_sseg, op->t_cut,  \
-                       op->t_vcut, u, sc, op->P);
+        const SlotRows sr{op->t_pvox, op->t_rbase, op->t_roff};
+#define COLS(TW, B, CAP, NT, FORM)                                                                                    \
+    hipLaunchKernelGGL((k_cols_tiled<TW, B, CAP, NT, FORM>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,     \
+                       op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval, op->t_sseg,     \
+                       op->t_cut, op->t_vcut, sr, u, sc, op->P);
         // 4 threads per voxel; LDS for 8192 (4096) gathered segments, so few
         // items need a second round
         if (op->tw == 256) {
-            COLS(256, 1024, 8192, false)
+            if (op->t_form == 1) {
+                COLS(256, 1024, 8192, false, 1)
+            } else if (op->t_form == 2) {
+                COLS(256, 1024, 8192, false, 2)
+            } else {
+                COLS(256, 1024, 8192, false, 0)
+            }
+        } else if (op->t_form == 1) {
+            COLS(TILE_W, 512, 4096, false, 1)
+        } else if (op->t_form == 2) {
+            COLS(TILE_W, 512, 4096, false, 2)
         } else if (op->nt_loads) {
-            COLS(TILE_W, 512, 4096, true)
+            COLS(TILE_W, 512, 4096, true, 0)
         } else {
-            COLS(TILE_W, 512, 4096, false)
+            COLS(TILE_W, 512, 4096, false, 0)
         }
 #undef COLS
         LAUNCH_CHECK(op);
@@ -709,15 +726,20 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         op->tT = L->T;
         op->nsl = L->S;
         op->tmax_seg = L->max_seg_per_item;
-        op->t_nslot = L->n_slot;
+        // the slot buffers end in TW zero slots: the packed-row loop of mode 2
+        // lets a lane past its length load (and drop) a slot up to TW - 1
+        // past its row's start, and the last row may end the array
+        op->t_nslot = L->n_slot + L->TW;
         op->t_nseg = L->n_seg;
         const long long items = (long long)L->T * L->S;
         if ((rc = dalloc(op, &op->t_start, (size_t)L->T + 1)) || (rc = dalloc(op, &op->t_slot, (size_t)items + 1)) ||
             (rc = dalloc(op, &op->t_seg, (size_t)items + 1)) || (rc = dalloc(op, &op->t_vlen, (size_t)items * L->TW)) ||
-            (rc = dalloc(op, &op->t_ray, (size_t)L->n_seg)) || (rc = dalloc(op, &op->t_sval, (size_t)L->n_slot)) ||
-            (rc = dalloc(op, &op->t_sseg, (size_t)L->n_slot)) || (rc = dalloc(op, &op->t_cut, (size_t)items + 1)) ||
+            (rc = dalloc(op, &op->t_ray, (size_t)L->n_seg)) || (rc = dalloc(op, &op->t_sval, (size_t)op->t_nslot)) ||
+            (rc = dalloc(op, &op->t_sseg, (size_t)op->t_nslot)) || (rc = dalloc(op, &op->t_cut, (size_t)items + 1)) ||
             (rc = dalloc(op, &op->t_vcut, L->vox_cut.size())))
             return rc;
+        HIP_TRY(op, hipMemset(op->t_sval + L->n_slot, 0, sizeof(double) * L->TW));
+        HIP_TRY(op, hipMemset(op->t_sseg + L->n_slot, 0, sizeof(uint16_t) * L->TW));
         HIP_TRY(op, hipMemcpy(op->t_cut, L->cut_base.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
         if (!L->vox_cut.empty())
             HIP_TRY(op, hipMemcpy(op->t_vcut, L->vox_cut.data(), sizeof(uint16_t) * L->vox_cut.size(),
@@ -731,6 +753,18 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
         if (L->n_slot > 0) {
             HIP_TRY(op, hipMemcpy(op->t_sval, L->sval.data(), sizeof(double) * L->n_slot, hipMemcpyHostToDevice));
             HIP_TRY(op, hipMemcpy(op->t_sseg, L->sseg.data(), sizeof(uint16_t) * L->n_slot, hipMemcpyHostToDevice));
+        }
+        if (L->compact) {
+            op->t_form = L->packed ? 1 : 2;
+            if ((rc = dalloc(op, &op->t_pvox, (size_t)items * L->TW))) return rc;
+            HIP_TRY(op, hipMemcpy(op->t_pvox, L->pos_vox.data(), (size_t)items * L->TW, hipMemcpyHostToDevice));
+        }
+        if (L->packed) {
+            const size_t nrow = L->row_off.size();
+            if ((rc = dalloc(op, &op->t_rbase, (size_t)items + 1)) || (rc = dalloc(op, &op->t_roff, std::max<size_t>(nrow, 1))))
+                return rc;
+            HIP_TRY(op, hipMemcpy(op->t_rbase, L->row_base.data(), sizeof(long long) * (items + 1), hipMemcpyHostToDevice));
+            if (nrow > 0) HIP_TRY(op, hipMemcpy(op->t_roff, L->row_off.data(), sizeof(int) * nrow, hipMemcpyHostToDevice));
         }
         if (R) {
             op->rtiled = true;
@@ -958,6 +992,14 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
                 return fail(set_err(op, PLSQR_ERR_STATE, why));
             }
         }
+        // mode 2 reads the slots by position: PLSQR_COMPACT=1 in packed rows,
+        // 2 in TW-wide rows, 0 by voxel
+        const int form = env_int("PLSQR_COMPACT", 1);
+        if (L && (form == 1 || form == 2)) {
+            plsqr::compact_tile_layout(L.get(), form == 1);
+            if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(sys, *L, &why))
+                return fail(set_err(op, PLSQR_ERR_STATE, why));
+        }
     }
     hipError_t e = hipGetDevice(&op->dev);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
@@ -1104,14 +1146,25 @@ static int ensure_fp32(plsqr_op *op);
 // FP32 mode 2 on the tiled layout (partials into op->P)
 static void launch_cols_tiled32(plsqr_op *op, const float *u, const Scal *sc) {
     using namespace plsqr;
-#define COLS32(TW, B, CAP)                                                                                            \
-    hipLaunchKernelGGL((k_cols_tiled32<TW, B, CAP>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,            \
+    const SlotRows sr{op->t_pvox, op->t_rbase, op->t_roff};
+#define COLS32(TW, B, CAP, FORM)                                                                                      \
+    hipLaunchKernelGGL((k_cols_tiled32<TW, B, CAP, FORM>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,      \
                        op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,   \
-                       op->t_cut, op->t_vcut, u, sc, op->P);
+                       op->t_cut, op->t_vcut, sr, u, sc, op->P);
     if (op->tw == 256) {
-        COLS32(256, 1024, 8192)
+        if (op->t_form == 1) {
+            COLS32(256, 1024, 8192, 1)
+        } else if (op->t_form == 2) {
+            COLS32(256, 1024, 8192, 2)
+        } else {
+            COLS32(256, 1024, 8192, 0)
+        }
+    } else if (op->t_form == 1) {
+        COLS32(TILE_W, 512, 4096, 1)
+    } else if (op->t_form == 2) {
+        COLS32(TILE_W, 512, 4096, 2)
     } else {
-        COLS32(TILE_W, 512, 4096)
+        COLS32(TILE_W, 512, 4096, 0)
     }
 #undef COLS32
 }

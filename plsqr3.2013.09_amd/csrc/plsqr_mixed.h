@@ -197,14 +197,16 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int
     }
 }
 
-// Mode 2 in FP32 on the tiled layout (same item structure as k_cols_tiled).
-template <int TW, int BS, int CAP>
+// Mode 2 in FP32 on the tiled layout (same item structure and slot forms as
+// k_cols_tiled).
+template <int TW, int BS, int CAP, int FORM>
 __global__ __launch_bounds__(BS) void k_cols_tiled32(
     int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const float *__restrict__ sval, const uint16_t *__restrict__ sseg,
-    const long long *__restrict__ cut_base, const uint16_t *__restrict__ vox_cut, const float *__restrict__ u,
-    const Scal *__restrict__ sc, double *__restrict__ P) {
+    const long long *__restrict__ cut_base, const uint16_t *__restrict__ vox_cut, SlotRows sr,
+    const float *__restrict__ u, const Scal *__restrict__ sc, double *__restrict__ P) {
+    static_assert(TW % kWave == 0, "the lanes of a wavefront share their row phase w");
     constexpr int NW = BS / TW;
     __shared__ float g[CAP];
     __shared__ double part[NW][TW];
@@ -217,10 +219,20 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     const int tw = tile_start[t + 1] - c0;
     const int v = threadIdx.x % TW;
     const int w = threadIdx.x / TW;
+    const int *ro = nullptr;
+    if constexpr (FORM == 1) ro = sr.row_off + sr.row_base[it];
     const long long base = slot_base[it];
     const long long s0 = seg_start[it];
     const int ns = (int)(seg_start[it + 1] - s0);
     const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
+    // packed rows: row starts of the wavefront's rows in its lanes, as in
+    // k_cols_tiled
+    int rs = 0, lenw = 0;
+    if constexpr (FORM == 1) {
+        lenw = __builtin_amdgcn_readfirstlane(len);
+        const int r = w + NW * (int)(threadIdx.x % kWave);
+        if (r < lenw) rs = ro[r];
+    }
     double acc = 0.0;
     int j = w;
     for (int r0 = 0; r0 < ns; r0 += CAP) {
@@ -248,26 +260,67 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
         if (ns <= CAP) {
             const float *sv = sval + base + v;
             const uint16_t *ss = sseg + base + v;
-            for (; j + 3 * NW < len; j += 4 * NW) {
-                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
-                const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
-                const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
-                dot32_add(acc, a0, g[g0]);
-                dot32_add(acc, a1, g[g1]);
-                dot32_add(acc, a2, g[g2]);
-                dot32_add(acc, a3, g[g3]);
-            }
-            for (; j < len; j += NW) {
-                const long long o = (long long)j * TW;
-                dot32_add(acc, sv[o], g[ss[o]]);
+            if constexpr (FORM == 1) {
+                const int wu = __builtin_amdgcn_readfirstlane(w);
+                const int kend = lenw > wu ? (lenw - wu + NW - 1) / NW : 0;
+                auto row = [&](int k) -> long long {
+                    return k < kWave ? (long long)__builtin_amdgcn_readlane(rs, k) : (long long)ro[wu + NW * k];
+                };
+                // loads not predicated (lanes past their length read slack
+                // slots and drop them), as in k_cols_tiled
+                int k = 0;
+                for (; k + 4 <= kend; k += 4) {
+                    const long long o0 = row(k), o1 = row(k + 1), o2 = row(k + 2), o3 = row(k + 3);
+                    const bool n0 = wu + NW * k < len, n1 = wu + NW * (k + 1) < len, n2 = wu + NW * (k + 2) < len,
+                               n3 = wu + NW * (k + 3) < len;
+                    const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
+                    const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                    const double p0 = (double)a0 * (double)g[n0 ? g0 : 0];
+                    const double p1 = (double)a1 * (double)g[n1 ? g1 : 0];
+                    const double p2 = (double)a2 * (double)g[n2 ? g2 : 0];
+                    const double p3 = (double)a3 * (double)g[n3 ? g3 : 0];
+                    acc = n0 ? acc + p0 : acc;
+                    acc = n1 ? acc + p1 : acc;
+                    acc = n2 ? acc + p2 : acc;
+                    acc = n3 ? acc + p3 : acc;
+                }
+                for (; k < kend; ++k) {
+                    const long long o = row(k);
+                    const bool on = wu + NW * k < len;
+                    const double pr = (double)sv[o] * (double)g[on ? (int)ss[o] : 0];
+                    acc = on ? acc + pr : acc;
+                }
+            } else {
+                for (; j + 3 * NW < len; j += 4 * NW) {
+                    const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                    const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
+                    const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
+                    dot32_add(acc, a0, g[g0]);
+                    dot32_add(acc, a1, g[g1]);
+                    dot32_add(acc, a2, g[g2]);
+                    dot32_add(acc, a3, g[g3]);
+                }
+                for (; j < len; j += NW) {
+                    const long long o = (long long)j * TW;
+                    dot32_add(acc, sv[o], g[ss[o]]);
+                }
             }
         } else {
             int jend = len;
             if (hi < ns && v < tw) jend = (int)vox_cut[(size_t)(cut_base[it] + hi / CUT - 1) * TW + v];
             const float *sv = sval + base + v;
             const uint16_t *ss = sseg + base + v;
+            auto roff = [&](int jj) -> long long {
+                if constexpr (FORM == 1) return ro[jj];
+                return (long long)jj * TW;
+            };
             for (; j + 3 * NW < jend; j += 4 * NW) {
-                const long long o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                long long o0, o1, o2, o3;
+                if constexpr (FORM == 1) {
+                    o0 = ro[j], o1 = ro[j + NW], o2 = ro[j + 2 * NW], o3 = ro[j + 3 * NW];
+                } else {
+                    o0 = (long long)j * TW, o1 = o0 + NW * TW, o2 = o1 + NW * TW, o3 = o2 + NW * TW;
+                }
                 const int g0 = ss[o0] - r0, g1 = ss[o1] - r0, g2 = ss[o2] - r0, g3 = ss[o3] - r0;
                 const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
                 dot32_add(acc, a0, g[g0]);
@@ -276,7 +329,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
                 dot32_add(acc, a3, g[g3]);
             }
             for (; j < jend; j += NW) {
-                const long long o = (long long)j * TW;
+                const long long o = roff(j);
                 dot32_add(acc, sv[o], g[ss[o] - r0]);
             }
         }
@@ -288,7 +341,9 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
         double tot = part[0][v];
 #pragma unroll
         for (int k = 1; k < NW; ++k) tot += part[k][v];
-        P[(size_t)s * n + c0 + v] = tot;
+        int vx = v;
+        if constexpr (FORM != 0) vx = sr.pos_vox[(size_t)it * TW + v];
+        P[(size_t)s * n + c0 + vx] = tot;
     }
 }
 

@@ -339,15 +339,28 @@ __global__ __launch_bounds__(BS) void k_cols_sliced(int n, int nsl, const int *_
 // Segments beyond the LDS capacity are processed in rounds of CAP.
 constexpr int CUT = 1024;  // segments per round cut of the tiled layout (TileLayout::CUT)
 
-template <int TW, int BS, int CAP, bool NT>
+// Row addressing of the tiled slots (FORM).  0: thread v is voxel v and row
+// j of an item starts at j * TW.  1 and 2: thread v is position v (voxels by
+// descending length, pos_vox[item * TW + v] = its voxel), so a row's active
+// threads are a prefix; with 1 row j starts at row_off[row_base[item] + j]
+// and holds only those (no padding in the slot stream), with 2 it starts at
+// j * TW.
+struct SlotRows {
+    const uint8_t *pos_vox;
+    const long long *row_base;
+    const int *row_off;
+};
+
+template <int TW, int BS, int CAP, bool NT, int FORM>
 __device__ __forceinline__ void cols_tiled_item(
     int it, int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
     const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
-    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, double ib,
+    const uint16_t *__restrict__ vox_cut, SlotRows sr, const double *__restrict__ u, double ib,
     double *__restrict__ P, double *g, double (*part)[TW]) {
     static_assert(CAP % CUT == 0, "rounds end on round cuts");
+    static_assert(TW % kWave == 0, "the lanes of a wavefront share their row phase w");
     constexpr int NW = BS / TW;
     constexpr int U = 4;  // slots per thread in flight (8 measured no faster, 2 slower)
     const int s = it % S;
@@ -356,10 +369,22 @@ __device__ __forceinline__ void cols_tiled_item(
     const int tw = tile_start[t + 1] - c0;
     const int v = threadIdx.x % TW;
     const int w = threadIdx.x / TW;
+    const int *ro = nullptr;
+    if constexpr (FORM == 1) ro = sr.row_off + sr.row_base[it];
     const long long base = slot_base[it];
     const long long s0 = seg_start[it];
     const int ns = (int)(seg_start[it + 1] - s0);
     const int len = v < tw ? (int)vox_len[(size_t)it * TW + v] : 0;
+    // Packed rows: the wavefront visits rows w, w + NW, ... up to its longest
+    // position (its first lane: positions descend in length); lane l loads
+    // the start of its l-th row here, before the segment gathers, so the slot
+    // loop takes row starts from registers instead of waiting on a load.
+    int rs = 0, lenw = 0;
+    if constexpr (FORM == 1) {
+        lenw = __builtin_amdgcn_readfirstlane(len);
+        const int r = w + NW * (int)(threadIdx.x % kWave);
+        if (r < lenw) rs = ro[r];
+    }
     double acc = 0.0;
     int j = w;
     if (ns <= CAP) {
@@ -389,20 +414,58 @@ __device__ __forceinline__ void cols_tiled_item(
         __syncthreads();
         const double *sv = sval + base + v;
         const uint16_t *ss = sseg + base + v;
-        for (; j + (U - 1) * NW < len; j += U * NW) {
-            const long long o = (long long)j * TW;
-            int gi[U];
-            double a[U];
+        if constexpr (FORM == 1) {
+            // a loop the whole wavefront runs together (k counts its rows);
+            // each lane adds its own rows, a prefix of them, in row order.
+            // The slot loads are not predicated, so all of a step's loads
+            // are in flight together: a lane past its length reads a slot
+            // of a later row (within TW slots of its row's start; the slot
+            // arrays carry TW slots of slack at the end) and drops it.
+            const int wu = __builtin_amdgcn_readfirstlane(w);
+            const int kend = lenw > wu ? (lenw - wu + NW - 1) / NW : 0;
+            auto row = [&](int k) -> long long {
+                return k < kWave ? (long long)__builtin_amdgcn_readlane(rs, k) : (long long)ro[wu + NW * k];
+            };
+            int k = 0;
+            for (; k + U <= kend; k += U) {
+                long long o[U];
 #pragma unroll
-            for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o + q * NW * TW);
+                for (int q = 0; q < U; ++q) o[q] = row(k + q);
+                int gi[U];
+                double a[U];
 #pragma unroll
-            for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o + q * NW * TW);
+                for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o[q]);
 #pragma unroll
-            for (int q = 0; q < U; ++q) acc += a[q] * g[gi[q]];
-        }
-        for (; j < len; j += NW) {
-            const long long o = (long long)j * TW;
-            acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o)];
+                for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o[q]);
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    const bool on = wu + NW * (k + q) < len;
+                    const double pr = a[q] * g[on ? gi[q] : 0];
+                    acc = on ? acc + pr : acc;
+                }
+            }
+            for (; k < kend; ++k) {
+                const long long o = row(k);
+                const bool on = wu + NW * k < len;
+                const double pr = ld_stream<NT>(sv + o) * g[on ? (int)ld_stream<NT>(ss + o) : 0];
+                acc = on ? acc + pr : acc;
+            }
+        } else {
+            for (; j + (U - 1) * NW < len; j += U * NW) {
+                const long long o = (long long)j * TW;
+                int gi[U];
+                double a[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o + q * NW * TW);
+#pragma unroll
+                for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o + q * NW * TW);
+#pragma unroll
+                for (int q = 0; q < U; ++q) acc += a[q] * g[gi[q]];
+            }
+            for (; j < len; j += NW) {
+                const long long o = (long long)j * TW;
+                acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o)];
+            }
         }
     } else {
         // rounds of CAP segments; the layout's round cuts give each voxel's
@@ -436,18 +499,25 @@ __device__ __forceinline__ void cols_tiled_item(
             const double *sv = sval + base + v;
             const uint16_t *ss = sseg + base + v;
             for (; j + (U - 1) * NW < jend; j += U * NW) {
-                const long long o = (long long)j * TW;
+                long long o[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    if constexpr (FORM == 1) o[q] = ro[j + q * NW];
+                    else o[q] = (long long)j * TW + q * NW * TW;
+                }
                 int gi[U];
                 double a[U];
 #pragma unroll
-                for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o + q * NW * TW) - r0;
+                for (int q = 0; q < U; ++q) gi[q] = ld_stream<NT>(ss + o[q]) - r0;
 #pragma unroll
-                for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o + q * NW * TW);
+                for (int q = 0; q < U; ++q) a[q] = ld_stream<NT>(sv + o[q]);
 #pragma unroll
                 for (int q = 0; q < U; ++q) acc += a[q] * g[gi[q]];
             }
             for (; j < jend; j += NW) {
-                const long long o = (long long)j * TW;
+                long long o;
+                if constexpr (FORM == 1) o = ro[j];
+                else o = (long long)j * TW;
                 acc += ld_stream<NT>(sv + o) * g[ld_stream<NT>(ss + o) - r0];
             }
             __syncthreads();
@@ -459,18 +529,20 @@ __device__ __forceinline__ void cols_tiled_item(
         double tot = part[0][v];
 #pragma unroll
         for (int k = 1; k < NW; ++k) tot += part[k][v];
-        P[(size_t)s * n + c0 + v] = tot;
+        int vx = v;
+        if constexpr (FORM != 0) vx = sr.pos_vox[(size_t)it * TW + v];
+        P[(size_t)s * n + c0 + vx] = tot;
     }
     __syncthreads();  // g and part are reused by the next item
 }
 
-template <int TW, int BS, int CAP, bool NT>
+template <int TW, int BS, int CAP, bool NT, int FORM>
 __global__ __launch_bounds__(BS) void k_cols_tiled(
     int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const double *__restrict__ sval,
     const uint16_t *__restrict__ sseg, const long long *__restrict__ cut_base,
-    const uint16_t *__restrict__ vox_cut, const double *__restrict__ u, const Scal *__restrict__ sc,
+    const uint16_t *__restrict__ vox_cut, SlotRows sr, const double *__restrict__ u, const Scal *__restrict__ sc,
     double *__restrict__ P) {
     __shared__ double g[CAP];
     __shared__ double part[BS / TW][TW];
@@ -479,8 +551,8 @@ __global__ __launch_bounds__(BS) void k_cols_tiled(
         if (sc->done || !sc->beta_pos) return;
         ib = sc->ib;
     }
-    cols_tiled_item<TW, BS, CAP, NT>(blockIdx.x, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
-                                     sval, sseg, cut_base, vox_cut, u, ib, P, g, part);
+    cols_tiled_item<TW, BS, CAP, NT, FORM>(blockIdx.x, n, S, tile_start, slot_base, seg_start, vox_len, seg_ray,
+                                          sval, sseg, cut_base, vox_cut, sr, u, ib, P, g, part);
 }
 
 // dst[i] = src[idx[i]]  (column renumbering at the API boundary)

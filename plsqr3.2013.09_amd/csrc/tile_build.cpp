@@ -186,28 +186,103 @@ bool tile_layout_apply(const TileLayout &L, int n, const double *u, double *P) {
                 const long long cb = L.cut_base[(size_t)it], ncut = L.cut_base[(size_t)it + 1] - cb;
                 g.resize((size_t)ns);
                 for (long long i = 0; i < ns; ++i) g[(size_t)i] = u[L.seg_ray[(size_t)(s0 + i)]];
-                for (int v = 0; v < tw; ++v) {
+                // p: voxel (padded form) or position (compact form)
+                for (int p = 0; p < tw; ++p) {
                     double acc = 0.0;
-                    const int len = L.vox_len[(size_t)it * TW + v];
+                    const int len = L.vox_len[(size_t)it * TW + p];
+                    const int v = L.compact ? L.pos_vox[(size_t)it * TW + p] : p;
+                    if (v >= tw) ok.store(false);
                     int j = 0;
                     for (long long q = 0; q <= ncut; ++q) {
                         const long long lo = q * TileLayout::CUT, hi = (q + 1) * TileLayout::CUT;
-                        const int jend = q < ncut ? L.vox_cut[(size_t)((cb + q) * TW + v)] : len;
+                        const int jend = q < ncut ? L.vox_cut[(size_t)((cb + q) * TW + p)] : len;
                         if (jend < j || jend > len) ok.store(false);
                         for (; j < jend && j < len; ++j) {
-                            const long long slot = L.slot_base[(size_t)it] + (long long)j * TW + v;
+                            const long long slot =
+                                L.slot_base[(size_t)it] +
+                                (L.packed ? (long long)L.row_off[(size_t)(L.row_base[(size_t)it] + j)] : (long long)j * TW) + p;
                             const int sg = L.sseg[(size_t)slot];
                             if (sg < lo || (q < ncut && sg >= hi)) ok.store(false);
                             acc += L.sval[(size_t)slot] * g[(size_t)sg];
                         }
                     }
                     if (j != len) ok.store(false);
-                    P[(size_t)sl * n + c0 + v] = acc;
+                    P[(size_t)sl * n + c0 + (v < tw ? v : 0)] = acc;
                 }
             }
         }
     });
     return ok.load();
+}
+
+void compact_tile_layout(TileLayout *L, bool pack) {
+    if (L->compact) return;
+    const int TW = L->TW;
+    const long long items = (long long)L->T * L->S;
+    std::vector<uint8_t> pos_vox((size_t)items * TW, 0);
+    std::vector<uint16_t> plen((size_t)items * TW, 0), pcut(L->vox_cut.size(), 0);
+    std::vector<long long> nrow((size_t)items + 1, 0), nslot((size_t)items + 1, 0);
+    // positions: voxels by descending length (ties: ascending voxel)
+    par_for(items, [&](long long ib, long long ie) {
+        std::vector<int> ord(TW);
+        for (long long it = ib; it < ie; ++it) {
+            const uint16_t *len = L->vox_len.data() + (size_t)it * TW;
+            for (int v = 0; v < TW; ++v) ord[(size_t)v] = v;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len[a] > len[b]; });
+            long long tot = 0;
+            for (int p = 0; p < TW; ++p) {
+                const int v = ord[(size_t)p];
+                pos_vox[(size_t)it * TW + p] = (uint8_t)v;
+                plen[(size_t)it * TW + p] = len[v];
+                tot += len[v];
+            }
+            for (long long q = L->cut_base[(size_t)it]; q < L->cut_base[(size_t)it + 1]; ++q)
+                for (int p = 0; p < TW; ++p) pcut[(size_t)(q * TW + p)] = L->vox_cut[(size_t)(q * TW + ord[(size_t)p])];
+            nrow[(size_t)it + 1] = pack ? len[ord[0]] : 0;
+            nslot[(size_t)it + 1] = pack ? tot : (long long)len[ord[0]] * TW;
+        }
+    });
+    for (long long it = 0; it < items; ++it) {
+        nrow[(size_t)it + 1] += nrow[(size_t)it];
+        nslot[(size_t)it + 1] += nslot[(size_t)it];
+    }
+    std::vector<int> row_off((size_t)nrow[(size_t)items], 0);
+    std::vector<double> sval((size_t)nslot[(size_t)items], 0.0);
+    std::vector<uint16_t> sseg((size_t)nslot[(size_t)items], 0);
+    par_for(items, [&](long long ib, long long ie) {
+        for (long long it = ib; it < ie; ++it) {
+            const uint16_t *len = plen.data() + (size_t)it * TW;
+            const int rows = (int)(nrow[(size_t)it + 1] - nrow[(size_t)it]);
+            int *ro = row_off.data() + nrow[(size_t)it];
+            // row j holds positions 0 .. c_j - 1, c_j = #{p : len[p] > j}
+            int c = TW, off = 0;
+            for (int j = 0; j < rows; ++j) {
+                while (c > 0 && len[c - 1] <= j) --c;
+                ro[j] = off;
+                off += c;
+            }
+            const long long ob = L->slot_base[(size_t)it], nb = nslot[(size_t)it];
+            for (int p = 0; p < TW; ++p) {
+                const int v = pos_vox[(size_t)it * TW + p];
+                for (int j = 0; j < len[p]; ++j) {
+                    const long long src = ob + (long long)j * TW + v, dst = nb + (pack ? ro[j] : (long long)j * TW) + p;
+                    sval[(size_t)dst] = L->sval[(size_t)src];
+                    sseg[(size_t)dst] = L->sseg[(size_t)src];
+                }
+            }
+        }
+    });
+    L->pos_vox.swap(pos_vox);
+    L->vox_len.swap(plen);
+    L->vox_cut.swap(pcut);
+    L->row_base.swap(nrow);
+    L->row_off.swap(row_off);
+    L->slot_base.swap(nslot);
+    L->sval.swap(sval);
+    L->sseg.swap(sseg);
+    L->n_slot = L->slot_base[(size_t)items];
+    L->compact = true;
+    L->packed = pack;
 }
 
 bool tile_layout_self_check(const plsqr_system *s, const TileLayout &L, std::string *err) {
@@ -277,6 +352,10 @@ void order_tile_columns(const plsqr_system *s, const std::vector<int> &tile_star
 bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err, int window) {
     const int TW = L.TW, G = RowTiles::G;
     const long long items = (long long)L.T * L.S;
+    if (L.compact) {
+        *err = "the mode-1 layout is built from the padded slots";
+        return false;
+    }
     R->grp_base.assign((size_t)items + 1, 0);
     long long gtot = 0;
     for (long long it = 0; it < items; ++it) {

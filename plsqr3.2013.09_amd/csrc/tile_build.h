@@ -66,7 +66,29 @@ struct TileLayout {
     std::vector<uint16_t> vox_cut;     // cut_base[items] * TW
     long long n_seg = 0, n_slot = 0;
     int max_seg_per_item = 0;
+    // Slots by position (compact_tile_layout): inside every item the voxels
+    // are taken by descending vox_len ("positions"; pos_vox[item * TW + p] =
+    // the voxel at position p), so the positions still active in row j are
+    // a prefix of the row.  Rows are either kept TW wide (slot (j, p) =
+    // slot_base[item] + j * TW + p) or packed: row j then holds only its
+    // active positions, from row_off[row_base[item] + j] (relative to
+    // slot_base[item]), and the slots carry no padding.  vox_len and vox_cut
+    // are indexed by position in both.
+    bool compact = false;
+    bool packed = false;
+    std::vector<uint8_t> pos_vox;      // items * TW
+    std::vector<long long> row_base;   // items+1 (packed rows)
+    std::vector<int> row_off;          // row_base[items] (packed rows)
 };
+
+// Rewrites a layout's slots by position, with packed rows if `pack` (no-op
+// if already done).  In the default form each voxel's j-th nonzero sits at
+// a fixed stride, so a row of 64 voxels of which only a few are still
+// active costs whole cache lines; by position the active lanes of a row are
+// a prefix, and packed rows make each item one contiguous stream.  The
+// segments, the sums and their order are unchanged: every voxel still adds
+// its slots in ascending j.
+void compact_tile_layout(TileLayout *L, bool pack = true);
 
 // Mode-1 layout on the same (tile, row slice) items.  Inside an item every
 // segment (one ray's entries in the tile) is a list of (voxel, value) pairs
@@ -96,8 +118,9 @@ struct RowTiles {
 // Inside an item the segments are taken in windows of `window` (ray order
 // between windows), sorted by descending length inside each window so that a
 // wavefront's 64 segments have similar lengths (window <= 1: ray order).
-// Returns false with *err set when a segment holds more than 255 entries or
-// the group count overflows.
+// Returns false with *err set when a segment holds more than 255 entries,
+// the group count overflows or the layout is already compacted (build the
+// mode-1 layout first, then compact_tile_layout).
 bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err, int window = 1);
 
 // Host evaluation of y = A_k x (kernel rows, new column numbering) through
@@ -162,7 +185,7 @@ bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
 // layout, used to check the layout against the CSC before it is uploaded.
 // The slots are visited round by round (CUT segments per round) through the
 // round cuts, as the batch kernel does; returns false if a cut disagrees with
-// the segments of the slots it bounds.
+// the segments of the slots it bounds.  Reads either slot form.
 bool tile_layout_apply(const TileLayout &L, int n, const double *u, double *P);
 
 // Compares tile_layout_apply with the CSC of s on a fixed test vector (the

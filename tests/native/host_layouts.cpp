@@ -125,9 +125,32 @@ static void layouts(const plsqr_system *s, const plsqr_grid *grid, int nsl, int 
     CHECK(ok, "build_row_tiles: %s", why.c_str());
     if (!ok) return;
     CHECK(plsqr::row_tiles_self_check(sys, L, R, &why), "row_tiles_self_check: %s", why.c_str());
+    // compacted slot rows: the same sums, fewer slots, and the mode-1 builder refuses them
+    const long long padded = L.n_slot;
+    std::vector<double> u((size_t)s->m_k), P0((size_t)L.S * s->n), P1((size_t)L.S * s->n);
+    for (int r = 0; r < s->m_k; ++r) u[(size_t)r] = std::cos(0.21 * r + 0.3);
+    CHECK(plsqr::tile_layout_apply(L, s->n, u.data(), P0.data()), "padded apply");
+    {
+        // by position in TW-wide rows
+        plsqr::TileLayout W = L;
+        plsqr::compact_tile_layout(&W, false);
+        CHECK(W.n_slot == padded && !W.packed, "position rows changed the slot count");
+        CHECK(plsqr::tile_layout_apply(W, s->n, u.data(), P1.data()), "position-row apply: round cuts disagree");
+        CHECK(P0 == P1, "slots by position give different sums");
+    }
+    plsqr::compact_tile_layout(&L);
+    CHECK(plsqr::tile_layout_apply(L, s->n, u.data(), P1.data()), "compact apply: round cuts disagree");
+    CHECK(P0 == P1, "compacted slots give different sums");
+    CHECK(plsqr::tile_layout_self_check(sys, L, &why), "compact self check: %s", why.c_str());
+    long long nnz_slots = 0;
+    for (uint16_t l : L.vox_len) nnz_slots += l;
+    CHECK(L.n_slot == nnz_slots && nnz_slots == sys->nnz_k, "compact form holds %lld slots for %lld nonzeros",
+          L.n_slot, sys->nnz_k);
+    plsqr::RowTiles R2;
+    CHECK(!plsqr::build_row_tiles(L, sys->m_k, &R2, &why), "mode-1 layout built from compacted slots");
     std::printf("layout n=%d m_k=%d slices=%d window=%d grid=%d: %lld segments (at most %d per item), %lld slots, "
-                "%lld round cuts, %d groups\n", s->n, s->m_k, L.S, window, grid ? 1 : 0, L.n_seg, L.max_seg_per_item,
-                L.n_slot, cuts, R.n_grp);
+                "%lld round cuts, %d groups; compacted %lld -> %lld slots\n", s->n, s->m_k, L.S, window, grid ? 1 : 0,
+                L.n_seg, L.max_seg_per_item, padded, cuts, R.n_grp, padded, L.n_slot);
 }
 
 static void files(const plsqr_system *s, const std::string &dir) {

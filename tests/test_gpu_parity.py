@@ -514,16 +514,24 @@ def test_mixed_warm_start_with_fewer_rows_than_columns(P, O, gpu_available):
             assert rel(got, want) < 1e-13
 
 
-@pytest.mark.parametrize("knob,values", [("PLSQR_SEGWIN", ("1", "64", "512")), ("PLSQR_DAMP_IL", ("0", "1"))])
-def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, knob, values):
+@pytest.mark.parametrize("knob,values,grid", [("PLSQR_SEGWIN", ("1", "64", "512"), "auto"),
+                                               ("PLSQR_DAMP_IL", ("0", "1"), "auto"),
+                                               ("PLSQR_COMPACT", ("0", "1", "2"), "auto"),
+                                               ("PLSQR_COMPACT", ("0", "1", "2"), None)])
+def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, knob, values, grid):
     """Layout choices that keep every sum's order: the mode-1 layout may take
     an item's segments sorted by length inside windows (PLSQR_SEGWIN; every
     segment still summed in voxel order and every ray's partials in tile
-    order), and the damping columns of mode 2 may be stored interleaved by
-    groups of 64 (PLSQR_DAMP_IL; each column's entries in the same order).
-    aprod of both modes, FP64 and FP32, and a whole solve are bitwise the same
-    either way."""
-    announce(f"layout knob {knob}")
+    order), the damping columns of mode 2 may be stored interleaved by
+    groups of 64 (PLSQR_DAMP_IL; each column's entries in the same order),
+    and mode 2's slots may be stored by voxel or by position, in packed or
+    TW-wide rows (PLSQR_COMPACT 0, 1, 2; every voxel still adds its slots in
+    ascending order), on 4 x 8 x 8 bricks
+    of the voxel grid ("auto": the system's grid) or on tiles of 128
+    consecutive columns (no grid).  aprod of
+    both modes, FP64 and FP32, and whole solves (FP64; mixed; mixed with
+    column scaling) are bitwise the same either way."""
+    announce(f"layout knob {knob}, grid {grid}")
     s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
     rng = np.random.default_rng(23)
     x = rng.standard_normal(s.n)
@@ -531,7 +539,7 @@ def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, kno
     res = {}
     for val in values:
         monkeypatch.setenv(knob, val)
-        with P.Operator(s) as op:
+        with P.Operator(s, grid=grid) as op:
             out = []
             for mode in (1, 2):
                 xg, yg = x.copy(), y.copy()
@@ -542,6 +550,8 @@ def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, kno
                 out.append(y32 if mode == 1 else x32)
             out.append(op.lsqr(s.b, 50)[0])
             out.append(op.lsqr(s.b, 400, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)[0])
+            out.append(op.lsqr(s.b, 400, tol=1e-6, precision=P.PREC_MIXED, refine_every=25,
+                               precond=P.PRECOND_COLSCALE)[0])
         res[val] = out
     for val in values[1:]:
         for a, b in zip(res[val], res[values[0]]):
