@@ -495,9 +495,11 @@ void default_opts(const plsqr_solve_opts *o, plsqr_solve_opts *d) {
 // number launched.  Records b->evs[first + i] after launch i.
 // ratio_d (may be NULL): per state the FP64 / inner test2 ratio of the last
 // check; with it the chunk also ends once a live state's predicted test2
-// reaches tol.
+// reaches tol, but not before min_pred launches: a check costs the whole
+// batch, and the pairs of a batch reach tol at different times, so checks
+// stay at most as frequent as the fixed schedule.
 int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, int *launched,
-                 const double *ratio_d = nullptr, double tol = 0.0) {
+                 const double *ratio_d = nullptr, double tol = 0.0, int min_pred = 0) {
     int rc;
     if ((rc = ensure_events(b, first + (size_t)count + 1))) return rc;
     *launched = 0;
@@ -516,7 +518,7 @@ int run_launches(plsqr_batch *b, hipGraphExec_t exec, int count, size_t first, i
             BT_TRY(b, hipMemcpyAsync(b->live_h, b->live_d, sizeof(int) * 2, hipMemcpyDeviceToHost, b->stream));
             BT_TRY(b, hipStreamSynchronize(b->stream));
             if (b->live_h[0] == 0) break;
-            if (ratio_d && b->live_h[1] > 0) break;
+            if (ratio_d && b->live_h[1] > 0 && i + 1 >= min_pred) break;
         }
     }
     return PLSQR_OK;
@@ -701,9 +703,9 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
     const int itn = std::max(0, o->itn);
     const int period = o->refine_every > 0 ? o->refine_every : std::max(itn, 1);
     const int S = b->P * b->K;
-    // adaptive checks: after the first, a chunk runs up to 4 periods and ends
-    // once a live pair's inner test2 estimate times its last FP64 / inner
-    // ratio reaches tol
+    // adaptive checks: after the first, a chunk runs from 1 to 4 periods and
+    // ends (after the first period) once a live pair's inner test2 estimate
+    // times its last FP64 / inner ratio reaches tol
     const bool adaptive = o->refine_adaptive != 0 && o->tol > 0.0;
     bool have_ratio = false;
     std::vector<double> ratio((size_t)S, -1.0);
@@ -770,7 +772,8 @@ int solve_mixed(plsqr_batch *b, const double *b_dev, const plsqr_solve_opts *o, 
             const bool pred = adaptive && have_ratio;
             const int k = std::min(pred ? 4 * period : period, itn - total);
             int launched = 0;
-            if ((rc = run_launches(b, exec, k, ev_next, &launched, pred ? b->ratio_d : nullptr, o->tol))) return rc;
+            if ((rc = run_launches(b, exec, k, ev_next, &launched, pred ? b->ratio_d : nullptr, o->tol, period)))
+                return rc;
             ev_next += (size_t)launched;
             total += launched;
             if ((rc = read_states(b, h))) return rc;
