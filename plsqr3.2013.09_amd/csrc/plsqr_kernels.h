@@ -100,10 +100,13 @@ __device__ double reduce_partials(const double *p, int count, int stride, double
 
 // FP32 dot products with FP64 accumulation: the product of two FP32 values
 // is exact in FP64 (24 + 24 significant bits <= 53), so acc += (double)a *
-// (double)x rounds once per term, in FP64 (the build keeps the multiply and
-// the add separate).  Each FP32 dot product is thus as accurate as if it were
-// computed in twice the working precision, at two FP64 operations per term.
-__device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc += (double)a * (double)x; }
+// (double)x rounds once per term, in FP64.  Each FP32 dot product is thus as
+// accurate as if it were computed in twice the working precision.  Since the
+// product is exact, the fused multiply-add rounds exactly as the separate
+// multiply and add would: one FP64 operation per term, the same bits.
+__device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc = fma((double)a, (double)x, acc); }
+// the same with x already widened (an FP32 value held in FP64)
+__device__ __forceinline__ void dot32_add(double &acc, float a, double x) { acc = fma((double)a, x, acc); }
 
 // Copies the cnt <= CH (value, voxel) entries at global index base into a
 // wavefront's LDS buffers with 16-byte loads (two FP64 or four FP32 values,

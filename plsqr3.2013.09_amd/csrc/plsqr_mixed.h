@@ -83,7 +83,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
     static_assert(BS >= TW, "one thread per voxel loads the tile of x");
-    __shared__ float xl[TW];
+    __shared__ double xl[TW];  // the tile of x, widened once per item
     __shared__ __align__(16) float lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
     if (sc->done) return;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     if (g0 == g1) return;  // the same for the whole workgroup
     const int t = it / S;
     const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
-    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
+    if (threadIdx.x < tw) xl[threadIdx.x] = (double)x[c0 + threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x % G;
     const int wv = threadIdx.x / G;
@@ -119,24 +119,24 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int e0 = beg > q0 ? beg : q0;
             const int e1 = end < q0 + cnt ? end : q0 + cnt;
-            // four, then two entries per step: the exact products formed before
-            // the adds (in entry order), so several LDS gathers are in flight
+            // four, then two entries per step: the operands of a step read
+            // first (several LDS gathers in flight), then the exact products
+            // added in entry order
             int e = e0;
             for (; e + 3 < e1; e += 4) {
-                const double p0 = (double)bv[sh + e] * (double)xl[bx[sh + e]];
-                const double p1 = (double)bv[sh + e + 1] * (double)xl[bx[sh + e + 1]];
-                const double p2 = (double)bv[sh + e + 2] * (double)xl[bx[sh + e + 2]];
-                const double p3 = (double)bv[sh + e + 3] * (double)xl[bx[sh + e + 3]];
-                acc += p0;
-                acc += p1;
-                acc += p2;
-                acc += p3;
+                const float a0 = bv[sh + e], a1 = bv[sh + e + 1], a2 = bv[sh + e + 2], a3 = bv[sh + e + 3];
+                const double x0 = xl[bx[sh + e]], x1 = xl[bx[sh + e + 1]], x2 = xl[bx[sh + e + 2]],
+                             x3 = xl[bx[sh + e + 3]];
+                dot32_add(acc, a0, x0);
+                dot32_add(acc, a1, x1);
+                dot32_add(acc, a2, x2);
+                dot32_add(acc, a3, x3);
             }
             for (; e + 1 < e1; e += 2) {
-                const double p0 = (double)bv[sh + e] * (double)xl[bx[sh + e]];
-                const double p1 = (double)bv[sh + e + 1] * (double)xl[bx[sh + e + 1]];
-                acc += p0;
-                acc += p1;
+                const float a0 = bv[sh + e], a1 = bv[sh + e + 1];
+                const double x0 = xl[bx[sh + e]], x1 = xl[bx[sh + e + 1]];
+                dot32_add(acc, a0, x0);
+                dot32_add(acc, a1, x1);
             }
             if (e < e1) dot32_add(acc, bv[sh + e], xl[bx[sh + e]]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     const float *__restrict__ u, const Scal *__restrict__ sc, double *__restrict__ P) {
     static_assert(TW % kWave == 0, "the lanes of a wavefront share their row phase w");
     constexpr int NW = BS / TW;
-    __shared__ float g[CAP];
+    __shared__ double g[CAP];  // fl32(u * ib) per segment, held widened
     __shared__ double part[NW][TW];
     if (sc->done || !sc->beta_pos) return;
     const float ib = (float)sc->ib;
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int i = threadIdx.x + q * BS;
-            if (i < cnt) g[i] = gv[q] * ib;
+            if (i < cnt) g[i] = (double)(gv[q] * ib);
         }
         __syncthreads();
         const int hi = r0 + cnt;
@@ -275,20 +275,20 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
                                n3 = wu + NW * (k + 3) < len;
                     const int g0 = ss[o0], g1 = ss[o1], g2 = ss[o2], g3 = ss[o3];
                     const float a0 = sv[o0], a1 = sv[o1], a2 = sv[o2], a3 = sv[o3];
-                    const double p0 = (double)a0 * (double)g[n0 ? g0 : 0];
-                    const double p1 = (double)a1 * (double)g[n1 ? g1 : 0];
-                    const double p2 = (double)a2 * (double)g[n2 ? g2 : 0];
-                    const double p3 = (double)a3 * (double)g[n3 ? g3 : 0];
-                    acc = n0 ? acc + p0 : acc;
-                    acc = n1 ? acc + p1 : acc;
-                    acc = n2 ? acc + p2 : acc;
-                    acc = n3 ? acc + p3 : acc;
+                    const double s0 = fma((double)a0, g[n0 ? g0 : 0], acc);
+                    acc = n0 ? s0 : acc;
+                    const double s1 = fma((double)a1, g[n1 ? g1 : 0], acc);
+                    acc = n1 ? s1 : acc;
+                    const double s2 = fma((double)a2, g[n2 ? g2 : 0], acc);
+                    acc = n2 ? s2 : acc;
+                    const double s3 = fma((double)a3, g[n3 ? g3 : 0], acc);
+                    acc = n3 ? s3 : acc;
                 }
                 for (; k < kend; ++k) {
                     const long long o = row(k);
                     const bool on = wu + NW * k < len;
-                    const double pr = (double)sv[o] * (double)g[on ? (int)ss[o] : 0];
-                    acc = on ? acc + pr : acc;
+                    const double s0 = fma((double)sv[o], g[on ? (int)ss[o] : 0], acc);
+                    acc = on ? s0 : acc;
                 }
             } else {
                 for (; j + 3 * NW < len; j += 4 * NW) {
