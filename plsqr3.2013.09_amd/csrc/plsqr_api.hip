@@ -86,6 +86,15 @@ struct plsqr_op {
     long long rt_nent = 0;
     int rt_lc = 4;
     int *rt_gbase = nullptr, *rt_rptr = nullptr, *rt_rseg = nullptr;
+    // the mode-1 layout's own tiles (stacks of two bricks, or the mode-2 tiles)
+    int rt_tw = 256, rtT = 0;
+    int *rt_start = nullptr;
+    long long *rt_seg = nullptr;
+    uint8_t *rt_lo = nullptr;  // lower-brick entries per segment (512-voxel tiles)
+    // pass 1 over chunks of at most rt_chunk groups of one item (0: one
+    // workgroup per item); chunk c = 8k + s is the k-th of row slice s
+    int rt_chunk = 0, rt_nch = 0;
+    int *rt_chitem = nullptr, *rt_chg0 = nullptr;
     long long *rt_ebase = nullptr;
     uint8_t *rt_len = nullptr, *rt_vox = nullptr;
     double *rt_val = nullptr;
@@ -270,15 +279,19 @@ constexpr int BS = 256;
 template <class T>
 void launch_pass1(plsqr_op *op, const T *val, const T *x, const Scal *sc, T *segv) {
     using namespace plsqr;
-    const dim3 g(op->tT * op->tS), b(BS);
+    const dim3 g(op->rt_chunk ? op->rt_nch : op->rtT * op->tS), b(BS);
 #define PASS1(CH, TW)                                                                                                 \
     if constexpr (std::is_same<T, double>::value)                                                                     \
-        hipLaunchKernelGGL((k_rows_tiled_c<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,         \
-                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);                     \
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,       \
+                           op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
+                           op->rt_lo, op->rt_vox, val, x, sc, segv);                                                  \
     else                                                                                                              \
-        hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->t_start, op->t_seg,       \
-                           op->rt_gbase, op->rt_ebase, op->rt_len, op->rt_vox, val, x, sc, segv);
-    if (op->tw == 256) {
+        hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,     \
+                           op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
+                           op->rt_lo, op->rt_vox, val, x, sc, segv);
+    if (op->rt_tw == 512) {
+        PASS1(448, 512)
+    } else if (op->rt_tw == 256) {
         PASS1(448, 256)
     } else {
         PASS1(512, 128)
@@ -589,8 +602,10 @@ int exclusive_scan(plsqr_op *op, const int *cnt, int *out, long long len) {
     return PLSQR_OK;
 }
 
+// L: the mode-2 layout; R: the mode-1 layout, built on the tiles of L1
+// (L itself, or stacks of two of its bricks).
 int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const plsqr::TileLayout *L,
-          const plsqr::RowTiles *R) {
+          const plsqr::TileLayout *L1, const plsqr::RowTiles *R) {
     using namespace plsqr;
     int rc;
     const long long nk = s->nnz_k, nd = s->nnz_d;
@@ -767,19 +782,56 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
             if (nrow > 0) HIP_TRY(op, hipMemcpy(op->t_roff, L->row_off.data(), sizeof(int) * nrow, hipMemcpyHostToDevice));
         }
         if (R) {
+            if (!L1 || L1->S != L->S) return set_err(op, PLSQR_ERR_STATE, "mode-1 layout without its tiles");
             op->rtiled = true;
             op->rt_nent = R->n_ent;
+            op->rt_tw = L1->TW;
+            op->rtT = L1->T;
             if ((rc = dalloc(op, &op->bounds, (size_t)L->S + 1))) return rc;
             HIP_TRY(op, hipMemcpy(op->bounds, L->slice_bounds.data(), sizeof(int) * (L->S + 1), hipMemcpyHostToDevice));
-            const long long items2 = (long long)L->T * L->S;
-            if ((rc = dalloc(op, &op->rt_gbase, (size_t)items2 + 1)) ||
+            const long long items2 = (long long)L1->T * L1->S;
+            if ((rc = dalloc(op, &op->rt_start, (size_t)L1->T + 1)) || (rc = dalloc(op, &op->rt_seg, (size_t)items2 + 1)) ||
+                (rc = dalloc(op, &op->rt_gbase, (size_t)items2 + 1)) ||
                 (rc = dalloc(op, &op->rt_ebase, (size_t)R->n_grp + 1)) ||
                 (rc = dalloc(op, &op->rt_len, (size_t)R->n_grp * plsqr::RowTiles::G)) ||
                 // entries carry 4 of slack: the 16-byte staging loads may read past the last group
                 (rc = dalloc(op, &op->rt_vox, (size_t)R->n_ent + 4)) || (rc = dalloc(op, &op->rt_val, (size_t)R->n_ent + 4)) ||
-                (rc = dalloc(op, &op->rt_rptr, (size_t)op->m_k + 1)) || (rc = dalloc(op, &op->rt_rseg, (size_t)L->n_seg)) ||
-                (rc = dalloc(op, &op->rt_segv, (size_t)L->n_seg)))
+                (rc = dalloc(op, &op->rt_rptr, (size_t)op->m_k + 1)) || (rc = dalloc(op, &op->rt_rseg, (size_t)L1->n_seg)) ||
+                (rc = dalloc(op, &op->rt_segv, (size_t)L1->n_seg)))
                 return rc;
+            HIP_TRY(op, hipMemcpy(op->rt_start, L1->tile_start.data(), sizeof(int) * (L1->T + 1), hipMemcpyHostToDevice));
+            HIP_TRY(op, hipMemcpy(op->rt_seg, L1->seg_start.data(), sizeof(long long) * (items2 + 1), hipMemcpyHostToDevice));
+            if (!R->seg_lo.empty()) {
+                if ((rc = dalloc(op, &op->rt_lo, R->seg_lo.size()))) return rc;
+                HIP_TRY(op, hipMemcpy(op->rt_lo, R->seg_lo.data(), R->seg_lo.size(), hipMemcpyHostToDevice));
+            }
+            // chunks of at most rt_chunk groups (PLSQR_M1_CHUNK, 0: whole
+            // items), dealt so that chunk 8k + s belongs to row slice s and
+            // runs on the XCD of the slice's other work
+            op->rt_chunk = std::max(0, env_int("PLSQR_M1_CHUNK", 16));
+            if (op->rt_chunk > 0) {
+                const int S = L1->S;
+                std::vector<std::vector<int>> ci((size_t)S), cg((size_t)S);
+                for (long long it = 0; it < items2; ++it)
+                    for (int g0 = R->grp_base[(size_t)it]; g0 < R->grp_base[(size_t)it + 1]; g0 += op->rt_chunk) {
+                        ci[(size_t)(it % S)].push_back((int)it);
+                        cg[(size_t)(it % S)].push_back(g0);
+                    }
+                size_t kmax = 0;
+                for (int q = 0; q < S; ++q) kmax = std::max(kmax, ci[(size_t)q].size());
+                const size_t nch = std::max<size_t>(kmax * (size_t)S, 1);
+                std::vector<int> hitem(nch, -1), hg0(nch, 0);
+                for (int q = 0; q < S; ++q)
+                    for (size_t k = 0; k < ci[(size_t)q].size(); ++k) {
+                        hitem[k * S + q] = ci[(size_t)q][k];
+                        hg0[k * S + q] = cg[(size_t)q][k];
+                    }
+                if (nch > (size_t)INT_MAX) return set_err(op, PLSQR_ERR_ARG, "too many mode-1 chunks");
+                op->rt_nch = (int)nch;
+                if ((rc = dalloc(op, &op->rt_chitem, nch)) || (rc = dalloc(op, &op->rt_chg0, nch))) return rc;
+                HIP_TRY(op, hipMemcpy(op->rt_chitem, hitem.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+                HIP_TRY(op, hipMemcpy(op->rt_chg0, hg0.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+            }
             HIP_TRY(op, hipMemcpy(op->rt_gbase, R->grp_base.data(), sizeof(int) * (items2 + 1), hipMemcpyHostToDevice));
             HIP_TRY(op, hipMemcpy(op->rt_ebase, R->ent_base.data(), sizeof(long long) * (R->n_grp + 1), hipMemcpyHostToDevice));
             HIP_TRY(op, hipMemcpy(op->rt_rptr, R->ray_ptr.data(), sizeof(int) * (op->m_k + 1), hipMemcpyHostToDevice));
@@ -789,8 +841,8 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
                 HIP_TRY(op, hipMemcpy(op->rt_vox, R->ent_vox.data(), (size_t)R->n_ent, hipMemcpyHostToDevice));
                 HIP_TRY(op, hipMemcpy(op->rt_val, R->ent_val.data(), sizeof(double) * R->n_ent, hipMemcpyHostToDevice));
             }
-            if (L->n_seg > 0)
-                HIP_TRY(op, hipMemcpy(op->rt_rseg, R->ray_seg.data(), sizeof(int) * L->n_seg, hipMemcpyHostToDevice));
+            if (L1->n_seg > 0)
+                HIP_TRY(op, hipMemcpy(op->rt_rseg, R->ray_seg.data(), sizeof(int) * L1->n_seg, hipMemcpyHostToDevice));
         }
         dfree(op, col, nk);
         dfree(op, op->kcp, (size_t)op->n + 1);
@@ -946,19 +998,30 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     if (s->m_k < nsl * 64) nsl = 1;
     std::vector<int> hb = slice_bounds_host(s, nsl);
     // tiled mode-2 layout on bricks of voxels (default operator only)
-    std::unique_ptr<plsqr::TileLayout> L;
+    std::unique_ptr<plsqr::TileLayout> L, L1;  // mode 2; mode 1 on stacked bricks (or none: L)
     std::unique_ptr<plsqr::RowTiles> R;
     std::unique_ptr<plsqr::RaySorted> rays;
     const plsqr_system *sys = s;  // s, or its ray-sorted renumbering
     int tw = TILE_W;
     if (!op->exact && env_int("PLSQR_TILES", 1) != 0) {
-        std::vector<int> perm, tiles;
+        std::vector<int> perm, tiles, stacks;
         if (grid) {
             // bricks of 4 x 8 x 8 voxels (PLSQR_BRICK=128: 4 x 4 x 8)
             const bool big = env_int("PLSQR_BRICK", 256) != 128;
             tw = big ? 256 : TILE_W;
-            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, big ? 8 : 4, 8, env_int("PLSQR_BRICKS", 1), perm,
-                               tiles);
+            const int order = env_int("PLSQR_BRICKS", 1);
+            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, big ? 8 : 4, 8, order, perm, tiles);
+            // mode 1 on stacks of two bricks (4 x 8 x 16, PLSQR_M1_STACK=0: on
+            // the bricks): the stacks number their voxels as two consecutive
+            // bricks do, so they are unions of the brick tiles and the column
+            // order serves both modes
+            if (big && order == 1 && env_int("PLSQR_M1_STACK", 1) != 0) {
+                std::vector<int> perm2;
+                plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 16, order, perm2, stacks);
+                bool same = perm2 == perm;
+                for (int b : stacks) same = same && std::binary_search(tiles.begin(), tiles.end(), b);
+                if (!same) stacks.clear();
+            }
         } else {
             perm.resize((size_t)s->n);
             for (int i = 0; i < s->n; ++i) perm[(size_t)i] = i;
@@ -980,15 +1043,22 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::tile_layout_self_check(sys, *L, &why)) {
             return fail(set_err(op, PLSQR_ERR_STATE, why));
         }
+        if (L && !stacks.empty()) {
+            L1.reset(new plsqr::TileLayout());
+            if (!plsqr::build_tile_layout(sys, perm, stacks, hb, 512, L1.get(), &why)) L1.reset();
+        }
         if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
             R.reset(new plsqr::RowTiles());
             // segments in ray order inside each item; PLSQR_SEGWIN=w sorts them by
             // length inside windows of w so a wavefront's lanes finish together
             // (mode 1 0.59 -> 0.69 ms at 67^3/2M rays with w = 256..1024: the
             // per-ray sums then gather partials from more cache lines)
-            if (!plsqr::build_row_tiles(*L, sys->m_k, R.get(), &why, env_int("PLSQR_SEGWIN", 1))) {
+            if (L1 && !plsqr::build_row_tiles(*L1, sys->m_k, R.get(), &why, env_int("PLSQR_SEGWIN", 1)))
+                L1.reset();  // a segment too long for the stacks: mode 1 on the bricks
+            const plsqr::TileLayout &LR = L1 ? *L1 : *L;
+            if (!L1 && !plsqr::build_row_tiles(*L, sys->m_k, R.get(), &why, env_int("PLSQR_SEGWIN", 1))) {
                 R.reset();  // mode 1 stays on the CSR kernel
-            } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::row_tiles_self_check(sys, *L, *R, &why)) {
+            } else if (env_int("PLSQR_TILE_CHECK", 0) != 0 && !plsqr::row_tiles_self_check(sys, LR, *R, &why)) {
                 return fail(set_err(op, PLSQR_ERR_STATE, why));
             }
         }
@@ -1011,8 +1081,9 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
     e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(set_err(op, PLSQR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     op->own_stream = true;
-    if ((rc = build(op, sys, hb, L.get(), R.get()))) return fail(rc);
+    if ((rc = build(op, sys, hb, L.get(), L1 ? L1.get() : L.get(), R.get()))) return fail(rc);
     R.reset();
+    L1.reset();
     L.reset();
     if (rays) {
         op->rowperm = true;

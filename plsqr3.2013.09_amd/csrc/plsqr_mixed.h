@@ -69,30 +69,45 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
 
 // Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
 // k_rows_tiled_c): FP32 entries and x tile, FP64 sum per segment, the
-// partial s + c rounded once to FP32.
+// partial s + c rounded once to FP32.  Tiles of 512 voxels as in
+// k_rows_tiled_c (seg_lo).
 template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
                                                        const long long *__restrict__ seg_start,
                                                        const int *__restrict__ grp_base,
+                                                     const int *__restrict__ ch_item, const int *__restrict__ ch_g0, int chunk,
                                                        const long long *__restrict__ ent_base,
                                                        const uint8_t *__restrict__ seg_len,
+                                                       const uint8_t *__restrict__ seg_lo,
                                                        const uint8_t *__restrict__ ent_vox,
                                                        const float *__restrict__ ent_val, const float *__restrict__ x,
                                                        const Scal *__restrict__ sc, float *__restrict__ segv) {
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(BS >= TW, "one thread per voxel loads the tile of x");
+    static_assert(TW <= 512, "8-bit voxels and one lower-brick count");
     __shared__ double xl[TW];  // the tile of x, widened once per item
     __shared__ __align__(16) float lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
     if (sc->done) return;
-    const int it = blockIdx.x;
-    const int g0 = grp_base[it], g1 = grp_base[it + 1];
-    if (g0 == g1) return;  // the same for the whole workgroup
+    // workgroup b runs groups ch_g0[b] .. + chunk - 1 of item ch_item[b]
+    // (ch_item NULL: all groups of item b)
+    int it = blockIdx.x;
+    int g0, g1;
+    if (ch_item) {
+        it = ch_item[blockIdx.x];
+        if (it < 0) return;  // padding of the chunk list
+        g0 = ch_g0[blockIdx.x];
+        g1 = min(g0 + chunk, grp_base[it + 1]);
+    } else {
+        g0 = grp_base[it];
+        g1 = grp_base[it + 1];
+    }
+    if (g0 >= g1) return;  // the same for the whole workgroup
+    const int gi0 = grp_base[it];  // the item's first group (segment numbering)
     const int t = it / S;
     const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
-    if (threadIdx.x < tw) xl[threadIdx.x] = (double)x[c0 + threadIdx.x];
+    for (int i = threadIdx.x; i < tw; i += BS) xl[i] = (double)x[c0 + i];
     __syncthreads();
     const int lane = threadIdx.x % G;
     const int wv = threadIdx.x / G;
@@ -101,6 +116,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     const long long sb = seg_start[it];
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
+        int hi = 0;  // first staged index of the lane's upper-brick entries
+        if constexpr (TW > 256) hi = seg_lo[(size_t)g * G + lane];
         int incl = len;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) {
@@ -109,6 +126,11 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
         }
         const int tot = __shfl(incl, G - 1);
         const int beg = incl - len, end = incl;
+        hi += beg;
+        auto vox = [&](int sh, int e) -> int {
+            if constexpr (TW > 256) return bx[sh + e] + (e >= hi ? 256 : 0);
+            return bx[sh + e];
+        };
         const long long gb = ent_base[g];
         double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
@@ -125,8 +147,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             int e = e0;
             for (; e + 3 < e1; e += 4) {
                 const float a0 = bv[sh + e], a1 = bv[sh + e + 1], a2 = bv[sh + e + 2], a3 = bv[sh + e + 3];
-                const double x0 = xl[bx[sh + e]], x1 = xl[bx[sh + e + 1]], x2 = xl[bx[sh + e + 2]],
-                             x3 = xl[bx[sh + e + 3]];
+                const double x0 = xl[vox(sh, e)], x1 = xl[vox(sh, e + 1)], x2 = xl[vox(sh, e + 2)],
+                             x3 = xl[vox(sh, e + 3)];
                 dot32_add(acc, a0, x0);
                 dot32_add(acc, a1, x1);
                 dot32_add(acc, a2, x2);
@@ -134,16 +156,16 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             }
             for (; e + 1 < e1; e += 2) {
                 const float a0 = bv[sh + e], a1 = bv[sh + e + 1];
-                const double x0 = xl[bx[sh + e]], x1 = xl[bx[sh + e + 1]];
+                const double x0 = xl[vox(sh, e)], x1 = xl[vox(sh, e + 1)];
                 dot32_add(acc, a0, x0);
                 dot32_add(acc, a1, x1);
             }
-            if (e < e1) dot32_add(acc, bv[sh + e], xl[bx[sh + e]]);
+            if (e < e1) dot32_add(acc, bv[sh + e], xl[vox(sh, e)]);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = (float)acc;
+        if (len > 0) segv[sb + (long long)(g - gi0) * G + lane] = (float)acc;
     }
 }
 

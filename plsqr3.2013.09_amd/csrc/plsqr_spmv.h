@@ -116,30 +116,47 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 // buffer with coalesced loads, CH at a time, and every lane adds the entries
 // of its segment found in the buffer (its first entry from a wavefront
 // prefix sum of the lane lengths), in entry order.  No padding is stored.
-// The partial goes to segv[seg_start[item] + segment index].
+// The partial goes to segv[seg_start[item] + segment index].  Tiles of 512
+// voxels (two stacked bricks) store 8-bit voxels and, per segment, the
+// number of its entries in the lower brick (seg_lo): entry k of a segment
+// is voxel ent_vox + 256 from k = seg_lo on.
 template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
                                                      const long long *__restrict__ seg_start,
                                                      const int *__restrict__ grp_base,
+                                                     const int *__restrict__ ch_item, const int *__restrict__ ch_g0, int chunk,
                                                      const long long *__restrict__ ent_base,
                                                      const uint8_t *__restrict__ seg_len,
+                                                     const uint8_t *__restrict__ seg_lo,
                                                      const uint8_t *__restrict__ ent_vox,
                                                      const double *__restrict__ ent_val, const double *__restrict__ x,
                                                      const Scal *__restrict__ sc, double *__restrict__ segv) {
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(BS >= TW, "one thread per voxel loads the tile of x");
+    static_assert(TW <= 512, "8-bit voxels and one lower-brick count");
     __shared__ double xl[TW];
     __shared__ __align__(16) double lv[NWV][CH + 4];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 4];
     if (sc && sc->done) return;
-    const int it = blockIdx.x;
-    const int g0 = grp_base[it], g1 = grp_base[it + 1];
-    if (g0 == g1) return;  // the same for the whole workgroup
+    // workgroup b runs groups ch_g0[b] .. + chunk - 1 of item ch_item[b]
+    // (ch_item NULL: all groups of item b)
+    int it = blockIdx.x;
+    int g0, g1;
+    if (ch_item) {
+        it = ch_item[blockIdx.x];
+        if (it < 0) return;  // padding of the chunk list
+        g0 = ch_g0[blockIdx.x];
+        g1 = min(g0 + chunk, grp_base[it + 1]);
+    } else {
+        g0 = grp_base[it];
+        g1 = grp_base[it + 1];
+    }
+    if (g0 >= g1) return;  // the same for the whole workgroup
+    const int gi0 = grp_base[it];  // the item's first group (segment numbering)
     const int t = it / S;
     const int c0 = tile_start[t], tw = tile_start[t + 1] - c0;
-    if (threadIdx.x < tw) xl[threadIdx.x] = x[c0 + threadIdx.x];
+    for (int i = threadIdx.x; i < tw; i += BS) xl[i] = x[c0 + i];
     __syncthreads();
     const int lane = threadIdx.x % G;
     const int wv = threadIdx.x / G;
@@ -148,6 +165,9 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     const long long sb = seg_start[it];
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
+        // staged entry e (lane's own, beg <= e < end): its voxel in the tile
+        int hi = 0;  // first staged index of the lane's upper-brick entries
+        if constexpr (TW > 256) hi = seg_lo[(size_t)g * G + lane];
         int incl = len;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) {
@@ -156,6 +176,11 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
         }
         const int tot = __shfl(incl, G - 1);  // entries of the group
         const int beg = incl - len, end = incl;
+        hi += beg;
+        auto vox = [&](int sh, int e) -> int {
+            if constexpr (TW > 256) return bx[sh + e] + (e >= hi ? 256 : 0);
+            return bx[sh + e];
+        };
         const long long gb = ent_base[g];
         double acc = 0.0;
         for (int q0 = 0; q0 < tot; q0 += CH) {
@@ -170,27 +195,27 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
             // adds (in entry order), so several LDS gathers are in flight
             int e = e0;
             for (; e + 3 < e1; e += 4) {
-                const double p0 = bv[sh + e] * xl[bx[sh + e]];
-                const double p1 = bv[sh + e + 1] * xl[bx[sh + e + 1]];
-                const double p2 = bv[sh + e + 2] * xl[bx[sh + e + 2]];
-                const double p3 = bv[sh + e + 3] * xl[bx[sh + e + 3]];
+                const double p0 = bv[sh + e] * xl[vox(sh, e)];
+                const double p1 = bv[sh + e + 1] * xl[vox(sh, e + 1)];
+                const double p2 = bv[sh + e + 2] * xl[vox(sh, e + 2)];
+                const double p3 = bv[sh + e + 3] * xl[vox(sh, e + 3)];
                 acc += p0;
                 acc += p1;
                 acc += p2;
                 acc += p3;
             }
             for (; e + 1 < e1; e += 2) {
-                const double p0 = bv[sh + e] * xl[bx[sh + e]];
-                const double p1 = bv[sh + e + 1] * xl[bx[sh + e + 1]];
+                const double p0 = bv[sh + e] * xl[vox(sh, e)];
+                const double p1 = bv[sh + e + 1] * xl[vox(sh, e + 1)];
                 acc += p0;
                 acc += p1;
             }
-            if (e < e1) acc += bv[sh + e] * xl[bx[sh + e]];
+            if (e < e1) acc += bv[sh + e] * xl[vox(sh, e)];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - g0) * G + lane] = acc;
+        if (len > 0) segv[sb + (long long)(g - gi0) * G + lane] = acc;
     }
 }
 

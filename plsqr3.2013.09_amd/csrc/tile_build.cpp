@@ -356,6 +356,11 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
         *err = "the mode-1 layout is built from the padded slots";
         return false;
     }
+    if (TW > 512) {
+        *err = "mode-1 tiles hold at most 512 voxels";
+        return false;
+    }
+    const bool stacked = TW > 256;
     R->grp_base.assign((size_t)items + 1, 0);
     long long gtot = 0;
     for (long long it = 0; it < items; ++it) {
@@ -370,6 +375,8 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     R->n_grp = (int)gtot;
     R->ent_base.assign((size_t)gtot + 1, 0);
     R->seg_len.assign((size_t)gtot * G, 0);
+    if (stacked) R->seg_lo.assign((size_t)gtot * G, 0);
+    else R->seg_lo.clear();
     // entries of an item = its slots' nonzeros
     std::vector<long long> item_ent((size_t)items + 1, 0);
     for (long long it = 0; it < items; ++it) {
@@ -389,14 +396,19 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     std::vector<int> pos((size_t)L.n_seg, 0);
     std::atomic<bool> ok{true};
     par_for(items, [&](long long ib, long long ie) {
-        std::vector<int> len, ord;
+        std::vector<int> len, ord, lo;
         std::vector<long long> off;
         for (long long it = ib; it < ie; ++it) {
             const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
             len.assign((size_t)ns, 0);
+            lo.assign((size_t)ns, 0);
             const long long base = L.slot_base[(size_t)it];
             for (int v = 0; v < TW; ++v)
-                for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) ++len[L.sseg[(size_t)(base + (long long)j * TW + v)]];
+                for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) {
+                    const int sg = L.sseg[(size_t)(base + (long long)j * TW + v)];
+                    ++len[(size_t)sg];
+                    if (v < 256) ++lo[(size_t)sg];
+                }
             ord.resize((size_t)ns);
             for (int i = 0; i < ns; ++i) ord[(size_t)i] = i;
             if (window > 1)
@@ -414,8 +426,11 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
             const int g0 = R->grp_base[(size_t)it];
             for (int g = 0; g * G < ns; ++g) {
                 R->ent_base[(size_t)g0 + g] = off[(size_t)g * G];
-                for (int l = 0; l < G && g * G + l < ns; ++l)
+                for (int l = 0; l < G && g * G + l < ns; ++l) {
                     R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)ord[(size_t)(g * G + l)]], 255);
+                    if (stacked)
+                        R->seg_lo[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(lo[(size_t)ord[(size_t)(g * G + l)]], 255);
+                }
             }
             // voxels ascending: each segment's list comes out in voxel order
             for (int v = 0; v < TW; ++v)
@@ -423,7 +438,7 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
                     const long long slot = base + (long long)j * TW + v;
                     const int sg = L.sseg[(size_t)slot];
                     const long long o = off[(size_t)pit[sg]]++;
-                    R->ent_vox[(size_t)o] = (uint8_t)v;
+                    R->ent_vox[(size_t)o] = (uint8_t)(v & 255);
                     R->ent_val[(size_t)o] = L.sval[(size_t)slot];
                 }
         }
@@ -460,10 +475,11 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
                     if (len == 0) continue;
                     long long lo = 0;  // the lengths of the lanes before l
                     for (int q = 0; q < l; ++q) lo += R.seg_len[(size_t)g * G + q];
+                    const int nlo = R.seg_lo.empty() ? len : R.seg_lo[(size_t)g * G + l];
                     double acc = 0.0;
                     for (int k = 0; k < len; ++k) {
                         const long long o = R.ent_base[(size_t)g] + lo + k;
-                        acc += R.ent_val[(size_t)o] * x[c0 + R.ent_vox[(size_t)o]];
+                        acc += R.ent_val[(size_t)o] * x[c0 + R.ent_vox[(size_t)o] + (k >= nlo ? 256 : 0)];
                     }
                     segv[(size_t)(L.seg_start[(size_t)it] + (long long)(g - R.grp_base[(size_t)it]) * G + l)] = acc;
                 }

@@ -106,7 +106,12 @@ struct RowTiles {
     std::vector<int> grp_base;         // items+1: first group of each item
     std::vector<long long> ent_base;   // groups+1: first entry of each group
     std::vector<uint8_t> seg_len;      // groups*G: entries of each lane's segment (0: no segment)
-    std::vector<uint8_t> ent_vox;      // entries: voxel inside the tile
+    // Tiles of 512 voxels (two stacked bricks): the voxel of an entry is
+    // ent_vox + 256 from the segment's seg_lo-th entry on (a segment's
+    // entries ascend in voxel, so its lower-brick entries come first);
+    // empty for tiles of at most 256 voxels.
+    std::vector<uint8_t> seg_lo;       // groups*G
+    std::vector<uint8_t> ent_vox;      // entries: voxel inside the tile (mod 256)
     std::vector<double> ent_val;       // entries
     std::vector<int> ray_ptr;          // m_k+1
     std::vector<int> ray_seg;          // n_seg: segment indices, tile order per ray
@@ -118,8 +123,9 @@ struct RowTiles {
 // Inside an item the segments are taken in windows of `window` (ray order
 // between windows), sorted by descending length inside each window so that a
 // wavefront's 64 segments have similar lengths (window <= 1: ray order).
-// Returns false with *err set when a segment holds more than 255 entries,
-// the group count overflows or the layout is already compacted (build the
+// Tiles may hold up to 512 voxels (seg_lo above).  Returns false with *err
+// set when a segment holds more than 255 entries, the group count overflows,
+// a tile is wider than 512 or the layout is already compacted (build the
 // mode-1 layout first, then compact_tile_layout).
 bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err, int window = 1);
 

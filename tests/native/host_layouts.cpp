@@ -153,6 +153,36 @@ static void layouts(const plsqr_system *s, const plsqr_grid *grid, int nsl, int 
                 L.n_seg, L.max_seg_per_item, padded, cuts, R.n_grp, padded, L.n_slot);
 }
 
+// Mode 1 on stacks of two bricks: 4 x 8 x 16 bricks number the voxels as
+// two consecutive 4 x 8 x 8 bricks do, so the stacked tiles are unions of
+// the 256-voxel tiles and the same column order serves both; entries keep
+// 8-bit voxels plus a per-segment count of lower-brick entries.
+static void stacked(const plsqr_system *s, const plsqr_grid *grid, int nsl) {
+    std::vector<int> hb = slice_bounds(s, nsl);
+    std::vector<int> perm, tiles, perm2, tiles2;
+    plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 8, 1, perm, tiles);
+    plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 16, 1, perm2, tiles2);
+    CHECK(perm == perm2, "stacked bricks number the columns differently");
+    bool subset = true;
+    for (int b : tiles2) subset = subset && std::binary_search(tiles.begin(), tiles.end(), b);
+    CHECK(subset, "a stacked tile is not a union of brick tiles");
+    plsqr::order_tile_columns(s, tiles, perm);
+    plsqr::RaySorted rays;
+    plsqr::sort_rays(s, perm, tiles, hb, grid, &rays);
+    plsqr::TileLayout L;
+    std::string why;
+    bool ok = plsqr::build_tile_layout(&rays.sys, perm, tiles2, hb, 512, &L, &why);
+    CHECK(ok, "stacked build_tile_layout: %s", why.c_str());
+    if (!ok) return;
+    plsqr::RowTiles R;
+    ok = plsqr::build_row_tiles(L, rays.sys.m_k, &R, &why);
+    CHECK(ok, "stacked build_row_tiles: %s", why.c_str());
+    if (!ok) return;
+    CHECK(!R.seg_lo.empty(), "no lower-brick counts for stacked tiles");
+    CHECK(plsqr::row_tiles_self_check(&rays.sys, L, R, &why), "stacked row_tiles_self_check: %s", why.c_str());
+    std::printf("stacked mode-1 tiles n=%d m_k=%d: %lld segments, %d groups\n", s->n, s->m_k, L.n_seg, R.n_grp);
+}
+
 static void files(const plsqr_system *s, const std::string &dir) {
     int rc = plsqr_write_reference_files(s, dir.c_str(), "k.bin", "k.info", "d.bin", "d.info", "dc.bin", "dc.info",
                                          "b.txt");
@@ -303,6 +333,7 @@ int main(int argc, char **argv) {
         }
         for (int window : {1, 64}) layouts(&s, c.grid ? &g : nullptr, c.slices, window, true);
         layouts(&s, c.grid ? &g : nullptr, c.slices, 1, false);
+        if (c.grid) stacked(&s, &g, c.slices);
         if (&c == &cases[0]) files(&s, dir);
         plsqr_system_free(&s);
     }

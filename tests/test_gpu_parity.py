@@ -517,7 +517,9 @@ def test_mixed_warm_start_with_fewer_rows_than_columns(P, O, gpu_available):
 @pytest.mark.parametrize("knob,values,grid", [("PLSQR_SEGWIN", ("1", "64", "512"), "auto"),
                                                ("PLSQR_DAMP_IL", ("0", "1"), "auto"),
                                                ("PLSQR_COMPACT", ("0", "1", "2"), "auto"),
-                                               ("PLSQR_COMPACT", ("0", "1", "2"), None)])
+                                               ("PLSQR_COMPACT", ("0", "1", "2"), None),
+                                               ("PLSQR_M1_CHUNK", ("0", "16", "4"), "auto"),
+                                               ("PLSQR_M1_CHUNK", ("0", "16", "4"), None)])
 def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, knob, values, grid):
     """Layout choices that keep every sum's order: the mode-1 layout may take
     an item's segments sorted by length inside windows (PLSQR_SEGWIN; every
@@ -526,7 +528,9 @@ def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, kno
     groups of 64 (PLSQR_DAMP_IL; each column's entries in the same order),
     and mode 2's slots may be stored by voxel or by position, in packed or
     TW-wide rows (PLSQR_COMPACT 0, 1, 2; every voxel still adds its slots in
-    ascending order), on 4 x 8 x 8 bricks
+    ascending order), mode 1's first pass may give each workgroup a whole
+    item or chunks of its segment groups (PLSQR_M1_CHUNK; each segment is
+    still summed by one lane), on 4 x 8 x 8 bricks
     of the voxel grid ("auto": the system's grid) or on tiles of 128
     consecutive columns (no grid).  aprod of
     both modes, FP64 and FP32, and whole solves (FP64; mixed; mixed with
@@ -556,3 +560,40 @@ def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, kno
     for val in values[1:]:
         for a, b in zip(res[val], res[values[0]]):
             assert np.array_equal(a, b), (knob, val)
+
+
+def test_mode1_stacks_match_bricks(P, O, MG, gpu_available, monkeypatch):
+    """Mode 1 on stacks of two bricks (4 x 8 x 16 voxels, 8-bit voxels plus a
+    lower-brick count per segment) or on the bricks (PLSQR_M1_STACK=0): a
+    ray's partials are grouped differently, so the two agree to rounding,
+    both match the oracle in FP64 and FP32, and a whole solve lands within
+    the oracle's own sensitivity of the reference x."""
+    announce("mode-1 stacks")
+    s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
+    g = load_golden("grid16_rays20k_itn100")
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    orc = O.Oracle(s)
+    yo = y.copy()
+    orc.aprod(1, x.copy(), yo)
+    out = {}
+    for val in ("0", "1"):
+        monkeypatch.setenv("PLSQR_M1_STACK", val)
+        with P.Operator(s) as op:
+            yg = y.copy()
+            op.aprod(1, x.copy(), yg)
+            y32 = y.astype(np.float32)
+            op.aprod_f32(1, x.astype(np.float32), y32)
+            xs, _ = op.lsqr(s.b, 100)
+            xm, im = op.lsqr(s.b, 400, tol=1e-6, precision=P.PREC_MIXED, refine_every=25)
+        out[val] = (yg, y32, xs)
+        assert rel(yg, yo) < 1e-13, (val, rel(yg, yo))
+        assert rel(y32.astype(np.float64), yo) < 1e-6, (val, rel(y32.astype(np.float64), yo))
+        assert im["arnorm"] <= 1e-6
+    assert rel(out["1"][0], out["0"][0]) < 1e-14
+    b2 = s.b * (1 + 1e-15 * np.random.default_rng(1).standard_normal(s.b.size))
+    xp, _, _ = orc.lsqr(b2, 100, trace=False)
+    sens = rel(xp, g["x"])
+    for val in ("0", "1"):
+        assert rel(out[val][2], g["x"]) <= max(1e-10, 100 * sens), (val, rel(out[val][2], g["x"]), sens)
