@@ -1222,13 +1222,15 @@ static void launch_cols_tiled32(plsqr_op *op, const float *u, const Scal *sc) {
     hipLaunchKernelGGL((k_cols_tiled32<TW, B, CAP, FORM>), dim3(op->tT * op->tS), dim3(B), 0, op->stream, op->n,      \
                        op->tS, op->t_start, op->t_slot, op->t_seg, op->t_vlen, op->t_ray, op->t_sval32, op->t_sseg,   \
                        op->t_cut, op->t_vcut, sr, u, sc, op->P);
+    // 512 threads (two per voxel) on 256-voxel tiles: 36 KB of LDS, so four
+    // items share a CU and one item's staging overlaps the others' slot loops
     if (op->tw == 256) {
         if (op->t_form == 1) {
-            COLS32(256, 1024, 8192, 1)
+            COLS32(256, 512, 8192, 1)
         } else if (op->t_form == 2) {
-            COLS32(256, 1024, 8192, 2)
+            COLS32(256, 512, 8192, 2)
         } else {
-            COLS32(256, 1024, 8192, 0)
+            COLS32(256, 512, 8192, 0)
         }
     } else if (op->t_form == 1) {
         COLS32(TILE_W, 512, 4096, 1)

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int
 // Mode 2 in FP32 on the tiled layout (same item structure and slot forms as
 // k_cols_tiled).
 template <int TW, int BS, int CAP, int FORM>
-__global__ __launch_bounds__(BS) void k_cols_tiled32(
+__global__ __launch_bounds__(BS, 8) void k_cols_tiled32(
     int n, int S, const int *__restrict__ tile_start, const long long *__restrict__ slot_base,
     const long long *__restrict__ seg_start, const uint16_t *__restrict__ vox_len,
     const int *__restrict__ seg_ray, const float *__restrict__ sval, const uint16_t *__restrict__ sseg,
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     const float *__restrict__ u, const Scal *__restrict__ sc, double *__restrict__ P) {
     static_assert(TW % kWave == 0, "the lanes of a wavefront share their row phase w");
     constexpr int NW = BS / TW;
-    __shared__ double g[CAP];  // fl32(u * ib) per segment, held widened
+    __shared__ float g[CAP];  // fl32(u * ib) per segment
     __shared__ double part[NW][TW];
     if (sc->done || !sc->beta_pos) return;
     const float ib = (float)sc->ib;
@@ -259,23 +259,31 @@ __global__ __launch_bounds__(BS) void k_cols_tiled32(
     int j = w;
     for (int r0 = 0; r0 < ns; r0 += CAP) {
         const int cnt = ns - r0 < CAP ? ns - r0 : CAP;
+        // the round's segments, at most 8 per thread in flight at a time
         constexpr int PER = CAP / BS;
-        int rr[PER];
+        constexpr int PB = PER < 8 ? PER : 8;
+        static_assert(PER % PB == 0, "whole batches");
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * BS;
-            rr[q] = i < cnt ? seg_ray[s0 + r0 + i] : 0;
-        }
-        float gv[PER];
+        for (int b0 = 0; b0 < PER; b0 += PB) {
+            if (b0 * BS < cnt) {
+                int rr[PB];
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * BS;
-            gv[q] = i < cnt ? u[rr[q]] : 0.f;
-        }
+                for (int q = 0; q < PB; ++q) {
+                    const int i = threadIdx.x + (b0 + q) * BS;
+                    rr[q] = i < cnt ? seg_ray[s0 + r0 + i] : 0;
+                }
+                float gv[PB];
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * BS;
-            if (i < cnt) g[i] = (double)(gv[q] * ib);
+                for (int q = 0; q < PB; ++q) {
+                    const int i = threadIdx.x + (b0 + q) * BS;
+                    gv[q] = i < cnt ? u[rr[q]] : 0.f;
+                }
+#pragma unroll
+                for (int q = 0; q < PB; ++q) {
+                    const int i = threadIdx.x + (b0 + q) * BS;
+                    if (i < cnt) g[i] = gv[q] * ib;
+                }
+            }
         }
         __syncthreads();
         const int hi = r0 + cnt;
