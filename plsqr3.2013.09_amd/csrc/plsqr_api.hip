@@ -273,28 +273,34 @@ struct SolveScope {
 // -------------------------------------------------------------------------
 constexpr int BS = 256;
 
-// Mode-1 pass 1 on the bricked layout (FP64 or the FP32 copy).  Entries are
-// staged 512 per LDS round on 128-voxel tiles and 448 on 256-voxel tiles, so
-// a workgroup's LDS stays under 20 KB (8 workgroups per CU).
+// Mode-1 pass 1 on the bricked layout (FP64 or the FP32 copy).  A wavefront
+// stages its group's entries in LDS rounds of CH; a round costs one memory
+// wait, so CH is as large as the loads per lane allow without losing
+// occupancy: 512-voxel tiles (64 segments hold 474 entries on average at
+// 67^3 / 2M rays) stage 510 FP64 entries (4 loads per lane, 22.6 KB of LDS,
+// 7 workgroups per CU) or 764 FP32 entries (3 loads, 19.5 KB, 8 per CU);
+// 1024-voxel tiles 382 FP64 / 764 FP32; 256- and 128-voxel tiles 448 / 512.
 template <class T>
 void launch_pass1(plsqr_op *op, const T *val, const T *x, const Scal *sc, T *segv) {
     using namespace plsqr;
     const dim3 g(op->rt_chunk ? op->rt_nch : op->rtT * op->tS), b(BS);
-#define PASS1(CH, TW)                                                                                                 \
+#define PASS1(CH64, CH32, TW)                                                                                       \
     if constexpr (std::is_same<T, double>::value)                                                                     \
-        hipLaunchKernelGGL((k_rows_tiled_c<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,       \
+        hipLaunchKernelGGL((k_rows_tiled_c<BS, CH64, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,     \
                            op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
                            op->rt_lo, op->rt_vox, val, x, sc, segv);                                                  \
     else                                                                                                              \
-        hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,     \
+        hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH32, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,   \
                            op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
                            op->rt_lo, op->rt_vox, val, x, sc, segv);
-    if (op->rt_tw == 512) {
-        PASS1(448, 512)
+    if (op->rt_tw == 1024) {
+        PASS1(382, 764, 1024)
+    } else if (op->rt_tw == 512) {
+        PASS1(510, 764, 512)
     } else if (op->rt_tw == 256) {
-        PASS1(448, 256)
+        PASS1(448, 448, 256)
     } else {
-        PASS1(512, 128)
+        PASS1(512, 512, 128)
     }
 #undef PASS1
 }
@@ -1010,17 +1016,23 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
             const bool big = env_int("PLSQR_BRICK", 256) != 128;
             tw = big ? 256 : TILE_W;
             const int order = env_int("PLSQR_BRICKS", 1);
-            plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, big ? 8 : 4, 8, order, perm, tiles);
-            // mode 1 on stacks of two bricks (4 x 8 x 16, PLSQR_M1_STACK=0: on
-            // the bricks): the stacks number their voxels as two consecutive
-            // bricks do, so they are unions of the brick tiles and the column
-            // order serves both modes
-            if (big && order == 1 && env_int("PLSQR_M1_STACK", 1) != 0) {
-                std::vector<int> perm2;
-                plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 16, order, perm2, stacks);
-                bool same = perm2 == perm;
-                for (int b : stacks) same = same && std::binary_search(tiles.begin(), tiles.end(), b);
-                if (!same) stacks.clear();
+            // mode 1 on super-bricks of the bricks (PLSQR_M1_STACK: 0 none, 1
+            // two stacked bricks, 4 x 8 x 16; 2 four, 8 x 8 x 16; 3, the
+            // default, two side by side, 8 x 8 x 8): every super-brick is a
+            // run of consecutive bricks, so its tile is a union of brick tiles
+            // and one column order serves both modes.  At 67^3 / 2M rays a
+            // ray crosses 13.5, 10.3 and 12.6 of them (15.9 bricks); 8 x 8 x 8
+            // gives the fastest mode 1 (the 1024-voxel tiles shorten the
+            // per-ray combine most but need two staging rounds per group)
+            const int stack = big && order == 1 ? env_int("PLSQR_M1_STACK", 3) : 0;
+            if (stack >= 1 && stack <= 3) {
+                const int qx = stack == 1 ? 1 : 2, qz = stack == 3 ? 1 : 2;
+                plsqr::brick_order_grouped(grid->nx, grid->ny, grid->nz, 4, 8, 8, qx, 1, qz, perm, tiles, stacks);
+                bool sub = true;
+                for (int b : stacks) sub = sub && std::binary_search(tiles.begin(), tiles.end(), b);
+                if (!sub) return fail(set_err(op, PLSQR_ERR_STATE, "super-bricks that are not unions of bricks"));
+            } else {
+                plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, big ? 8 : 4, 8, order, perm, tiles);
             }
         } else {
             perm.resize((size_t)s->n);
@@ -1045,7 +1057,10 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         }
         if (L && !stacks.empty()) {
             L1.reset(new plsqr::TileLayout());
-            if (!plsqr::build_tile_layout(sys, perm, stacks, hb, 512, L1.get(), &why)) L1.reset();
+            int tw1 = 0;  // the widest super-brick, rounded up to a kernel tile width
+            for (size_t k = 0; k + 1 < stacks.size(); ++k) tw1 = std::max(tw1, stacks[k + 1] - stacks[k]);
+            tw1 = tw1 <= 512 ? 512 : 1024;
+            if (!plsqr::build_tile_layout(sys, perm, stacks, hb, tw1, L1.get(), &why)) L1.reset();
         }
         if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
             R.reset(new plsqr::RowTiles());

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(BS) void k_rows32(int m_k, int m, const int *__rest
 
 // Mode 1 in FP32 on the bricked layout, pass 1 (the structure of
 // k_rows_tiled_c): FP32 entries and x tile, FP64 sum per segment, the
-// partial s + c rounded once to FP32.  Tiles of 512 voxels as in
+// partial s + c rounded once to FP32.  Tiles of 512 and 1024 voxels as in
 // k_rows_tiled_c (seg_lo).
 template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restrict__ tile_start,
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(TW <= 512, "8-bit voxels and one lower-brick count");
+    static_assert(TW <= 1024, "8-bit voxels and at most three per-segment counts");
     __shared__ double xl[TW];  // the tile of x, widened once per item
     __shared__ __align__(16) float lv[NWV][CH + 8];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 8];
@@ -116,8 +116,18 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     const long long sb = seg_start[it];
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
-        int hi = 0;  // first staged index of the lane's upper-brick entries
-        if constexpr (TW > 256) hi = seg_lo[(size_t)g * G + lane];
+        // staged entry e (lane's own, beg <= e < end): its voxel in the tile
+        // is its 8-bit voxel + 256 for each of h1, h2, h3 it has reached
+        // (the first staged indices at voxel 256, 512, 768)
+        int h1 = 0, h2 = 0, h3 = 0;
+        if constexpr (TW > 512) {
+            const uint32_t c = reinterpret_cast<const uint32_t *>(seg_lo)[(size_t)g * G + lane];
+            h1 = (int)(c & 255u);
+            h2 = (int)((c >> 8) & 255u);
+            h3 = (int)((c >> 16) & 255u);
+        } else if constexpr (TW > 256) {
+            h1 = seg_lo[(size_t)g * G + lane];
+        }
         int incl = len;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) {
@@ -126,9 +136,12 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
         }
         const int tot = __shfl(incl, G - 1);
         const int beg = incl - len, end = incl;
-        hi += beg;
+        h1 += beg;
+        h2 += beg;
+        h3 += beg;
         auto vox = [&](int sh, int e) -> int {
-            if constexpr (TW > 256) return bx[sh + e] + (e >= hi ? 256 : 0);
+            if constexpr (TW > 512) return bx[sh + e] + (((e >= h1) + (e >= h2) + (e >= h3)) << 8);
+            if constexpr (TW > 256) return bx[sh + e] + (e >= h1 ? 256 : 0);
             return bx[sh + e];
         };
         const long long gb = ent_base[g];

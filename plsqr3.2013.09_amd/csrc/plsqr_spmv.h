@@ -116,10 +116,12 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 // buffer with coalesced loads, CH at a time, and every lane adds the entries
 // of its segment found in the buffer (its first entry from a wavefront
 // prefix sum of the lane lengths), in entry order.  No padding is stored.
-// The partial goes to segv[seg_start[item] + segment index].  Tiles of 512
-// voxels (two stacked bricks) store 8-bit voxels and, per segment, the
-// number of its entries in the lower brick (seg_lo): entry k of a segment
-// is voxel ent_vox + 256 from k = seg_lo on.
+// The partial goes to segv[seg_start[item] + segment index].  Tiles of more
+// than 256 voxels (two or four bricks) store 8-bit voxels and, per segment,
+// the number of its entries below voxel 256 (512 voxels: one byte of
+// seg_lo; entry k is voxel ent_vox + 256 from k = seg_lo on) or below 256,
+// 512, 768 (1024 voxels: the bytes of a 32-bit word; entry k adds 256 for
+// each count it has reached).
 template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
                                                      const long long *__restrict__ seg_start,
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     constexpr int G = 64;
     constexpr int NWV = BS / G;
     static_assert(G == kWave, "one segment group per wavefront");
-    static_assert(TW <= 512, "8-bit voxels and one lower-brick count");
+    static_assert(TW <= 1024, "8-bit voxels and at most three per-segment counts");
     __shared__ double xl[TW];
     __shared__ __align__(16) double lv[NWV][CH + 4];
     __shared__ __align__(16) uint8_t lx[NWV][CH + 4];
@@ -166,8 +168,17 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
         // staged entry e (lane's own, beg <= e < end): its voxel in the tile
-        int hi = 0;  // first staged index of the lane's upper-brick entries
-        if constexpr (TW > 256) hi = seg_lo[(size_t)g * G + lane];
+        // is its 8-bit voxel + 256 for each of h1, h2, h3 it has reached
+        // (the first staged indices at voxel 256, 512, 768)
+        int h1 = 0, h2 = 0, h3 = 0;
+        if constexpr (TW > 512) {
+            const uint32_t c = reinterpret_cast<const uint32_t *>(seg_lo)[(size_t)g * G + lane];
+            h1 = (int)(c & 255u);
+            h2 = (int)((c >> 8) & 255u);
+            h3 = (int)((c >> 16) & 255u);
+        } else if constexpr (TW > 256) {
+            h1 = seg_lo[(size_t)g * G + lane];
+        }
         int incl = len;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) {
@@ -176,9 +187,12 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
         }
         const int tot = __shfl(incl, G - 1);  // entries of the group
         const int beg = incl - len, end = incl;
-        hi += beg;
+        h1 += beg;
+        h2 += beg;
+        h3 += beg;
         auto vox = [&](int sh, int e) -> int {
-            if constexpr (TW > 256) return bx[sh + e] + (e >= hi ? 256 : 0);
+            if constexpr (TW > 512) return bx[sh + e] + (((e >= h1) + (e >= h2) + (e >= h3)) << 8);
+            if constexpr (TW > 256) return bx[sh + e] + (e >= h1 ? 256 : 0);
             return bx[sh + e];
         };
         const long long gb = ent_base[g];

@@ -66,6 +66,32 @@ void brick_order(int nx, int ny, int nz, int bx, int by, int bz, int order,
     tile_start.push_back(next);
 }
 
+void brick_order_grouped(int nx, int ny, int nz, int bx, int by, int bz, int qx, int qy, int qz,
+                         std::vector<int> &perm, std::vector<int> &tile_start, std::vector<int> &super_start) {
+    const long long n = (long long)nx * ny * nz;
+    perm.assign((size_t)n, 0);
+    tile_start.clear();
+    super_start.clear();
+    int next = 0;
+    const int sx = bx * qx, sy = by * qy, sz = bz * qz;  // super-brick extent in voxels
+    for (int y1 = 0; y1 < ny; y1 += sy)
+        for (int x1 = 0; x1 < nx; x1 += sx)
+            for (int z1 = 0; z1 < nz; z1 += sz) {
+                super_start.push_back(next);
+                for (int y0 = y1; y0 < std::min(ny, y1 + sy); y0 += by)
+                    for (int x0 = x1; x0 < std::min(nx, x1 + sx); x0 += bx)
+                        for (int z0 = z1; z0 < std::min(nz, z1 + sz); z0 += bz) {
+                            tile_start.push_back(next);
+                            for (int z = z0; z < std::min(nz, z0 + bz); ++z)
+                                for (int y = y0; y < std::min(ny, y0 + by); ++y)
+                                    for (int x = x0; x < std::min(nx, x0 + bx); ++x)
+                                        perm[(size_t)z * ny * nx + (size_t)y * nx + x] = next++;
+                        }
+            }
+    tile_start.push_back(next);
+    super_start.push_back(next);
+}
+
 bool build_tile_layout(const plsqr_system *s, const std::vector<int> &perm,
                        const std::vector<int> &tile_start, const std::vector<int> &slice_bounds,
                        int TW, TileLayout *L, std::string *err) {
@@ -356,11 +382,14 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
         *err = "the mode-1 layout is built from the padded slots";
         return false;
     }
-    if (TW > 512) {
-        *err = "mode-1 tiles hold at most 512 voxels";
+    if (TW > 1024) {
+        *err = "mode-1 tiles hold at most 1024 voxels";
         return false;
     }
-    const bool stacked = TW > 256;
+    // per segment: the number of its entries below voxel 256 (j + 1), j < NC
+    const int NC = (TW - 1) / 256;
+    const int CB = NC == 0 ? 0 : NC == 1 ? 1 : 4;
+    R->cut_bytes = CB;
     R->grp_base.assign((size_t)items + 1, 0);
     long long gtot = 0;
     for (long long it = 0; it < items; ++it) {
@@ -375,8 +404,7 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     R->n_grp = (int)gtot;
     R->ent_base.assign((size_t)gtot + 1, 0);
     R->seg_len.assign((size_t)gtot * G, 0);
-    if (stacked) R->seg_lo.assign((size_t)gtot * G, 0);
-    else R->seg_lo.clear();
+    R->seg_lo.assign((size_t)gtot * G * CB, 255);
     // entries of an item = its slots' nonzeros
     std::vector<long long> item_ent((size_t)items + 1, 0);
     for (long long it = 0; it < items; ++it) {
@@ -401,13 +429,13 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
         for (long long it = ib; it < ie; ++it) {
             const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
             len.assign((size_t)ns, 0);
-            lo.assign((size_t)ns, 0);
+            lo.assign((size_t)ns * (NC > 0 ? NC : 1), 0);
             const long long base = L.slot_base[(size_t)it];
             for (int v = 0; v < TW; ++v)
                 for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) {
                     const int sg = L.sseg[(size_t)(base + (long long)j * TW + v)];
                     ++len[(size_t)sg];
-                    if (v < 256) ++lo[(size_t)sg];
+                    for (int c = v / 256; c < NC; ++c) ++lo[(size_t)sg * NC + c];
                 }
             ord.resize((size_t)ns);
             for (int i = 0; i < ns; ++i) ord[(size_t)i] = i;
@@ -427,9 +455,10 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
             for (int g = 0; g * G < ns; ++g) {
                 R->ent_base[(size_t)g0 + g] = off[(size_t)g * G];
                 for (int l = 0; l < G && g * G + l < ns; ++l) {
-                    R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)ord[(size_t)(g * G + l)]], 255);
-                    if (stacked)
-                        R->seg_lo[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(lo[(size_t)ord[(size_t)(g * G + l)]], 255);
+                    const int sg = ord[(size_t)(g * G + l)];
+                    R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)sg], 255);
+                    for (int c = 0; c < NC; ++c)
+                        R->seg_lo[((size_t)(g0 + g) * G + l) * CB + c] = (uint8_t)std::min(lo[(size_t)sg * NC + c], 255);
                 }
             }
             // voxels ascending: each segment's list comes out in voxel order
@@ -463,6 +492,7 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
 
 void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const double *x, double *y) {
     const int G = RowTiles::G;
+    const int NC = R.cut_bytes == 0 ? 0 : (L.TW - 1) / 256;
     const long long items = (long long)L.T * L.S;
     std::vector<double> segv((size_t)L.n_seg, 0.0);
     par_for(items, [&](long long ib, long long ie) {
@@ -475,11 +505,13 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
                     if (len == 0) continue;
                     long long lo = 0;  // the lengths of the lanes before l
                     for (int q = 0; q < l; ++q) lo += R.seg_len[(size_t)g * G + q];
-                    const int nlo = R.seg_lo.empty() ? len : R.seg_lo[(size_t)g * G + l];
                     double acc = 0.0;
                     for (int k = 0; k < len; ++k) {
                         const long long o = R.ent_base[(size_t)g] + lo + k;
-                        acc += R.ent_val[(size_t)o] * x[c0 + R.ent_vox[(size_t)o] + (k >= nlo ? 256 : 0)];
+                        int v = R.ent_vox[(size_t)o];
+                        for (int c = 0; c < NC; ++c)
+                            if (k >= R.seg_lo[((size_t)g * G + l) * R.cut_bytes + c]) v += 256;
+                        acc += R.ent_val[(size_t)o] * x[c0 + v];
                     }
                     segv[(size_t)(L.seg_start[(size_t)it] + (long long)(g - R.grp_base[(size_t)it]) * G + l)] = acc;
                 }

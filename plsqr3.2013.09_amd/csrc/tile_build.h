@@ -106,11 +106,16 @@ struct RowTiles {
     std::vector<int> grp_base;         // items+1: first group of each item
     std::vector<long long> ent_base;   // groups+1: first entry of each group
     std::vector<uint8_t> seg_len;      // groups*G: entries of each lane's segment (0: no segment)
-    // Tiles of 512 voxels (two stacked bricks): the voxel of an entry is
-    // ent_vox + 256 from the segment's seg_lo-th entry on (a segment's
-    // entries ascend in voxel, so its lower-brick entries come first);
-    // empty for tiles of at most 256 voxels.
-    std::vector<uint8_t> seg_lo;       // groups*G
+    // Tiles of more than 256 voxels: a segment's entries ascend in voxel, so
+    // the entries below voxel 256, 512, 768 are prefixes of it, and seg_lo
+    // holds their lengths, cut_bytes per segment: one byte (tiles of at most
+    // 512 voxels: the voxel of entry k is ent_vox + 256 from k = seg_lo on),
+    // or a 32-bit word (at most 1024 voxels) whose byte j is the number of
+    // entries below voxel 256 (j + 1) (entry k is ent_vox + 256 times the
+    // number of bytes <= k; bytes past the tile's width hold 255).  Empty,
+    // cut_bytes 0, for tiles of at most 256 voxels.
+    int cut_bytes = 0;
+    std::vector<uint8_t> seg_lo;       // groups*G*cut_bytes
     std::vector<uint8_t> ent_vox;      // entries: voxel inside the tile (mod 256)
     std::vector<double> ent_val;       // entries
     std::vector<int> ray_ptr;          // m_k+1
@@ -123,9 +128,9 @@ struct RowTiles {
 // Inside an item the segments are taken in windows of `window` (ray order
 // between windows), sorted by descending length inside each window so that a
 // wavefront's 64 segments have similar lengths (window <= 1: ray order).
-// Tiles may hold up to 512 voxels (seg_lo above).  Returns false with *err
+// Tiles may hold up to 1024 voxels (seg_lo above).  Returns false with *err
 // set when a segment holds more than 255 entries, the group count overflows,
-// a tile is wider than 512 or the layout is already compacted (build the
+// a tile is wider than 1024 or the layout is already compacted (build the
 // mode-1 layout first, then compact_tile_layout).
 bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *out, std::string *err, int window = 1);
 
@@ -178,6 +183,17 @@ void sort_rays(const plsqr_system *s, const std::vector<int> &perm, const std::v
 // Fills perm (old -> new) and the tile starts (one tile per brick).
 void brick_order(int nx, int ny, int nz, int bx, int by, int bz, int order,
                  std::vector<int> &perm, std::vector<int> &tile_start);
+
+// Brick order 1 with the bricks grouped into super-bricks of qx * qy * qz
+// bricks: super-bricks z fastest, then x, then y, and inside each its bricks
+// z fastest, then x, then y (qx = qy = qz = 1 is order 1 itself, and
+// 1 x 1 x 2 numbers the columns as order 1 does).  Every super-brick is a
+// run of consecutive bricks, so its columns are a contiguous range: tiles of
+// mode 1 that are unions of mode 2's brick tiles under one column order.
+// Fills perm, the brick starts (tile_start) and the super-brick starts
+// (super_start, a subset of tile_start).
+void brick_order_grouped(int nx, int ny, int nz, int bx, int by, int bz, int qx, int qy, int qz,
+                         std::vector<int> &perm, std::vector<int> &tile_start, std::vector<int> &super_start);
 
 // Builds the layout from the kernel CSC of `s`.  perm/tile_start describe the
 // column renumbering and the tiles (tile widths <= TW); slice_bounds the row

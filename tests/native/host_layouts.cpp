@@ -153,34 +153,73 @@ static void layouts(const plsqr_system *s, const plsqr_grid *grid, int nsl, int 
                 L.n_seg, L.max_seg_per_item, padded, cuts, R.n_grp, padded, L.n_slot);
 }
 
-// Mode 1 on stacks of two bricks: 4 x 8 x 16 bricks number the voxels as
-// two consecutive 4 x 8 x 8 bricks do, so the stacked tiles are unions of
-// the 256-voxel tiles and the same column order serves both; entries keep
-// 8-bit voxels plus a per-segment count of lower-brick entries.
-static void stacked(const plsqr_system *s, const plsqr_grid *grid, int nsl) {
+// Mode 1 on super-bricks of 4 x 8 x 8 bricks: 1 x 1 x 2 of them (4 x 8 x
+// 16, 512 voxels) number the voxels as brick order 1 does, 2 x 1 x 2 (8 x 8
+// x 16, 1024 voxels) in an order of their own; either way every super-brick
+// is a run of whole bricks, so the mode-1 tiles are unions of the 256-voxel
+// tiles and one column order serves both modes.  Entries keep 8-bit voxels
+// plus per-segment counts of the entries below voxel 256, 512, 768.
+static void stacked(const plsqr_system *s, const plsqr_grid *grid, int nsl, int qx, int qz) {
+    const int nx = grid->nx, ny = grid->ny, nz = grid->nz;
     std::vector<int> hb = slice_bounds(s, nsl);
-    std::vector<int> perm, tiles, perm2, tiles2;
-    plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 8, 1, perm, tiles);
-    plsqr::brick_order(grid->nx, grid->ny, grid->nz, 4, 8, 16, 1, perm2, tiles2);
-    CHECK(perm == perm2, "stacked bricks number the columns differently");
+    std::vector<int> perm, tiles, supers;
+    plsqr::brick_order_grouped(nx, ny, nz, 4, 8, 8, qx, 1, qz, perm, tiles, supers);
+    if (qx == 1) {
+        std::vector<int> perm1, tiles1, perm2, tiles2;
+        plsqr::brick_order(nx, ny, nz, 4, 8, 8, 1, perm1, tiles1);
+        plsqr::brick_order(nx, ny, nz, 4, 8, 16, 1, perm2, tiles2);
+        CHECK(perm == perm1 && perm == perm2 && tiles == tiles1 && supers == tiles2,
+              "1 x 1 x 2 super-bricks number the columns differently from brick order 1");
+    }
+    // a permutation; every brick and super-brick a box of the stated extent
+    std::vector<int> inv((size_t)s->n, -1);
+    for (int c = 0; c < s->n; ++c) inv[(size_t)perm[(size_t)c]] = c;
+    CHECK(std::find(inv.begin(), inv.end(), -1) == inv.end(), "grouped brick order is not a permutation");
+    auto box = [&](const std::vector<int> &st, int ex, int ey, int ez, const char *what) {
+        for (size_t t = 0; t + 1 < st.size(); ++t) {
+            int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+            for (int k = st[t]; k < st[t + 1]; ++k) {
+                const int o = inv[(size_t)k], xyz[3] = {o % nx, (o / nx) % ny, o / (nx * ny)};
+                for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], xyz[d]), hi[d] = std::max(hi[d], xyz[d]);
+            }
+            const long long vol = (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+            CHECK(hi[0] - lo[0] < ex && hi[1] - lo[1] < ey && hi[2] - lo[2] < ez && vol == st[t + 1] - st[t],
+                  "%s %zu is not a box of at most %d x %d x %d voxels", what, t, ex, ey, ez);
+        }
+    };
+    box(tiles, 4, 8, 8, "brick");
+    box(supers, 4 * qx, 8, 8 * qz, "super-brick");
     bool subset = true;
-    for (int b : tiles2) subset = subset && std::binary_search(tiles.begin(), tiles.end(), b);
-    CHECK(subset, "a stacked tile is not a union of brick tiles");
+    for (int b : supers) subset = subset && std::binary_search(tiles.begin(), tiles.end(), b);
+    CHECK(subset, "a super-brick is not a union of brick tiles");
     plsqr::order_tile_columns(s, tiles, perm);
     plsqr::RaySorted rays;
     plsqr::sort_rays(s, perm, tiles, hb, grid, &rays);
     plsqr::TileLayout L;
     std::string why;
-    bool ok = plsqr::build_tile_layout(&rays.sys, perm, tiles2, hb, 512, &L, &why);
+    const int TW = 256 * qx * qz;
+    bool ok = plsqr::build_tile_layout(&rays.sys, perm, supers, hb, TW, &L, &why);
     CHECK(ok, "stacked build_tile_layout: %s", why.c_str());
     if (!ok) return;
     plsqr::RowTiles R;
     ok = plsqr::build_row_tiles(L, rays.sys.m_k, &R, &why);
     CHECK(ok, "stacked build_row_tiles: %s", why.c_str());
     if (!ok) return;
-    CHECK(!R.seg_lo.empty(), "no lower-brick counts for stacked tiles");
+    CHECK(R.cut_bytes == (TW > 512 ? 4 : 1) && R.seg_lo.size() == (size_t)R.n_grp * plsqr::RowTiles::G * R.cut_bytes,
+          "per-segment counts of %d bytes for %d-voxel tiles", R.cut_bytes, TW);
+    // the counts: non-decreasing prefixes of each segment, none past its length
+    for (int g = 0; g < R.n_grp; ++g)
+        for (int l = 0; l < plsqr::RowTiles::G; ++l) {
+            const uint8_t *c = &R.seg_lo[((size_t)g * plsqr::RowTiles::G + l) * R.cut_bytes];
+            const int len = R.seg_len[(size_t)g * plsqr::RowTiles::G + l], nc = (TW - 1) / 256;
+            if (len == 0) continue;  // no segment in this lane
+            for (int j = 0; j < nc; ++j)
+                CHECK(c[j] <= len && (j == 0 || c[j - 1] <= c[j]), "segment counts out of order");
+            for (int j = nc; j < R.cut_bytes; ++j) CHECK(c[j] == 255, "unused count byte not 255");
+        }
     CHECK(plsqr::row_tiles_self_check(&rays.sys, L, R, &why), "stacked row_tiles_self_check: %s", why.c_str());
-    std::printf("stacked mode-1 tiles n=%d m_k=%d: %lld segments, %d groups\n", s->n, s->m_k, L.n_seg, R.n_grp);
+    std::printf("mode-1 tiles of %d x 1 x %d bricks n=%d m_k=%d: %lld segments, %d groups\n", qx, qz, s->n, s->m_k,
+                L.n_seg, R.n_grp);
 }
 
 static void files(const plsqr_system *s, const std::string &dir) {
@@ -333,7 +372,11 @@ int main(int argc, char **argv) {
         }
         for (int window : {1, 64}) layouts(&s, c.grid ? &g : nullptr, c.slices, window, true);
         layouts(&s, c.grid ? &g : nullptr, c.slices, 1, false);
-        if (c.grid) stacked(&s, &g, c.slices);
+        if (c.grid) {
+            stacked(&s, &g, c.slices, 1, 2);
+            stacked(&s, &g, c.slices, 2, 2);
+            stacked(&s, &g, c.slices, 2, 1);
+        }
         if (&c == &cases[0]) files(&s, dir);
         plsqr_system_free(&s);
     }

@@ -562,12 +562,17 @@ def test_layout_knobs_are_bitwise_neutral(P, MG, gpu_available, monkeypatch, kno
             assert np.array_equal(a, b), (knob, val)
 
 
+STACKS = ("0", "1", "2", "3")
+
+
 def test_mode1_stacks_match_bricks(P, O, MG, gpu_available, monkeypatch):
-    """Mode 1 on stacks of two bricks (4 x 8 x 16 voxels, 8-bit voxels plus a
-    lower-brick count per segment) or on the bricks (PLSQR_M1_STACK=0): a
-    ray's partials are grouped differently, so the two agree to rounding,
-    both match the oracle in FP64 and FP32, and a whole solve lands within
-    the oracle's own sensitivity of the reference x."""
+    """Mode 1 on super-bricks of the 4 x 8 x 8 bricks (PLSQR_M1_STACK 1: two
+    stacked, 4 x 8 x 16 voxels; 2: four, 8 x 8 x 16; 3: two side by side,
+    8 x 8 x 8; 8-bit voxels plus per-segment counts of the entries below
+    voxel 256, 512, 768) or on the bricks (0): a ray's partials are grouped
+    differently, so all agree to rounding, each matches the oracle in FP64
+    and FP32, and a whole solve lands within the oracle's own sensitivity of
+    the reference x."""
     announce("mode-1 stacks")
     s = MG.make_system(P, MG.CASES["grid16_rays20k_itn100"])
     g = load_golden("grid16_rays20k_itn100")
@@ -578,7 +583,7 @@ def test_mode1_stacks_match_bricks(P, O, MG, gpu_available, monkeypatch):
     yo = y.copy()
     orc.aprod(1, x.copy(), yo)
     out = {}
-    for val in ("0", "1"):
+    for val in STACKS:
         monkeypatch.setenv("PLSQR_M1_STACK", val)
         with P.Operator(s) as op:
             yg = y.copy()
@@ -591,9 +596,41 @@ def test_mode1_stacks_match_bricks(P, O, MG, gpu_available, monkeypatch):
         assert rel(yg, yo) < 1e-13, (val, rel(yg, yo))
         assert rel(y32.astype(np.float64), yo) < 1e-6, (val, rel(y32.astype(np.float64), yo))
         assert im["arnorm"] <= 1e-6
-    assert rel(out["1"][0], out["0"][0]) < 1e-14
+    for val in STACKS[1:]:
+        assert rel(out[val][0], out["0"][0]) < 1e-14, val
     b2 = s.b * (1 + 1e-15 * np.random.default_rng(1).standard_normal(s.b.size))
     xp, _, _ = orc.lsqr(b2, 100, trace=False)
     sens = rel(xp, g["x"])
-    for val in ("0", "1"):
+    for val in STACKS:
         assert rel(out[val][2], g["x"]) <= max(1e-10, 100 * sens), (val, rel(out[val][2], g["x"]), sens)
+
+
+def test_mode1_stacks_on_a_ragged_grid(P, O, gpu_available, monkeypatch):
+    """The same on a 19 x 13 x 21 grid, whose last bricks and super-bricks
+    are cut short in every direction (so a 1024-voxel tile's sub-brick
+    boundaries fall below 256, 512, 768 and a tile narrower than 512 voxels
+    still takes the 1024-voxel kernel when others are wider): modes 1 and 2
+    in FP64 and mode 1 in FP32 against the oracle for each PLSQR_M1_STACK
+    (the grouped brick orders renumber the columns for mode 2 as well)."""
+    announce("mode-1 stacks, ragged grid")
+    s = P.generate(nx=19, ny=13, nz=21, n_rays=12000, geometry=1, seed=5, sigma=0.01, lam=0.1)
+    rng = np.random.default_rng(37)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    orc = O.Oracle(s)
+    yo = y.copy()
+    orc.aprod(1, x.copy(), yo)
+    xo = x.copy()
+    orc.aprod(2, xo, y.copy())
+    for val in STACKS:
+        monkeypatch.setenv("PLSQR_M1_STACK", val)
+        with P.Operator(s) as op:
+            yg = y.copy()
+            op.aprod(1, x.copy(), yg)
+            y32 = y.astype(np.float32)
+            op.aprod_f32(1, x.astype(np.float32), y32)
+            xg = x.copy()
+            op.aprod(2, xg, y.copy())
+        assert rel(yg, yo) < 1e-13, (val, rel(yg, yo))
+        assert rel(y32.astype(np.float64), yo) < 1e-6, (val, rel(y32.astype(np.float64), yo))
+        assert rel(xg, xo) < 1e-13, (val, rel(xg, xo))
