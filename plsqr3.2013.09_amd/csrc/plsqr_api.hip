@@ -90,7 +90,8 @@ struct plsqr_op {
     int rt_tw = 256, rtT = 0;
     int *rt_start = nullptr;
     long long *rt_seg = nullptr;
-    uint8_t *rt_lo = nullptr;  // lower-brick entries per segment (512-voxel tiles)
+    uint8_t *rt_lo = nullptr;  // per-segment counts of entries below voxel 256 (512, 768)
+    uint16_t *rt_dst = nullptr;  // per lane: the partial's ray-order index (length-sorted windows)
     // pass 1 over chunks of at most rt_chunk groups of one item (0: one
     // workgroup per item); chunk c = 8k + s is the k-th of row slice s
     int rt_chunk = 0, rt_nch = 0;
@@ -288,11 +289,11 @@ void launch_pass1(plsqr_op *op, const T *val, const T *x, const Scal *sc, T *seg
     if constexpr (std::is_same<T, double>::value)                                                                     \
         hipLaunchKernelGGL((k_rows_tiled_c<BS, CH64, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,     \
                            op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
-                           op->rt_lo, op->rt_vox, val, x, sc, segv);                                                  \
+                           op->rt_lo, op->rt_dst, op->rt_vox, val, x, sc, segv);                                      \
     else                                                                                                              \
         hipLaunchKernelGGL((k_rows_tiled_c32<BS, CH32, TW>), g, b, 0, op->stream, op->tS, op->rt_start, op->rt_seg,   \
                            op->rt_gbase, op->rt_chitem, op->rt_chg0, op->rt_chunk, op->rt_ebase, op->rt_len,         \
-                           op->rt_lo, op->rt_vox, val, x, sc, segv);
+                           op->rt_lo, op->rt_dst, op->rt_vox, val, x, sc, segv);
     if (op->rt_tw == 1024) {
         PASS1(382, 764, 1024)
     } else if (op->rt_tw == 512) {
@@ -811,6 +812,11 @@ int build(plsqr_op *op, const plsqr_system *s, const std::vector<int> &hb, const
                 if ((rc = dalloc(op, &op->rt_lo, R->seg_lo.size()))) return rc;
                 HIP_TRY(op, hipMemcpy(op->rt_lo, R->seg_lo.data(), R->seg_lo.size(), hipMemcpyHostToDevice));
             }
+            if (!R->seg_dst.empty()) {
+                if ((rc = dalloc(op, &op->rt_dst, R->seg_dst.size()))) return rc;
+                HIP_TRY(op, hipMemcpy(op->rt_dst, R->seg_dst.data(), sizeof(uint16_t) * R->seg_dst.size(),
+                                      hipMemcpyHostToDevice));
+            }
             // chunks of at most rt_chunk groups (PLSQR_M1_CHUNK, 0: whole
             // items), dealt so that chunk 8k + s belongs to row slice s and
             // runs on the XCD of the slice's other work
@@ -1064,10 +1070,12 @@ int plsqr_op_create_ex(const plsqr_system *s, int flags, const plsqr_grid *grid,
         }
         if (L && env_int("PLSQR_ROWTILES", 1) != 0) {
             R.reset(new plsqr::RowTiles());
-            // segments in ray order inside each item; PLSQR_SEGWIN=w sorts them by
-            // length inside windows of w so a wavefront's lanes finish together
-            // (mode 1 0.59 -> 0.69 ms at 67^3/2M rays with w = 256..1024: the
-            // per-ray sums then gather partials from more cache lines)
+            // segments in ray order inside each item; PLSQR_SEGWIN=w deals them
+            // to the lanes sorted by length inside windows of w (the partials
+            // keep ray order): the lanes of a group finish together, but pass 1
+            // is slower at 67^3/2M rays (FP64 0.408 -> 0.472 ms, FP32 0.246 ->
+            // 0.268 ms at w = 1024): neighbouring lanes no longer read the same
+            // x entries, and the groups of long segments take two LDS rounds
             if (L1 && !plsqr::build_row_tiles(*L1, sys->m_k, R.get(), &why, env_int("PLSQR_SEGWIN", 1)))
                 L1.reset();  // a segment too long for the stacks: mode 1 on the bricks
             const plsqr::TileLayout &LR = L1 ? *L1 : *L;

@@ -79,6 +79,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
                                                        const long long *__restrict__ ent_base,
                                                        const uint8_t *__restrict__ seg_len,
                                                        const uint8_t *__restrict__ seg_lo,
+                                                       const uint16_t *__restrict__ seg_dst,
                                                        const uint8_t *__restrict__ ent_vox,
                                                        const float *__restrict__ ent_val, const float *__restrict__ x,
                                                        const Scal *__restrict__ sc, float *__restrict__ segv) {
@@ -116,6 +117,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
     const long long sb = seg_start[it];
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
+        // where the partial goes: its ray-order index in the item
+        const int dst = seg_dst ? (int)seg_dst[(size_t)g * G + lane] : (g - gi0) * G + lane;
         // staged entry e (lane's own, beg <= e < end): its voxel in the tile
         // is its 8-bit voxel + 256 for each of h1, h2, h3 it has reached
         // (the first staged indices at voxel 256, 512, 768)
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c32(int S, const int *__restr
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - gi0) * G + lane] = (float)acc;
+        if (len > 0) segv[sb + dst] = (float)acc;
     }
 }
 

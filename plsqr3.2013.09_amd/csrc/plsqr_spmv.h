@@ -121,7 +121,9 @@ __global__ __launch_bounds__(BS) void k_rows(int m_k, int m, const int *__restri
 // the number of its entries below voxel 256 (512 voxels: one byte of
 // seg_lo; entry k is voxel ent_vox + 256 from k = seg_lo on) or below 256,
 // 512, 768 (1024 voxels: the bytes of a 32-bit word; entry k adds 256 for
-// each count it has reached).
+// each count it has reached).  Inside windows of an item the lanes may take
+// the segments sorted by length (seg_dst non-null): each partial still goes
+// to the segment's ray-order index, seg_dst.
 template <int BS, int CH, int TW = 128>
 __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restrict__ tile_start,
                                                      const long long *__restrict__ seg_start,
@@ -130,6 +132,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
                                                      const long long *__restrict__ ent_base,
                                                      const uint8_t *__restrict__ seg_len,
                                                      const uint8_t *__restrict__ seg_lo,
+                                                     const uint16_t *__restrict__ seg_dst,
                                                      const uint8_t *__restrict__ ent_vox,
                                                      const double *__restrict__ ent_val, const double *__restrict__ x,
                                                      const Scal *__restrict__ sc, double *__restrict__ segv) {
@@ -167,6 +170,8 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
     const long long sb = seg_start[it];
     for (int g = g0 + wv; g < g1; g += NWV) {
         const int len = seg_len[(size_t)g * G + lane];
+        // where the partial goes: its ray-order index in the item
+        const int dst = seg_dst ? (int)seg_dst[(size_t)g * G + lane] : (g - gi0) * G + lane;
         // staged entry e (lane's own, beg <= e < end): its voxel in the tile
         // is its 8-bit voxel + 256 for each of h1, h2, h3 it has reached
         // (the first staged indices at voxel 256, 512, 768)
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_c(int S, const int *__restric
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (len > 0) segv[sb + (long long)(g - gi0) * G + lane] = acc;
+        if (len > 0) segv[sb + dst] = acc;
     }
 }
 

@@ -415,16 +415,18 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     R->n_ent = item_ent[(size_t)items];
     R->ent_vox.assign((size_t)R->n_ent, 0);
     R->ent_val.assign((size_t)R->n_ent, 0.0);
-    // Position of every segment in its item's mode-1 order: the item's
-    // segments (ray order) cut into windows of `window`, each window sorted by
-    // descending length (stable), so that the 64 segments of a wavefront's
-    // group have similar lengths and its lanes finish together.  pos[k] is
-    // the mode-1 position of global segment k; the partial of the segment at
-    // position j of item it is written to seg_start[it] + j.
-    std::vector<int> pos((size_t)L.n_seg, 0);
+    // Lane order of every item's segments: the segments (ray order) cut into
+    // windows of `window`, each window sorted by descending length (stable),
+    // so that the 64 segments of a wavefront's group have similar lengths and
+    // its lanes finish together.  ord[j] is the item-relative ray-order index
+    // of the segment at position j; with sorted windows seg_dst sends its
+    // partial back to that index, so segv stays in ray order either way.
+    const bool sorted = window > 1;
+    if (sorted) R->seg_dst.assign((size_t)gtot * G, 0);
+    else R->seg_dst.clear();
     std::atomic<bool> ok{true};
     par_for(items, [&](long long ib, long long ie) {
-        std::vector<int> len, ord, lo;
+        std::vector<int> len, ord, lo, pit;
         std::vector<long long> off;
         for (long long it = ib; it < ie; ++it) {
             const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
@@ -443,8 +445,8 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
                 for (int w0 = 0; w0 < ns; w0 += window)
                     std::stable_sort(ord.begin() + w0, ord.begin() + std::min(ns, w0 + window),
                                      [&](int a, int b) { return len[(size_t)a] > len[(size_t)b]; });
-            int *pit = pos.data() + L.seg_start[(size_t)it];
-            for (int j = 0; j < ns; ++j) pit[ord[(size_t)j]] = j;
+            pit.resize((size_t)ns);
+            for (int j = 0; j < ns; ++j) pit[(size_t)ord[(size_t)j]] = j;
             off.assign((size_t)ns + 1, item_ent[(size_t)it]);  // by position
             for (int j = 0; j < ns; ++j) {
                 const int ln = len[(size_t)ord[(size_t)j]];
@@ -457,6 +459,7 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
                 for (int l = 0; l < G && g * G + l < ns; ++l) {
                     const int sg = ord[(size_t)(g * G + l)];
                     R->seg_len[(size_t)(g0 + g) * G + l] = (uint8_t)std::min(len[(size_t)sg], 255);
+                    if (sorted) R->seg_dst[(size_t)(g0 + g) * G + l] = (uint16_t)sg;
                     for (int c = 0; c < NC; ++c)
                         R->seg_lo[((size_t)(g0 + g) * G + l) * CB + c] = (uint8_t)std::min(lo[(size_t)sg * NC + c], 255);
                 }
@@ -466,7 +469,7 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
                 for (int j = 0; j < L.vox_len[(size_t)it * TW + v]; ++j) {
                     const long long slot = base + (long long)j * TW + v;
                     const int sg = L.sseg[(size_t)slot];
-                    const long long o = off[(size_t)pit[sg]]++;
+                    const long long o = off[(size_t)pit[(size_t)sg]]++;
                     R->ent_vox[(size_t)o] = (uint8_t)(v & 255);
                     R->ent_val[(size_t)o] = L.sval[(size_t)slot];
                 }
@@ -483,10 +486,8 @@ bool build_row_tiles(const TileLayout &L, int m_k, RowTiles *R, std::string *err
     R->ray_seg.assign((size_t)L.n_seg, 0);
     std::vector<int> next(R->ray_ptr.begin(), R->ray_ptr.end() - 1);
     // per ray, its segments in tile order (items ascending), each named by its
-    // mode-1 partial's index
-    for (long long it = 0; it < items; ++it)
-        for (long long k = L.seg_start[(size_t)it]; k < L.seg_start[(size_t)it + 1]; ++k)
-            R->ray_seg[(size_t)next[(size_t)L.seg_ray[(size_t)k]]++] = (int)(L.seg_start[(size_t)it] + pos[(size_t)k]);
+    // mode-1 partial's index (the segment's own index: segv is in ray order)
+    for (long long k = 0; k < L.n_seg; ++k) R->ray_seg[(size_t)next[(size_t)L.seg_ray[(size_t)k]]++] = (int)k;
     return true;
 }
 
@@ -513,7 +514,9 @@ void row_tiles_apply(const TileLayout &L, const RowTiles &R, int m_k, const doub
                             if (k >= R.seg_lo[((size_t)g * G + l) * R.cut_bytes + c]) v += 256;
                         acc += R.ent_val[(size_t)o] * x[c0 + v];
                     }
-                    segv[(size_t)(L.seg_start[(size_t)it] + (long long)(g - R.grp_base[(size_t)it]) * G + l)] = acc;
+                    const long long dst = R.seg_dst.empty() ? (long long)(g - R.grp_base[(size_t)it]) * G + l
+                                                            : (long long)R.seg_dst[(size_t)g * G + l];
+                    segv[(size_t)(L.seg_start[(size_t)it] + dst)] = acc;
                 }
         }
     });

@@ -116,6 +116,12 @@ struct RowTiles {
     // cut_bytes 0, for tiles of at most 256 voxels.
     int cut_bytes = 0;
     std::vector<uint8_t> seg_lo;       // groups*G*cut_bytes
+    // Segments sorted by length inside windows (build_row_tiles): the
+    // partial of lane l of group g goes to segv[seg_start[item] +
+    // seg_dst[g*G + l]], the segment's index in the item's ray order, so the
+    // partials keep ray order whatever the lanes' order.  Empty: lane order
+    // is ray order, and the partial goes to (g - grp_base[item])*G + l.
+    std::vector<uint16_t> seg_dst;     // groups*G
     std::vector<uint8_t> ent_vox;      // entries: voxel inside the tile (mod 256)
     std::vector<double> ent_val;       // entries
     std::vector<int> ray_ptr;          // m_k+1
@@ -128,6 +134,8 @@ struct RowTiles {
 // Inside an item the segments are taken in windows of `window` (ray order
 // between windows), sorted by descending length inside each window so that a
 // wavefront's 64 segments have similar lengths (window <= 1: ray order).
+// With sorted windows the partials still keep ray order (seg_dst), so each
+// segment's sum and each ray's sum of partials are the same as in ray order.
 // Tiles may hold up to 1024 voxels (seg_lo above).  Returns false with *err
 // set when a segment holds more than 255 entries, the group count overflows,
 // a tile is wider than 1024 or the layout is already compacted (build the

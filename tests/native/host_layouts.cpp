@@ -125,6 +125,32 @@ static void layouts(const plsqr_system *s, const plsqr_grid *grid, int nsl, int 
     CHECK(ok, "build_row_tiles: %s", why.c_str());
     if (!ok) return;
     CHECK(plsqr::row_tiles_self_check(sys, L, R, &why), "row_tiles_self_check: %s", why.c_str());
+    if (window > 1) {
+        // lanes sorted by length inside each window; seg_dst a permutation of
+        // each item's ray-order indices; the same sums, bit for bit, as ray order
+        const int G = plsqr::RowTiles::G;
+        CHECK(R.seg_dst.size() == (size_t)R.n_grp * G, "seg_dst holds %zu lanes", R.seg_dst.size());
+        for (long long it = 0; it < (long long)L.T * L.S; ++it) {
+            const int ns = (int)(L.seg_start[(size_t)it + 1] - L.seg_start[(size_t)it]);
+            std::vector<char> seen((size_t)ns, 0);
+            for (int j = 0; j < ns; ++j) {
+                const size_t lane = (size_t)R.grp_base[(size_t)it] * G + j;
+                const int d = R.seg_dst[lane];
+                CHECK(d < ns && !seen[(size_t)d], "item %lld: seg_dst %d repeated or out of range", it, d);
+                if (d < ns) seen[(size_t)d] = 1;
+                if (j % window != 0)
+                    CHECK(R.seg_len[lane - 1] >= R.seg_len[lane], "item %lld: lengths not descending in a window", it);
+            }
+        }
+        plsqr::RowTiles R1;
+        CHECK(plsqr::build_row_tiles(L, sys->m_k, &R1, &why, 1) && R1.seg_dst.empty(), "ray-order layout");
+        std::vector<double> xv((size_t)s->n), y1((size_t)sys->m_k), yw((size_t)sys->m_k);
+        for (int c = 0; c < s->n; ++c) xv[(size_t)c] = std::sin(0.37 * c + 0.1);
+        plsqr::row_tiles_apply(L, R1, sys->m_k, xv.data(), y1.data());
+        plsqr::row_tiles_apply(L, R, sys->m_k, xv.data(), yw.data());
+        CHECK(y1 == yw, "sorted windows change mode-1 sums");
+        CHECK(R1.ray_seg == R.ray_seg, "sorted windows change the per-ray partial indices");
+    }
     // compacted slot rows: the same sums, fewer slots, and the mode-1 builder refuses them
     const long long padded = L.n_slot;
     std::vector<double> u((size_t)s->m_k), P0((size_t)L.S * s->n), P1((size_t)L.S * s->n);
@@ -370,7 +396,7 @@ int main(int argc, char **argv) {
             plsqr_system_free(&s);
             continue;
         }
-        for (int window : {1, 64}) layouts(&s, c.grid ? &g : nullptr, c.slices, window, true);
+        for (int window : {1, 64, 1024}) layouts(&s, c.grid ? &g : nullptr, c.slices, window, true);
         layouts(&s, c.grid ? &g : nullptr, c.slices, 1, false);
         if (c.grid) {
             stacked(&s, &g, c.slices, 1, 2);

@@ -514,7 +514,7 @@ def test_mixed_warm_start_with_fewer_rows_than_columns(P, O, gpu_available):
             assert rel(got, want) < 1e-13
 
 
-@pytest.mark.parametrize("knob,values,grid", [("PLSQR_SEGWIN", ("1", "64", "512"), "auto"),
+@pytest.mark.parametrize("knob,values,grid", [("PLSQR_SEGWIN", ("1", "64", "512", "1024"), "auto"),
                                                ("PLSQR_DAMP_IL", ("0", "1"), "auto"),
                                                ("PLSQR_COMPACT", ("0", "1", "2"), "auto"),
                                                ("PLSQR_COMPACT", ("0", "1", "2"), None),
