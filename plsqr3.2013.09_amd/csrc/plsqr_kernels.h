@@ -108,6 +108,37 @@ __device__ __forceinline__ void dot32_add(double &acc, float a, float x) { acc =
 // the same with x already widened (an FP32 value held in FP64)
 __device__ __forceinline__ void dot32_add(double &acc, float a, double x) { acc = fma((double)a, x, acc); }
 
+// A damping row's entries b .. e-1 (1 to 3 for the grid stencils), four at a
+// time: the column indices and values, then the x gathers in flight
+// together, added in entry order as acc += a * x (FP64 values) or as
+// dot32_add (FP32 values, the exact product).
+template <class A, class X>
+__device__ __forceinline__ double damp_row_sum(int b, int e, const int *__restrict__ ci, const A *__restrict__ av,
+                                               const X *__restrict__ x) {
+    double acc = 0.0;
+    for (int k0 = b; k0 < e; k0 += 4) {
+        int i[4];
+        A a[4];
+        X g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool on = k0 + j < e;
+            i[j] = on ? ci[k0 + j] : 0;
+            a[j] = on ? av[k0 + j] : A(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] = k0 + j < e ? x[i[j]] : X(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k0 + j < e) {
+                if constexpr (std::is_same<A, double>::value) acc += a[j] * g[j];
+                else dot32_add(acc, a[j], g[j]);
+            }
+        }
+    }
+    return acc;
+}
+
 // Copies the cnt <= CH (value, voxel) entries at global index base into a
 // wavefront's LDS buffers with 16-byte loads (two FP64 or four FP32 values,
 // and the matching voxel bytes per lane and step).  The copy starts at the

@@ -221,8 +221,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine32(int m_k, int m, int
     }
     for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
          r += (long long)gridDim.x * BS) {
-        double acc = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) dot32_add(acc, av[k], x[ci[k]]);
+        const double acc = damp_row_sum(rp[r], rp[r + 1], ci, av, x);
         const float yn = (y[r] * s1) * s2 + (float)acc;
         y[r] = yn;
         ssq_d += (double)yn * (double)yn;
@@ -419,9 +418,7 @@ __global__ __launch_bounds__(BS) void k_cols_combine32(int n, int m_k, int nsl, 
         for (int s = 1; s < nsl; ++s) tk += P[(size_t)s * n + c];
         double td = 0.0;
         if (dlen) {  // damping columns interleaved by groups of 64 (k_cols_combine)
-            const int len = dlen[c];
-            int q = dcp[c];
-            for (int k = 0; k < len; ++k, q += kWave) td += dav[q] * ((double)ud[dri[q]] * ib);
+            td = damp_col_sum_il(dlen[c], dcp[c], dri, dav, ud, ib);
         } else {
             for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * ((double)ud[dri[k]] * ib);
         }

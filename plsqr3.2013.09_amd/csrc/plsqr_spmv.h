@@ -284,8 +284,7 @@ __global__ __launch_bounds__(BS) void k_rows_tiled_combine(int m_k, int m, int S
     }
     for (long long r = (long long)m_k + (long long)blockIdx.x * BS + threadIdx.x; r < m;
          r += (long long)gridDim.x * BS) {
-        double acc = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) acc += av[k] * x[ci[k]];
+        const double acc = damp_row_sum(rp[r], rp[r + 1], ci, av, x);
         double yn = y[r];
         if (scaled) yn = (yn * s1) * s2;
         yn = yn + acc;
@@ -627,14 +626,37 @@ __global__ void k_remap(long long len, const int *__restrict__ map, int *__restr
 // entries at dcp[c], dcp[c] + 64, ...: lane c % 64 of a wavefront reads
 // the address next to lane c % 64 - 1's), entries in the same order either
 // way.
+// Interleaved damping column: its len entries at q, q + 64, ..., four at a
+// time (their indices and values, then the four u gathers in flight
+// together), added in entry order.
+template <class U>
+__device__ __forceinline__ double damp_col_sum_il(int len, int q, const int *__restrict__ dri,
+                                                  const double *__restrict__ dav, const U *__restrict__ ud, double ib) {
+    double td = 0.0;
+    for (int k0 = 0; k0 < len; k0 += 4, q += 4 * kWave) {
+        int i[4];
+        double a[4], g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool on = k0 + j < len;
+            i[j] = on ? dri[q + j * kWave] : 0;
+            a[j] = on ? dav[q + j * kWave] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] = k0 + j < len ? (double)ud[i[j]] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k0 + j < len) td += a[j] * (g[j] * ib);
+    }
+    return td;
+}
+
 __device__ __forceinline__ double damp_col_sum(int c, const int *__restrict__ dcp, const uint8_t *__restrict__ dlen,
                                                const int *__restrict__ dri, const double *__restrict__ dav,
                                                const double *__restrict__ ud, double ib) {
     double td = 0.0;
     if (dlen) {
-        const int len = dlen[c];
-        int q = dcp[c];
-        for (int k = 0; k < len; ++k, q += kWave) td += dav[q] * (ud[dri[q]] * ib);
+        td = damp_col_sum_il(dlen[c], dcp[c], dri, dav, ud, ib);
     } else {
         for (int k = dcp[c]; k < dcp[c + 1]; ++k) td += dav[k] * (ud[dri[k]] * ib);
     }
