@@ -615,17 +615,6 @@ __global__ void k_remap(long long len, const int *__restrict__ map, int *__restr
         a[i] = map[a[i]];
 }
 
-// v[c] <- fl(v[c]*vscale) + (tk + td) with tk the slice partials summed in
-// slice order and td the damping column sum in row order
-// (cpuSpMVTranspose_CSC_v2, CPUSpMV.c:281-291); x += (tk + td) groups as
-// aprod_impl.c:487-489.  part[b] = block b's sum of squares of the new v.
-// With column scales cs (LSQR on A D, D = diag(cs)): the solver keeps
-// v^ = D v, so v^ <- v^*vscale + cs^2 (tk + td) and part holds ||v^ / cs||^2.
-// The damping columns come as a CSC (dlen NULL: column c's entries at
-// dcp[c] .. dcp[c+1]-1) or interleaved by groups of 64 columns (dlen[c]
-// entries at dcp[c], dcp[c] + 64, ...: lane c % 64 of a wavefront reads
-// the address next to lane c % 64 - 1's), entries in the same order either
-// way.
 // Interleaved damping column: its len entries at q, q + 64, ..., four at a
 // time (their indices and values, then the four u gathers in flight
 // together), added in entry order.
@@ -651,6 +640,17 @@ __device__ __forceinline__ double damp_col_sum_il(int len, int q, const int *__r
     return td;
 }
 
+// v[c] <- fl(v[c]*vscale) + (tk + td) with tk the slice partials summed in
+// slice order and td the damping column sum in row order
+// (cpuSpMVTranspose_CSC_v2, CPUSpMV.c:281-291); x += (tk + td) groups as
+// aprod_impl.c:487-489.  part[b] = block b's sum of squares of the new v.
+// With column scales cs (LSQR on A D, D = diag(cs)): the solver keeps
+// v^ = D v, so v^ <- v^*vscale + cs^2 (tk + td) and part holds ||v^ / cs||^2.
+// The damping columns come as a CSC (dlen NULL: column c's entries at
+// dcp[c] .. dcp[c+1]-1) or interleaved by groups of 64 columns (dlen[c]
+// entries at dcp[c], dcp[c] + 64, ...: lane c % 64 of a wavefront reads
+// the address next to lane c % 64 - 1's), entries in the same order either
+// way.
 __device__ __forceinline__ double damp_col_sum(int c, const int *__restrict__ dcp, const uint8_t *__restrict__ dlen,
                                                const int *__restrict__ dri, const double *__restrict__ dav,
                                                const double *__restrict__ ud, double ib) {
