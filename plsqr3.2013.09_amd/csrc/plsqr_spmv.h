@@ -904,8 +904,18 @@ __global__ __launch_bounds__(256) void k_s_init_alpha(Scal *sc, const double *pa
 __global__ __launch_bounds__(256) void k_s_beta(Scal *sc, const double *part, int count) {
     __shared__ double sh[4];
     if (sc->done) return;  // uniform across the block
-    const double sk = reduce_partials<256>(part, count, 2, sh);
-    const double sd = reduce_partials<256>(part + 1, count, 2, sh);
+    // the (kernel rows, damping rows) pairs in one pass of 16-byte loads:
+    // thread t adds pairs t, t + 256, ... of each in that order, as
+    // reduce_partials does for each half, then the same block tree
+    double a = 0.0, d = 0.0;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < count; i += 256) {
+        const double2 p = reinterpret_cast<const double2 *>(part)[i];
+        a += p.x;
+        d += p.y;
+    }
+    const double sk = block_sum<256>(a, sh);
+    const double sd = block_sum<256>(d, sh);
     if (threadIdx.x == 0) s_beta(*sc, sk, sd);
 }
 
