@@ -923,8 +923,8 @@ def main():
     if a.exact_order:
         kname = "k_rows_exact: reference row order" if dom_mode == 1 else "k_cols_exact: reference column order"
     elif dom_mode == 1:
-        kname = (f"k_rows_tiled_c{sfx} + k_rows_tiled_combine{sfx}: y += A x as per-brick ray segments "
-                 "then per-ray sums")
+        kname = (f"k_rows_tiled_c{sfx} + k_rows_tiled_combine{sfx}: y += A x as ray segments per tile of "
+                 "voxels then per-ray sums")
     else:
         kname = f"k_cols_tiled{sfx} + k_cols_combine{sfx}: x += A^T y on voxel bricks"
     roofline = {"bound": "hbm", "achieved": round(dom_b / dom_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
