@@ -2,7 +2,8 @@
 
 The oracle runs single-threaded, so full 100-iteration solves are not
 compared here; instead:
-  * one aprod of each mode against the oracle (1e-12 relative),
+  * one aprod of each mode against the oracle (1e-12 relative), and mode 1
+    on the mode-1 tile shapes other than the default,
   * the adjoint identity <A x, y> = <x, A^T y> on the device operator,
   * bitwise run-to-run reproducibility,
   * the first 3 LSQR iterations against the oracle (1e-10 relative in x),
@@ -43,6 +44,29 @@ def test_big_aprod_vs_oracle(big):
         xg2, yg2 = x.copy(), y.copy()
         op.aprod(mode, xg2, yg2)
         assert np.array_equal(xg2, xg) and np.array_equal(yg2, yg)
+
+
+@pytest.mark.parametrize("stack", ["1", "2"])
+def test_big_mode1_tile_shapes_vs_oracle(big, P, monkeypatch, stack):
+    """Mode 1 at full size on the tile shapes other than the default (two
+    stacked bricks, 4 x 8 x 16 voxels with one count per segment; four
+    bricks, 8 x 8 x 16 with three): FP64 against the oracle as the default
+    operator is held, FP32 within 1e-6."""
+    s, _, orc = big
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(s.n)
+    y = rng.standard_normal(s.m_k + s.m_d)
+    yo = y.copy()
+    orc.aprod(1, x.copy(), yo)
+    monkeypatch.setenv("PLSQR_M1_STACK", stack)
+    print(f"mode 1 on tile shape {stack}", flush=True)
+    with P.Operator(s) as op:
+        yg = y.copy()
+        op.aprod(1, x.copy(), yg)
+        y32 = y.astype(np.float32)
+        op.aprod_f32(1, x.astype(np.float32), y32)
+    assert rel(yg, yo) < 1e-12, (stack, rel(yg, yo))
+    assert rel(y32.astype(np.float64), yo) < 1e-6, (stack, rel(y32.astype(np.float64), yo))
 
 
 def test_big_adjoint_identity(big):
